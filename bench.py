@@ -1,0 +1,209 @@
+"""Benchmark of the MI355X PSDNS RK2 step (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--prec 4]
+
+Workload: forced isotropic DNS, velocity only, n^3 (default 1024^3) FP32,
+RK2 with phase shifts (kshift 1), CFL pi (Courant 0.5 in the reference's
+convention), FEK forcing of shells |k| <= 2 (kf_shell 3), viscosity from
+kmax*eta = 1.5.  Synthetic seeded initial field (no datasets).  A "step" is
+one full RK2 step: 2 x (stage_nonlinear + stage_update) + FEK forcing, all
+state resident in HBM.  Prints one JSON line.
+
+The CPU baseline times the FP64 oracle (a port of the reference algorithm)
+on a bounded sample (RK2 steps at 256^3 for about 10-20 s, all host cores)
+and scales it to n^3 by n^3 log2 n.
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+REPO = pathlib.Path(__file__).resolve().parent
+PKG_DIR = REPO / "nasa-bluewater-psdns-benchmark-minsky_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    spec = importlib.util.spec_from_file_location("psdns_amd", PKG_DIR / "__init__.py",
+                                                  submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["psdns_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pass_bytes(n, s, num_al, max_al_x):
+    """Algorithmic HBM bytes per launch of each pass (external inputs and
+    outputs at storage precision s; complex = 2s):
+      y-inverse   3 vars: read cylinder, write z-pencil columns     2 * 3 * num_al * n * 2s
+      z-inverse   3 vars: read cylinder columns, write full lines   3 * n * max_al_x * (nkept + n) * 2s
+      x fused     3 spectral in, 5 spectral out (n/2 per x line)    8 * n^2 * (n/2) * 2s
+      z-forward   5 lines in, 4 cylinder columns out                (5 * n * max_al_x * n + 4 * num_al * n) * 2s
+      y-fwd+upd   4 vars gathered + un read, u and un written       (4 + 3 + 3 + 3) * num_al * n * 2s
+    (nkept * max_al_x = num_al)."""
+    c = 2 * s
+    return {
+        "y_inverse": 2 * 3 * num_al * n * c,
+        "z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
+        "x_fused": 8 * n * n * (n // 2) * c,
+        "z_forward": (5 * n * max_al_x * n + 4 * num_al * n) * c,
+        "y_forward_update": 13 * num_al * n * c,
+    }
+
+
+def cpu_baseline(n_target, budget_s=15.0):
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as O  # the checker; timed here only as the reported CPU baseline
+
+    O.build()
+    n = 256
+    g = O.Grid(n)
+    u0 = g.field()
+    r = O.Run(g, u0, 0.01, cfl=math.pi, kshift=1, kf_shell=3)
+    r.step()  # first-touch warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        r.step()
+        k += 1
+        if time.perf_counter() - t0 > budget_s or k >= 20:
+            break
+    per = (time.perf_counter() - t0) / k
+    scale = (n_target / n) ** 3 * math.log2(n_target) / math.log2(n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": per * scale,
+        "unit": "s/step",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{k} RK2 steps of the FP64 oracle at {n}^3 ({per:.3f} s/step, {threads} OpenMP threads), "
+                  f"scaled to {n_target}^3 by n^3 log2 n (x{scale:.1f})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--prec", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    P = load_package()
+    n, prec = args.n, args.prec
+    s_bytes = 4 if prec == 4 else 8
+    seed = 122929298
+    print(f"[bench] n={n} prec=fp{8 * s_bytes} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    solver = P.Solver(n, prec, viscos=1.0)
+    solver.init_field(seed=seed, kpeak=3.0, tke=0.5)
+    # viscosity from kmax eta = 1.5: eps = 2 nu D, eta = (nu^3/eps)^(1/4)
+    _, _, tke0, eps_nu1 = solver.shell_stats(0)
+    D = eps_nu1 / 2.0
+    kmax = math.sqrt(2.0) / 3.0 * n
+    viscos = math.sqrt(2.0 * D) * (1.5 / kmax) ** 2
+    solver.viscos = viscos
+    kf_shell = 3
+    ekinf = solver.ek_shell(kf_shell + 1)
+    run = P.Run(solver, viscos, math.pi, kf_shell=kf_shell, ekinf=ekinf)
+    rng = np.random.default_rng(seed)
+    gsh = np.zeros(6)
+    gsh[3:] = rng.random(3)
+
+    def next_gsh():
+        r = rng.random(3)
+        gsh[:3] = np.mod(gsh[3:] + 0.5, 1.0)  # shifts.f kshift = 1
+        gsh[3:] = r
+        return gsh.copy()
+
+    for _ in range(args.warmup):
+        run.step(next_gsh())
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    solver.stage_times(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        run.step(next_gsh())
+    t1 = time.perf_counter()
+    times = solver.stage_times(False)
+    print("[bench] timed steps done", file=sys.stderr, flush=True)
+    sps = (t1 - t0) / args.steps
+    _, _, tke, eps = solver.shell_stats(0)
+
+    names = ["y_inverse", "z_inverse", "x_fused", "z_forward", "y_forward_update"]
+    nb = pass_bytes(n, s_bytes, solver.num_al, solver.max_al_x)
+    launches = 2 * args.steps
+    passes = {}
+    for i, nm in enumerate(names):
+        ms = times[i] / launches
+        passes[nm] = {"ms": round(ms, 4), "GBps": round(nb[nm] / (ms * 1e-3) / 1e9, 1)}
+    fft_bytes = sum(nb.values())
+    fft_ms = sum(times[:5]) / launches
+    dominant = max(names, key=lambda k: times[names.index(k)])
+    dom_ms = times[names.index(dominant)] / launches
+    achieved = nb[dominant] / (dom_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = REPO / "profiles" / f"traffic_{dominant}_{n}_fp{8 * s_bytes}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+
+    line = {
+        "metric": "s/RK2 step & FFT HBM-roofline % at 1024^3 FP32; LES-closure speedup vs DNS",
+        "value": sps,
+        "unit": "s/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sps * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic: seeded random-phase solenoidal field, E(k) ~ k^4 exp(-2(k/3)^2), TKE 0.5",
+        "config": {
+            "workload": f"forced isotropic DNS {n}^3 {'FP32' if prec == 4 else 'FP64'}, velocity only (nc=0), "
+                        "RK2 + phase shifts (kshift 1), FEK forcing kf_shell 3, CFL pi",
+            "n": n, "nc": 0, "viscos": viscos, "cfl": math.pi, "kmax_eta": 1.5,
+            "grid_points_per_s": n ** 3 / sps,
+            "tke": tke, "epsilon": eps, "mean_dt": float(np.mean(run.dts[-args.steps:])),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": nb[dominant],
+            "ms_per_launch": round(dom_ms, 4),
+        },
+        "fft_passes": {
+            "per_stage_ms": round(fft_ms, 3),
+            "alg_GB_per_stage": round(fft_bytes / 1e9, 3),
+            "frac": round(fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "passes": passes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(n)
+    else:
+        line["cpu_baseline"] = None
+    solver.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

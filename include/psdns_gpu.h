@@ -1,0 +1,137 @@
+/*
+ * psdns_gpu.h - C ABI of the MI355X (gfx950) implementation of the PSDNS
+ * RK2 pseudo-spectral step.  Plain C types only, callable from Fortran
+ * through ISO_C_BINDING (see fortran/psdns_gpu_mod.f90 in the package) and
+ * from C/C++/Python (ctypes).
+ *
+ * The reference (deepanjan90/NASA-Bluewater-PSDNS-Benchmark-Minsky) has no
+ * FFI: its hot path is a set of Fortran subroutines with implicit
+ * interfaces working on module-global state, called from rksubstep
+ * (PSDNS/timestep_module.f:152-317).  Each entry point below names the
+ * reference routine it replaces.  Module globals (grid, shifts, integrating
+ * factors, dt2) become arguments or context state.
+ *
+ * Conventions
+ *  - nx = ny = nz = n, a power of two, 32 <= n <= 1024; one GPU plays the
+ *    role of one task with iproc = jproc = 1 (xisz = n/2).
+ *  - Host arrays use the reference layouts (Fortran column-major shown):
+ *      cylinder  uy(ny, num_al, nv)            complex, precision prec
+ *      physical  ux(nx, zisz=nz, yjsz=ny, nv)  real,    precision prec
+ *    Cylinder column a enumerates (x, z) as next_xz (mpisetup.f:710-732).
+ *  - prec = 8 (FP64) or 4 (FP32) for state and transforms; velmax, shell
+ *    sums, dt and integrating factors are always FP64.
+ *  - Every call is synchronous at return, runs on the context's own HIP
+ *    stream, and returns 0 on success or a negative PSDNS_E* code; the
+ *    message of the last failure on the calling thread is returned by
+ *    psdns_last_error().  The Fortran caller maps non-zero to abrt().
+ *  - One host thread per context.  Host pointers are borrowed for the call
+ *    only.  The context owns all device memory.
+ */
+#ifndef PSDNS_GPU_H
+#define PSDNS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSDNS_OK 0
+#define PSDNS_EINVAL (-1)   /* bad argument */
+#define PSDNS_EHIP (-2)     /* HIP runtime error */
+#define PSDNS_ENOMEM (-3)   /* device allocation failed */
+#define PSDNS_ESTATE (-4)   /* call out of order (e.g. update before nonlinear) */
+
+typedef struct psdns_ctx psdns_ctx;
+
+typedef struct {
+  int nx, ny, nz;    /* grid, must be equal */
+  int nc;            /* passive scalars: only 0 is supported */
+  int prec;          /* 4 = FP32, 8 = FP64 */
+  int kshift;        /* informational (shifts come in through psdns_set_shifts) */
+  int mask_kind;     /* 0 = reference sphere |k|/n <= sqrt(2)/3 (masks.f:67-77) */
+  double viscos;     /* kinematic viscosity (input, PSDNS/input) */
+  double pr[8];      /* Prandtl/Schmidt numbers (scalars; unused while nc = 0) */
+  double grad[8][3]; /* mean scalar gradients (unused while nc = 0) */
+} psdns_params;
+
+/* Set-up: mpisetup2 cylinder (mpisetup.f:292-358), masks (masks.f:46-83),
+ * init/waveno wavenumbers (init.f:15-37, waveno.f:16-38), FFT set-up in
+ * place of epfftw (epfftw.f:27-221). */
+int psdns_create(const psdns_params *p, psdns_ctx **out);
+int psdns_destroy(psdns_ctx *c);
+/* num_al and max_al_x of the cylinder (mpisetup.f:311-354) */
+int psdns_geometry(const psdns_ctx *c, int *num_al, int *max_al_x);
+/* column map: col_x[a], col_z[a] 0-based (next_xz order); cut_z[max_al_x] */
+int psdns_columns(const psdns_ctx *c, int *col_x, int *col_z, int *cut_z);
+
+/* State transfer (host <-> device), layout 0 = reference cylinder
+ * uny(ny, num_al, 3).  put_state also sets u = un (main.f:357-359). */
+int psdns_put_state(psdns_ctx *c, const void *uny, int layout);
+int psdns_get_state(psdns_ctx *c, void *uny, int layout);
+
+/* Phase-shift offsets gsh(3,2) in grid cells, as produced by shifts.f
+ * (shifts.f:65-84); the factors sx, sy, sz (shifts.f:146-178) are formed
+ * in FP64 inside. */
+int psdns_set_shifts(psdns_ctx *c, const double gsh[6]);
+
+/* One RK2 stage, split where the reference calls step (timestep_module.f:261):
+ * stage_nonlinear = phase1 + itransform_vel + realspace_vel + transform up to
+ * the convective combination (timestep_module.f:152-244); on kstep 1 it
+ * returns velmax (FP64) for the caller's dt = cfl/velmax (step.f:64-78).
+ * stage_update = the y-forward part of transform + wavespace_vel + proc4a
+ * (kstep 1) or proc4b (kstep 2) (timestep_module.f:244-317).  xfac(n/2),
+ * yfac(n), zfac(n) are the integrating factors of step.f:366-385; forcing
+ * must be NULL (EPFOR forcing increments are not supported; FEK forcing is
+ * psdns_ek_force). */
+int psdns_stage_nonlinear(psdns_ctx *c, int kstep, double *velmax);
+int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
+                       const double *yfac, const double *zfac, const void *forcing);
+
+/* FEK forcing (ek_force.f:1-76 with ek_shell.f:81-176): shell energies of
+ * un for ik = int(|k| + 1.5) <= kfor, then u and un scaled by
+ * sqrt(ekinf(ik)/ek(ik)) for ik <= kf_shell (ratio 0 for ik = 1).
+ * ek_out (nullable) receives the shell energies after the rescale. */
+int psdns_ek_shell(psdns_ctx *c, int kfor, double *ek);
+int psdns_ek_force(psdns_ctx *c, int kf_shell, int kfor, const double *ekinf, double *ek_out);
+
+/* FP64 shell statistics of un (homogeneous/sptr.f:139-176,235):
+ * ek[ik-1] = sum tfact |u|^2/2, dk[ik-1] = 2 viscos k^2 ek, ik = int(|k|+1.5),
+ * nshell shells; tke and eps over all modes. */
+int psdns_shell_stats(psdns_ctx *c, double *ek, double *dk, int nshell, double *tke, double *eps);
+
+/* Random-phase solenoidal initial field on the device (homogeneous/field.f:
+ * 172-257 construction, consym.f symmetry), E(k) ~ k^4 exp(-2 (k/kpeak)^2),
+ * rescaled to the given TKE; sets un and u. */
+int psdns_init_field(psdns_ctx *c, unsigned long long seed, double kpeak, double tke);
+
+/* Per-routine entry points, each the device counterpart of one reference
+ * subroutine, on the context's work arrays:
+ *   slot "u"  = uy(ny, num_al, 5) cylinder   (PSDNS_FIELD_U)
+ *   slot "un" = uny(ny, num_al, 3) cylinder  (PSDNS_FIELD_UN)
+ *   slot "ux" = ux(nx, nz, ny, 5) physical   (PSDNS_FIELD_UX) */
+#define PSDNS_FIELD_U 0
+#define PSDNS_FIELD_UN 1
+#define PSDNS_FIELD_UX 2
+int psdns_put_field(psdns_ctx *c, int which, int nv, const void *host);
+int psdns_get_field(psdns_ctx *c, int which, int nv, void *host);
+
+int psdns_phase1(psdns_ctx *c, int kstep);                 /* homogeneous/phase1.f:136-216 */
+int psdns_itransform_vel(psdns_ctx *c);                    /* itransform_vel.f:11-129: u -> ux(1:3) */
+int psdns_realspace_vel(psdns_ctx *c, int rkstep, double *velmax); /* realspace_vel.f:7-187 */
+int psdns_transform(psdns_ctx *c);                         /* transform.f:10-301: ux(1:5) -> u(1:4) */
+int psdns_wavespace_vel(psdns_ctx *c, int kstep, double dt2); /* wavespace_vel.f:11-235 */
+int psdns_proc4a(psdns_ctx *c, const double *xfac, const double *yfac, const double *zfac); /* proc4a.f */
+int psdns_proc4b(psdns_ctx *c);                            /* homogeneous/proc4b.f:3-208 */
+int psdns_kxtran(psdns_ctx *c, int nv);                    /* kxtran.f:17-93: u(1:nv) -> ux(1:nv) */
+int psdns_xktran(psdns_ctx *c, int nv);                    /* xktran.f:1-88: ux(1:nv) -> u(1:nv) */
+
+/* Device time of the last stage calls, per pass (ms), the reference's
+ * t_itrans/t_trans split (timestep_module.f:98-437); times[0..5] =
+ * y-inverse, z-inverse, x-fused, z-forward, y-forward+update, total. */
+int psdns_stage_times(psdns_ctx *c, int enable, double *times);
+
+const char *psdns_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

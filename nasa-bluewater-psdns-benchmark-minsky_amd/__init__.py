@@ -1,0 +1,281 @@
+"""MI355X (gfx950) implementation of the PSDNS RK2 pseudo-spectral step.
+
+The product is the C-ABI library lib/libpsdns_gpu.so (include/psdns_gpu.h)
+and the Fortran ISO_C_BINDING module + driver (fortran/).  This Python module
+is a thin ctypes binding to the same C ABI, used by the tests and bench.py;
+it never computes anything itself, and it raises if the library is missing.
+
+Host arrays follow the reference layouts (C order, fastest index last):
+    cylinder uy(ny, num_al, nv) -> numpy (nv, num_al, n) complex
+    physical ux(nx, nz, ny, nv) -> numpy (nv, n, n, n)   real, [v][y][z][x]
+"""
+import ctypes
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libpsdns_gpu.so"
+DRIVER_PATH = PKG / "bin" / "psdns_driver"
+
+FIELD_U, FIELD_UN, FIELD_UX = 0, 1, 2
+
+# exported C-ABI symbols (include/psdns_gpu.h)
+EXPORTS = (
+    "psdns_create", "psdns_destroy", "psdns_geometry", "psdns_columns", "psdns_put_state",
+    "psdns_get_state", "psdns_set_shifts", "psdns_stage_nonlinear", "psdns_stage_update",
+    "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_init_field", "psdns_put_field",
+    "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
+    "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
+    "psdns_stage_times", "psdns_last_error",
+)
+
+
+class PsdnsError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("nc", ctypes.c_int),
+        ("prec", ctypes.c_int), ("kshift", ctypes.c_int), ("mask_kind", ctypes.c_int),
+        ("viscos", ctypes.c_double), ("pr", ctypes.c_double * 8), ("grad", (ctypes.c_double * 3) * 8),
+    ]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def build(verbose=False):
+    """Compile lib/libpsdns_gpu.so and the Fortran driver (build.py)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_psdns_build", PKG / "build.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.build(verbose)
+
+
+def load_library():
+    """Load lib/libpsdns_gpu.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise PsdnsError(f"{LIB_PATH} is missing: run build() (hipcc --offload-arch=gfx950) first")
+    L = ctypes.CDLL(str(LIB_PATH))
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "psdns_create": [ctypes.POINTER(Params), ctypes.POINTER(_vp)],
+        "psdns_destroy": [_vp],
+        "psdns_geometry": [_vp, _ip, _ip],
+        "psdns_columns": [_vp, _ip, _ip, _ip],
+        "psdns_put_state": [_vp, _vp, i],
+        "psdns_get_state": [_vp, _vp, i],
+        "psdns_set_shifts": [_vp, _dp],
+        "psdns_stage_nonlinear": [_vp, i, _dp],
+        "psdns_stage_update": [_vp, i, d, _dp, _dp, _dp, _vp],
+        "psdns_ek_shell": [_vp, i, _dp],
+        "psdns_ek_force": [_vp, i, i, _dp, _dp],
+        "psdns_shell_stats": [_vp, _dp, _dp, i, _dp, _dp],
+        "psdns_init_field": [_vp, ctypes.c_ulonglong, d, d],
+        "psdns_put_field": [_vp, i, i, _vp],
+        "psdns_get_field": [_vp, i, i, _vp],
+        "psdns_phase1": [_vp, i],
+        "psdns_itransform_vel": [_vp],
+        "psdns_realspace_vel": [_vp, i, _dp],
+        "psdns_transform": [_vp],
+        "psdns_wavespace_vel": [_vp, i, d],
+        "psdns_proc4a": [_vp, _dp, _dp, _dp],
+        "psdns_proc4b": [_vp],
+        "psdns_kxtran": [_vp, i],
+        "psdns_xktran": [_vp, i],
+        "psdns_stage_times": [_vp, i, _dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.psdns_last_error.restype = ctypes.c_char_p
+    L.psdns_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Solver:
+    """One device context (psdns_ctx) for an n^3 box at precision prec (4|8)."""
+
+    def __init__(self, n, prec=8, viscos=0.01):
+        self.L = load_library()
+        self.n, self.prec, self.viscos = n, prec, viscos
+        p = Params(nx=n, ny=n, nz=n, nc=0, prec=prec, kshift=1, mask_kind=0, viscos=viscos)
+        h = _vp()
+        self._chk(self.L.psdns_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        na, mx = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.L.psdns_geometry(self.h, ctypes.byref(na), ctypes.byref(mx)))
+        self.num_al, self.max_al_x = na.value, mx.value
+        self.cdtype = np.complex128 if prec == 8 else np.complex64
+        self.rdtype = np.float64 if prec == 8 else np.float32
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PsdnsError(f"psdns error {rc}: {self.L.psdns_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.psdns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def columns(self):
+        cx = np.zeros(self.num_al, np.int32)
+        cz = np.zeros(self.num_al, np.int32)
+        cut = np.zeros(self.max_al_x, np.int32)
+        self._chk(self.L.psdns_columns(self.h, cx.ctypes.data_as(_ip), cz.ctypes.data_as(_ip), cut.ctypes.data_as(_ip)))
+        return cx, cz, cut
+
+    # state and fields
+    def put_state(self, uny):
+        a = np.ascontiguousarray(uny[:3], self.cdtype)
+        self._chk(self.L.psdns_put_state(self.h, a.ctypes.data, 0))
+
+    def get_state(self):
+        a = np.zeros((3, self.num_al, self.n), self.cdtype)
+        self._chk(self.L.psdns_get_state(self.h, a.ctypes.data, 0))
+        return a
+
+    def put_field(self, which, arr):
+        dt = self.rdtype if which == FIELD_UX else self.cdtype
+        a = np.ascontiguousarray(arr, dt)
+        self._chk(self.L.psdns_put_field(self.h, which, a.shape[0], a.ctypes.data))
+
+    def get_field(self, which, nv):
+        if which == FIELD_UX:
+            a = np.zeros((nv, self.n, self.n, self.n), self.rdtype)
+        else:
+            a = np.zeros((nv, self.num_al, self.n), self.cdtype)
+        self._chk(self.L.psdns_get_field(self.h, which, nv, a.ctypes.data))
+        return a
+
+    def set_shifts(self, gsh):
+        g = np.ascontiguousarray(np.asarray(gsh, np.float64).reshape(6))
+        self._chk(self.L.psdns_set_shifts(self.h, _dptr(g)))
+
+    def init_field(self, seed=122929298, kpeak=3.0, tke=0.5):
+        self._chk(self.L.psdns_init_field(self.h, seed, kpeak, tke))
+
+    # fused stage entry points
+    def stage_nonlinear(self, kstep):
+        vm = np.zeros(1)
+        self._chk(self.L.psdns_stage_nonlinear(self.h, kstep, _dptr(vm)))
+        return float(vm[0])
+
+    def stage_update(self, kstep, dt, xfac=None, yfac=None, zfac=None):
+        if xfac is None:
+            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, None, None, None, None))
+        else:
+            xf, yf, zf = (np.ascontiguousarray(f, np.float64) for f in (xfac, yfac, zfac))
+            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, _dptr(xf), _dptr(yf), _dptr(zf), None))
+
+    def step_factors(self, dt):
+        """step.f:366-385 integrating factors (host side, as the kept driver)."""
+        n = self.n
+        k = np.arange(n)
+        kw = np.where(k > n // 2, k - n, k).astype(np.float64)
+        kx = np.arange(n // 2, dtype=np.float64)
+        return (np.exp(-self.viscos * kx * kx * dt), np.exp(-self.viscos * kw * kw * dt),
+                np.exp(-self.viscos * kw * kw * dt))
+
+    def ek_shell(self, kfor):
+        ek = np.zeros(kfor)
+        self._chk(self.L.psdns_ek_shell(self.h, kfor, _dptr(ek)))
+        return ek
+
+    def ek_force(self, kf_shell, kfor, ekinf):
+        e = np.ascontiguousarray(ekinf, np.float64)
+        out = np.zeros(kfor)
+        self._chk(self.L.psdns_ek_force(self.h, kf_shell, kfor, _dptr(e), _dptr(out)))
+        return out
+
+    def shell_stats(self, nshell=None):
+        nshell = self.n if nshell is None else nshell
+        ek, dk = np.zeros(max(nshell, 1)), np.zeros(max(nshell, 1))
+        t, e = np.zeros(1), np.zeros(1)
+        self._chk(self.L.psdns_shell_stats(self.h, _dptr(ek), _dptr(dk), nshell, _dptr(t), _dptr(e)))
+        return ek[:nshell], dk[:nshell], float(t[0]), float(e[0])
+
+    def stage_times(self, enable=True):
+        t = np.zeros(6)
+        self._chk(self.L.psdns_stage_times(self.h, int(enable), _dptr(t)))
+        return t
+
+    # per-routine entry points
+    def phase1(self, kstep):
+        self._chk(self.L.psdns_phase1(self.h, kstep))
+
+    def itransform_vel(self):
+        self._chk(self.L.psdns_itransform_vel(self.h))
+
+    def realspace_vel(self, rkstep):
+        vm = np.zeros(1)
+        self._chk(self.L.psdns_realspace_vel(self.h, rkstep, _dptr(vm)))
+        return float(vm[0])
+
+    def transform(self):
+        self._chk(self.L.psdns_transform(self.h))
+
+    def wavespace_vel(self, kstep, dt2):
+        self._chk(self.L.psdns_wavespace_vel(self.h, kstep, dt2))
+
+    def proc4a(self, xfac, yfac, zfac):
+        xf, yf, zf = (np.ascontiguousarray(f, np.float64) for f in (xfac, yfac, zfac))
+        self._chk(self.L.psdns_proc4a(self.h, _dptr(xf), _dptr(yf), _dptr(zf)))
+
+    def proc4b(self):
+        self._chk(self.L.psdns_proc4b(self.h))
+
+    def kxtran(self, nv):
+        self._chk(self.L.psdns_kxtran(self.h, nv))
+
+    def xktran(self, nv):
+        self._chk(self.L.psdns_xktran(self.h, nv))
+
+
+class Run:
+    """RK2 time loop over the fused entry points, as the kept driver runs it
+    (main.f:392-685 / timestep_module.f:25-481): shifts, then per stage
+    stage_nonlinear -> (stage 1: dt = cfl/velmax, integrating factors) ->
+    stage_update, then FEK forcing.  Shift offsets come from the caller."""
+
+    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None):
+        self.s, self.viscos, self.cfl = solver, viscos, cfl
+        self.kf_shell = kf_shell
+        self.kfor = kf_shell + 1 if kf_shell > 0 else 0
+        self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
+        self.dts = []
+
+    def step(self, gsh):
+        s = self.s
+        s.set_shifts(gsh)
+        vm = s.stage_nonlinear(1)
+        self.velmax = vm
+        self.dt = self.cfl / vm
+        xf, yf, zf = s.step_factors(self.dt)
+        s.stage_update(1, self.dt, xf, yf, zf)
+        s.stage_nonlinear(2)
+        s.stage_update(2, self.dt, xf, yf, zf)
+        if self.kfor:
+            s.ek_force(self.kf_shell, self.kfor, self.ekinf)
+        self.dts.append(self.dt)

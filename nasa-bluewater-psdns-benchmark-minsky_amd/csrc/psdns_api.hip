@@ -1,0 +1,622 @@
+// C ABI of the gfx950 PSDNS RK2 step (include/psdns_gpu.h): context set-up
+// (cylinder, twiddles, shift tables), state transfer, stage sequencing and
+// the per-routine entry points.  All device work runs on the context's
+// stream; every entry point synchronises before it returns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/psdns_gpu.h"
+#include "psdns_internal.h"
+
+using psdns::Ctx;
+using psdns::Pass;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(PSDNS_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+inline int kwave(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+}  // namespace
+
+struct psdns_ctx {
+  Ctx c;
+  int n = 0, nxh = 0, na = 0, max_al_x = 0, prec = 8;
+  double viscos = 0.0;
+  std::vector<int> colx, colz, cutz;
+  // device allocations
+  int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr;
+  void *d_u = nullptr, *d_un = nullptr, *d_w = nullptr, *d_ux = nullptr, *d_tw = nullptr;
+  double2* d_shift = nullptr;  // sx[2][nxh], sy[2][n], sz[2][n]
+  double* d_fac = nullptr;     // xfac[nxh], yfac[n], zfac[n]
+  double* d_red = nullptr;     // reductions scratch: [2 + 2*n + n]
+  unsigned long long* d_velmax = nullptr;
+  int2* d_force = nullptr;
+  int force_kfor = -1, nforce = 0;
+  int last_nonlinear = 0;
+  // pass timers
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  double times[6] = {0, 0, 0, 0, 0, 0};
+
+  size_t csize() const { return prec == 8 ? 16 : 8; }  // bytes per complex
+  size_t rsize() const { return prec == 8 ? 8 : 4; }
+  size_t cyl_elems() const { return (size_t)na * n; }
+};
+
+namespace {
+
+int check_ctx(const psdns_ctx* c) {
+  if (!c) return fail(PSDNS_EINVAL, "null context");
+  return 0;
+}
+
+int ensure_ux(psdns_ctx* c) {
+  if (c->d_ux) return 0;
+  size_t bytes = (size_t)5 * c->n * c->n * c->n * c->rsize();
+  if (hipMalloc(&c->d_ux, bytes) != hipSuccess)
+    return fail(PSDNS_ENOMEM, "hipMalloc of the physical array (%zu bytes) failed", bytes);
+  HIPCHK(hipMemsetAsync(c->d_ux, 0, bytes, c->c.stream));
+  c->c.ux = c->d_ux;
+  return 0;
+}
+
+int sync(psdns_ctx* c) {
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->c.stream));
+  return 0;
+}
+
+// (a, y) entries with |k| < kfor + 0.5 on the mask: the FEK forcing region
+int ensure_force_list(psdns_ctx* c, int kfor) {
+  if (c->force_kfor == kfor) return 0;
+  std::vector<int2> list;
+  const int n = c->n;
+  for (int a = 0; a < c->na; a++) {
+    const int x = c->colx[a], z = c->colz[a], kz = kwave(z, n);
+    if (x > kfor || std::abs(kz) > kfor) continue;
+    for (int y = 0; y < n; y++) {
+      const int ky = kwave(y, n);
+      if (std::abs(ky) > kfor) continue;
+      if (9 * (x * x + ky * ky + kz * kz) > 2 * n * n || y == n / 2 || z == n / 2) continue;
+      const int ik = (int)(std::sqrt((double)x * x + (double)ky * ky + (double)kz * kz) + 1.5);
+      if (ik > kfor) continue;
+      list.push_back(make_int2(a, y));
+    }
+  }
+  if (c->d_force) HIPCHK(hipFree(c->d_force));
+  c->d_force = nullptr;
+  c->nforce = (int)list.size();
+  if (!list.empty()) {
+    HIPCHK(hipMalloc(&c->d_force, sizeof(int2) * list.size()));
+    HIPCHK(hipMemcpy(c->d_force, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice));
+  }
+  c->c.force_list = c->d_force;
+  c->c.nforce = c->nforce;
+  c->force_kfor = kfor;
+  return 0;
+}
+
+int upload_factors(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
+  if (!xfac || !yfac || !zfac) return fail(PSDNS_EINVAL, "null integrating-factor array");
+  HIPCHK(hipMemcpyAsync(c->d_fac, xfac, sizeof(double) * c->nxh, hipMemcpyHostToDevice, c->c.stream));
+  HIPCHK(hipMemcpyAsync(c->d_fac + c->nxh, yfac, sizeof(double) * c->n, hipMemcpyHostToDevice, c->c.stream));
+  HIPCHK(hipMemcpyAsync(c->d_fac + c->nxh + c->n, zfac, sizeof(double) * c->n, hipMemcpyHostToDevice,
+                        c->c.stream));
+  return 0;
+}
+
+int read_velmax(psdns_ctx* c, double* velmax) {
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, c->d_velmax, sizeof bits, hipMemcpyDeviceToHost, c->c.stream));
+  HIPCHK(hipStreamSynchronize(c->c.stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof v);
+  if (velmax) *velmax = v;
+  return 0;
+}
+
+int pass(psdns_ctx* c, Pass p, int a0, int a1 = 0) {
+  if (psdns::launch_pass(c->c, p, a0, a1) != 0) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int mark(psdns_ctx* c, int i) {
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[i], c->c.stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* psdns_last_error(void) { return g_err.c_str(); }
+
+int psdns_create(const psdns_params* p, psdns_ctx** out) {
+  if (!p || !out) return fail(PSDNS_EINVAL, "null argument");
+  *out = nullptr;
+  const int n = p->nx;
+  if (p->ny != n || p->nz != n) return fail(PSDNS_EINVAL, "nx, ny, nz must be equal (got %d %d %d)", p->nx, p->ny, p->nz);
+  if (n < 32 || n > 1024 || (n & (n - 1))) return fail(PSDNS_EINVAL, "n = %d: need a power of two in [32, 1024]", n);
+  if (p->nc != 0) return fail(PSDNS_EINVAL, "nc = %d: passive scalars are not supported", p->nc);
+  if (p->prec != 4 && p->prec != 8) return fail(PSDNS_EINVAL, "prec = %d: need 4 or 8", p->prec);
+  if (p->mask_kind != 0) return fail(PSDNS_EINVAL, "mask_kind = %d: only the reference sphere (0)", p->mask_kind);
+
+  psdns_ctx* c = new psdns_ctx();
+  c->n = n;
+  c->nxh = n / 2;
+  c->prec = p->prec;
+  c->viscos = p->viscos;
+  // mpisetup.f:311-340
+  const int ikmax = (int)(n * std::sqrt(2.0) / 3.0);
+  c->max_al_x = std::min(ikmax + 1, c->nxh);
+  c->cutz.resize(c->max_al_x);
+  const double rnxi = 1.0 / n;
+  for (int x = 0; x < c->max_al_x; x++) {
+    const double kxr = (double)x * rnxi;
+    const double rc = n * (1.0 - 2.0 * std::sqrt(2.0 / 9.0 - kxr * kxr));
+    c->cutz[x] = (int)(rc / 2);
+  }
+  for (int x = 0; x < c->max_al_x; x++) {
+    const int cz = c->cutz[x];
+    for (int z = 0; z < n / 2 - cz; z++) { c->colx.push_back(x); c->colz.push_back(z); }
+    for (int z = n / 2 + cz + 1; z < n; z++) { c->colx.push_back(x); c->colz.push_back(z); }
+  }
+  c->na = (int)c->colx.size();
+  std::vector<int> colz0(n, -1);
+  for (int a = 0; a < c->na && c->colx[a] == 0; a++) colz0[c->colz[a]] = a;
+
+  auto bail = [&](int rc) {
+    psdns_destroy(c);
+    return rc;
+  };
+#define CHK(expr)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return bail(fail(PSDNS_EHIP, "%s: %s", #expr, hipGetErrorString(e_))); \
+  } while (0)
+#define ALLOC(ptr, bytes)                                                                 \
+  do {                                                                                    \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)                                 \
+      return bail(fail(PSDNS_ENOMEM, "hipMalloc(%zu) failed for " #ptr, (size_t)(bytes))); \
+  } while (0)
+
+  CHK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+  const size_t cyl = c->cyl_elems();
+  ALLOC(c->d_colx, sizeof(int) * c->na);
+  ALLOC(c->d_colz, sizeof(int) * c->na);
+  ALLOC(c->d_cutz, sizeof(int) * c->max_al_x);
+  ALLOC(c->d_colz0, sizeof(int) * n);
+  ALLOC(c->d_u, c->csize() * 5 * cyl);
+  ALLOC(c->d_un, c->csize() * 3 * cyl);
+  ALLOC(c->d_w, c->csize() * 5 * (size_t)n * c->nxh * n);
+  ALLOC(c->d_tw, c->csize() * n);
+  ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
+  ALLOC(c->d_fac, sizeof(double) * (c->nxh + 2 * n));
+  ALLOC(c->d_red, sizeof(double) * (2 + 3 * (size_t)n + 64));
+  ALLOC(c->d_velmax, sizeof(unsigned long long));
+  CHK(hipMemcpy(c->d_colx, c->colx.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->d_colz, c->colz.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->d_cutz, c->cutz.data(), sizeof(int) * c->max_al_x, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->d_colz0, colz0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  CHK(hipMemsetAsync(c->d_u, 0, c->csize() * 5 * cyl, c->c.stream));
+  CHK(hipMemsetAsync(c->d_un, 0, c->csize() * 3 * cyl, c->c.stream));
+  CHK(hipMemsetAsync(c->d_w, 0, c->csize() * 5 * (size_t)n * c->nxh * n, c->c.stream));
+  // twiddles exp(-2 pi i k / n) formed in FP64
+  {
+    std::vector<double> twd(2 * n);
+    for (int k = 0; k < n; k++) {
+      const double ang = 2.0 * M_PI * k / n;
+      twd[2 * k] = std::cos(ang);
+      twd[2 * k + 1] = -std::sin(ang);
+    }
+    if (c->prec == 8) {
+      CHK(hipMemcpy(c->d_tw, twd.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> twf(twd.begin(), twd.end());
+      CHK(hipMemcpy(c->d_tw, twf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    }
+  }
+  Ctx& k = c->c;
+  k.prec = c->prec;
+  k.geo = psdns::Geo{n, c->nxh, c->na, c->max_al_x, c->d_colx, c->d_colz, c->d_cutz};
+  k.u = c->d_u;
+  k.un = c->d_un;
+  k.w = c->d_w;
+  k.ux = nullptr;
+  k.tw = c->d_tw;
+  k.kstep = 0;
+  for (int s = 0; s < 2; s++) {
+    k.sx[s] = c->d_shift + s * c->nxh;
+    k.sy[s] = c->d_shift + 2 * c->nxh + s * n;
+    k.sz[s] = c->d_shift + 2 * c->nxh + 2 * n + s * n;
+  }
+  k.xfac = c->d_fac;
+  k.yfac = c->d_fac + c->nxh;
+  k.zfac = c->d_fac + c->nxh + n;
+  k.velmax_bits = c->d_velmax;
+  k.force_list = nullptr;
+  k.nforce = 0;
+  for (auto& e : c->ev) CHK(hipEventCreate(&e));
+  CHK(hipStreamSynchronize(k.stream));
+#undef CHK
+#undef ALLOC
+  const double zero[6] = {0, 0, 0, 0, 0, 0};
+  int rc = psdns_set_shifts(c, zero);
+  if (rc) return bail(rc);
+  *out = c;
+  return PSDNS_OK;
+}
+
+int psdns_destroy(psdns_ctx* c) {
+  if (!c) return PSDNS_OK;
+  if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
+  void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_colz0, c->d_u, c->d_un, c->d_w, c->d_ux,
+                  c->d_tw, c->d_shift, c->d_fac, c->d_red, c->d_velmax, c->d_force};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->c.stream) (void)hipStreamDestroy(c->c.stream);
+  delete c;
+  return PSDNS_OK;
+}
+
+int psdns_geometry(const psdns_ctx* c, int* num_al, int* max_al_x) {
+  if (int rc = check_ctx(c)) return rc;
+  if (num_al) *num_al = c->na;
+  if (max_al_x) *max_al_x = c->max_al_x;
+  return PSDNS_OK;
+}
+
+int psdns_columns(const psdns_ctx* c, int* col_x, int* col_z, int* cut_z) {
+  if (int rc = check_ctx(c)) return rc;
+  if (col_x) std::memcpy(col_x, c->colx.data(), sizeof(int) * c->na);
+  if (col_z) std::memcpy(col_z, c->colz.data(), sizeof(int) * c->na);
+  if (cut_z) std::memcpy(cut_z, c->cutz.data(), sizeof(int) * c->max_al_x);
+  return PSDNS_OK;
+}
+
+int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!gsh) return fail(PSDNS_EINVAL, "null gsh");
+  const int n = c->n, nxh = c->nxh;
+  std::vector<double2> h(2 * (nxh + 2 * n));
+  // shifts.f:146-178: exp(i 2 pi k gsh(dir, stage) / n)
+  for (int s = 0; s < 2; s++) {
+    for (int x = 0; x < nxh; x++) {
+      const double arg = 2.0 * M_PI / n * x * gsh[3 * s + 0];
+      h[s * nxh + x] = make_double2(std::cos(arg), std::sin(arg));
+    }
+    for (int y = 0; y < n; y++) {
+      const double arg = 2.0 * M_PI / n * kwave(y, n) * gsh[3 * s + 1];
+      h[2 * nxh + s * n + y] = make_double2(std::cos(arg), std::sin(arg));
+    }
+    for (int z = 0; z < n; z++) {
+      const double arg = 2.0 * M_PI / n * kwave(z, n) * gsh[3 * s + 2];
+      h[2 * nxh + 2 * n + s * n + z] = make_double2(std::cos(arg), std::sin(arg));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->d_shift, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->c.stream));
+  return sync(c);
+}
+
+int psdns_put_state(psdns_ctx* c, const void* uny, int layout) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!uny) return fail(PSDNS_EINVAL, "null state");
+  if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
+  const size_t bytes = c->csize() * 3 * c->cyl_elems();
+  HIPCHK(hipMemcpyAsync(c->d_un, uny, bytes, hipMemcpyHostToDevice, c->c.stream));
+  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, bytes, hipMemcpyDeviceToDevice, c->c.stream));
+  c->last_nonlinear = 0;
+  return sync(c);
+}
+
+int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!uny) return fail(PSDNS_EINVAL, "null state");
+  if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
+  HIPCHK(hipMemcpyAsync(uny, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToHost, c->c.stream));
+  return sync(c);
+}
+
+int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
+  c->c.kstep = kstep - 1;
+  int rc;
+  if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, sizeof(unsigned long long), c->c.stream));
+  if ((rc = mark(c, 0))) return rc;
+  if ((rc = pass(c, Pass::YInvPhase, 3))) return rc;
+  if ((rc = mark(c, 1))) return rc;
+  if ((rc = pass(c, Pass::ZInv, 3))) return rc;
+  if ((rc = mark(c, 2))) return rc;
+  if ((rc = pass(c, Pass::XFused, kstep == 1))) return rc;
+  if ((rc = mark(c, 3))) return rc;
+  if ((rc = pass(c, Pass::ZFwdConv, 5))) return rc;
+  if ((rc = mark(c, 4))) return rc;
+  if (kstep == 1) {
+    if ((rc = read_velmax(c, velmax))) return rc;
+  }
+  if ((rc = sync(c))) return rc;
+  if (c->timing) {
+    float ms;
+    for (int i = 0; i < 4; i++) {
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+      c->times[i] += ms;
+    }
+  }
+  c->last_nonlinear = kstep;
+  return PSDNS_OK;
+}
+
+int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, const double* yfac,
+                       const double* zfac, const void* forcing) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
+  if (forcing) return fail(PSDNS_EINVAL, "EPFOR forcing increments are not supported (pass NULL)");
+  if (c->last_nonlinear != kstep)
+    return fail(PSDNS_ESTATE, "stage_update(%d) without a preceding stage_nonlinear(%d)", kstep, kstep);
+  if (!(dt > 0.0)) return fail(PSDNS_EINVAL, "dt = %g must be positive", dt);
+  int rc;
+  if (kstep == 1 || xfac) {
+    if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
+  }
+  c->c.kstep = kstep - 1;
+  if ((rc = mark(c, 5))) return rc;
+  if ((rc = pass(c, Pass::YFwd, 4))) return rc;
+  const double dt2 = dt / 2.0;
+  const double norm = dt2 / c->n / c->n;  // wavespace_vel.f:80
+  psdns::launch_wavespace_update(c->c, norm, kstep);
+  HIPCHK(hipGetLastError());
+  if ((rc = mark(c, 6))) return rc;
+  if ((rc = sync(c))) return rc;
+  if (c->timing) {
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+    c->times[4] += ms;
+  }
+  c->last_nonlinear = 0;
+  return PSDNS_OK;
+}
+
+int psdns_stage_times(psdns_ctx* c, int enable, double* times) {
+  if (int rc = check_ctx(c)) return rc;
+  if (times) {
+    double tot = 0;
+    for (int i = 0; i < 5; i++) {
+      times[i] = c->times[i];
+      tot += c->times[i];
+    }
+    times[5] = tot;
+  }
+  for (double& t : c->times) t = 0;
+  c->timing = enable != 0;
+  return PSDNS_OK;
+}
+
+int psdns_ek_shell(psdns_ctx* c, int kfor, double* ek) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kfor < 1 || kfor > 64 || !ek) return fail(PSDNS_EINVAL, "kfor = %d: need 1..64 and an output array", kfor);
+  int rc;
+  if ((rc = ensure_force_list(c, kfor))) return rc;
+  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(double) * kfor, c->c.stream));
+  if (c->nforce) psdns::launch_ek_shell(c->c, kfor, c->d_red);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ek, c->d_red, sizeof(double) * kfor, hipMemcpyDeviceToHost, c->c.stream));
+  return sync(c);
+}
+
+int psdns_ek_force(psdns_ctx* c, int kf_shell, int kfor, const double* ekinf, double* ek_out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kf_shell < 1 || kf_shell > kfor || !ekinf)
+    return fail(PSDNS_EINVAL, "need 1 <= kf_shell (%d) <= kfor (%d) and ekinf", kf_shell, kfor);
+  std::vector<double> ek(kfor), ratio(kfor, 0.0);
+  int rc;
+  if ((rc = psdns_ek_shell(c, kfor, ek.data()))) return rc;
+  // ek_force.f:31-35
+  ratio[0] = 0.0;
+  for (int ik = 2; ik <= kfor; ik++) {
+    if (!(ek[ik - 1] > 0.0)) return fail(PSDNS_EINVAL, "shell %d has no energy to rescale", ik);
+    ratio[ik - 1] = std::sqrt(ekinf[ik - 1] / ek[ik - 1]);
+  }
+  HIPCHK(hipMemcpyAsync(c->d_red + 64, ratio.data(), sizeof(double) * kfor, hipMemcpyHostToDevice, c->c.stream));
+  if (c->nforce) psdns::launch_ek_scale(c->c, kf_shell, c->d_red + 64);
+  HIPCHK(hipGetLastError());
+  if ((rc = sync(c))) return rc;
+  if (ek_out) return psdns_ek_shell(c, kfor, ek_out);
+  return PSDNS_OK;
+}
+
+int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* tke, double* eps) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nshell < 0 || nshell > c->n) return fail(PSDNS_EINVAL, "nshell = %d out of range", nshell);
+  if (nshell > 0 && (!ek || !dk)) return fail(PSDNS_EINVAL, "null spectrum arrays");
+  double* d_tot = c->d_red;
+  double* d_ek = c->d_red + 2;
+  double* d_dk = c->d_red + 2 + c->n;
+  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(double) * (2 + 2 * (size_t)c->n), c->c.stream));
+  psdns::launch_spectrum(c->c, c->viscos, nshell, d_ek, d_dk, d_tot);
+  HIPCHK(hipGetLastError());
+  double tot[2];
+  HIPCHK(hipMemcpyAsync(tot, d_tot, sizeof tot, hipMemcpyDeviceToHost, c->c.stream));
+  if (nshell > 0) {
+    HIPCHK(hipMemcpyAsync(ek, d_ek, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
+    HIPCHK(hipMemcpyAsync(dk, d_dk, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
+  }
+  int rc;
+  if ((rc = sync(c))) return rc;
+  if (tke) *tke = tot[0];
+  if (eps) *eps = tot[1];
+  return PSDNS_OK;
+}
+
+int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double tke) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!(kpeak > 0) || !(tke > 0)) return fail(PSDNS_EINVAL, "kpeak and tke must be positive");
+  psdns::launch_field(c->c, seed, kpeak, c->d_colz0);
+  HIPCHK(hipGetLastError());
+  double t = 0;
+  int rc;
+  if ((rc = psdns_shell_stats(c, nullptr, nullptr, 0, &t, nullptr))) return rc;
+  if (!(t > 0)) return fail(PSDNS_EINVAL, "initial field has no energy");
+  psdns::launch_scale(c->c, c->d_un, 3 * c->cyl_elems(), std::sqrt(tke / t));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToDevice, c->c.stream));
+  c->last_nonlinear = 0;
+  return sync(c);
+}
+
+int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!host) return fail(PSDNS_EINVAL, "null host array");
+  size_t bytes;
+  void* dst;
+  if (which == PSDNS_FIELD_U && nv >= 1 && nv <= 5) {
+    dst = c->d_u;
+    bytes = c->csize() * nv * c->cyl_elems();
+  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3) {
+    dst = c->d_un;
+    bytes = c->csize() * nv * c->cyl_elems();
+  } else if (which == PSDNS_FIELD_UX && nv >= 1 && nv <= 5) {
+    if (int rc = ensure_ux(c)) return rc;
+    dst = c->d_ux;
+    bytes = c->rsize() * nv * (size_t)c->n * c->n * c->n;
+  } else {
+    return fail(PSDNS_EINVAL, "bad field %d / nv %d", which, nv);
+  }
+  HIPCHK(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->c.stream));
+  return sync(c);
+}
+
+int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!host) return fail(PSDNS_EINVAL, "null host array");
+  size_t bytes;
+  const void* src;
+  if (which == PSDNS_FIELD_U && nv >= 1 && nv <= 5) {
+    src = c->d_u;
+    bytes = c->csize() * nv * c->cyl_elems();
+  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3) {
+    src = c->d_un;
+    bytes = c->csize() * nv * c->cyl_elems();
+  } else if (which == PSDNS_FIELD_UX && nv >= 1 && nv <= 5) {
+    if (int rc = ensure_ux(c)) return rc;
+    src = c->d_ux;
+    bytes = c->rsize() * nv * (size_t)c->n * c->n * c->n;
+  } else {
+    return fail(PSDNS_EINVAL, "bad field %d / nv %d", which, nv);
+  }
+  HIPCHK(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->c.stream));
+  return sync(c);
+}
+
+int psdns_phase1(psdns_ctx* c, int kstep) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
+  c->c.kstep = kstep - 1;
+  psdns::launch_phshift(c->c, 3);
+  return sync(c);
+}
+
+int psdns_itransform_vel(psdns_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  if ((rc = pass(c, Pass::YInvPlain, 3))) return rc;
+  if ((rc = pass(c, Pass::ZInv, 3))) return rc;
+  if ((rc = pass(c, Pass::XC2R, 3))) return rc;
+  return sync(c);
+}
+
+int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
+  if (int rc = check_ctx(c)) return rc;
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  HIPCHK(hipMemsetAsync(c->d_velmax, 0, sizeof(unsigned long long), c->c.stream));
+  psdns::launch_realspace(c->c, rkstep == 1);
+  HIPCHK(hipGetLastError());
+  if (rkstep == 1) return read_velmax(c, velmax);
+  return sync(c);
+}
+
+int psdns_transform(psdns_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  if ((rc = pass(c, Pass::XR2C, 5))) return rc;
+  if ((rc = pass(c, Pass::ZFwdConv, 5))) return rc;
+  if ((rc = pass(c, Pass::YFwd, 4))) return rc;
+  return sync(c);
+}
+
+int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
+  if (int rc = check_ctx(c)) return rc;
+  if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
+  c->c.kstep = kstep - 1;
+  psdns::launch_wavespace(c->c, dt2 / c->n / c->n);
+  return sync(c);
+}
+
+int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
+  if (int rc = check_ctx(c)) return rc;
+  int rc;
+  if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
+  psdns::launch_update(c->c, 1);
+  return sync(c);
+}
+
+int psdns_proc4b(psdns_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  psdns::launch_update(c->c, 2);
+  return sync(c);
+}
+
+int psdns_kxtran(psdns_ctx* c, int nv) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  if ((rc = pass(c, Pass::YInvKx, nv))) return rc;
+  if ((rc = pass(c, Pass::ZInv, nv))) return rc;
+  if ((rc = pass(c, Pass::XC2R, nv))) return rc;
+  return sync(c);
+}
+
+int psdns_xktran(psdns_ctx* c, int nv) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  if ((rc = pass(c, Pass::XR2C, nv))) return rc;
+  if ((rc = pass(c, Pass::ZFwdNorm, nv))) return rc;
+  if ((rc = pass(c, Pass::YFwd, nv))) return rc;
+  return sync(c);
+}
+
+}  // extern "C"

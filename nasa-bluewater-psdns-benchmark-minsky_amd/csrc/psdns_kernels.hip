@@ -1,0 +1,1009 @@
+// HIP kernels of the PSDNS RK2 hot path for gfx950 (MI355X).
+//
+// Device data (one variable = one slot; complex = interleaved re, im):
+//   cylinder  u[v][a][y]       reference uy(ny, num_al, nv)      (state)
+//   z-pencil  W[v][y][kx][z]   reference uz(nz, xisz, yjsz, nv)  (work)
+//   physical  ux[v][y][z][x]   reference ux(nx, zisz, yjsz, nv)  (only for
+//             the per-routine entry points; the fused step never writes it)
+// Only cylinder columns (kx < max_al_x, z outside the cut) are ever stored
+// in W by the inverse y pass and read by the forward y pass; z entries in
+// the cut are synthesised as zeros by the z pass instead of being stored.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_line.hpp"
+#include "psdns_internal.h"
+
+namespace psdns {
+
+template <typename R>
+using C = cpx<R>;
+
+// thread-team shape of a length-L line: E points per thread, T threads
+template <int L>
+struct Team {
+  static constexpr int E = L >= 1024 ? 16 : (L >= 512 ? 8 : 4);
+  static constexpr int T = L / E;
+};
+
+// lines per work-group for passes that transpose through an LDS tile:
+// as many as fit about 80 KB of LDS, at most 16, at least one 256-thread WG
+template <typename R, int L>
+struct TileLines {
+  static constexpr int kLineBytes = 2 * LdsLine<L>::kWords * (int)sizeof(R);
+  static constexpr int kFit = 82000 / kLineBytes;
+  static constexpr int kCap = kFit >= 16 ? 16 : (kFit >= 8 ? 8 : (kFit >= 4 ? 4 : (kFit >= 2 ? 2 : 1)));
+  static constexpr int kMin = 256 / Team<L>::T;
+  static constexpr int B = kCap > kMin ? kCap : kMin;
+};
+
+__device__ __forceinline__ int kwave(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// masks.f:67-77 in exact integer form: (k/n)^2 summed <= 2/9 <=> 9 k^2 <= 2 n^2
+// (k/n is exact in binary for n a power of two; the sum never equals 2/9)
+__device__ __forceinline__ bool in_mask(int kx, int ky, int kz, int y, int z, int n) {
+  return 9 * (kx * kx + ky * ky + kz * kz) <= 2 * n * n && y != n / 2 && z != n / 2;
+}
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename R>
+__device__ __forceinline__ C<R> to_c(double2 a) {
+  return {(R)a.x, (R)a.y};
+}
+
+// W index of (v, y, kx, z)
+__device__ __forceinline__ size_t widx(int v, int y, int kx, int z, int n, int nxh) {
+  return (((size_t)v * n + y) * nxh + kx) * n + z;
+}
+
+// ---------------------------------------------------------------------------
+// Inverse y pass: cylinder -> z-pencils.
+// MODE 0: phase1 (phshift_inplace, homogeneous/phshift.f:71-135) fused on
+//         load: u *= sy(y) (sx(x) sz(z)) on the mask, else 0.  The shifted
+//         field is not written back: in the step the next writer of u is
+//         the forward transform (itransform_vel overwrites u in place).
+// MODE 1: kxtran (kxtran.f:36): zero the y Nyquist row.
+// MODE 2: plain inverse transform (itransform_vel on an already shifted u).
+// Then y c2c +i (kxcomm1_clean.f:118) and scatter of the column into W
+// (kxcomm1_clean.f:205-255) through an LDS tile so stores run along z.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int MODE>
+__global__ void __launch_bounds__((TileLines<R, N>::B * Team<N>::T))
+    k_y_inv(const C<R>* __restrict__ u, C<R>* __restrict__ W, Geo g, const double2* __restrict__ sx,
+            const double2* __restrict__ sy, const double2* __restrict__ sz,
+            const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = TileLines<R, N>::B;
+  constexpr int WW = LdsLine<N>::kWords;
+  using F = LineFft<R, N, E, +1>;
+  __shared__ R lds[2 * B * WW];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int var = blockIdx.y;
+  const int a = blockIdx.x * B + line;
+  const bool act = a < g.na;
+  const int x = act ? g.colx[a] : 0, z = act ? g.colz[a] : 0;
+  const C<R>* col = u + ((size_t)var * g.na + (act ? a : 0)) * N;
+  C<R> v[E];
+  double2 sxz = make_double2(1.0, 0.0);
+  if (MODE == 0 && act) sxz = zmul(sx[x], sz[z]);
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int y = t + T * j;
+    C<R> val = act ? col[y] : C<R>{0, 0};
+    if constexpr (MODE == 0) {
+      if (in_mask(x, kwave(y, N), kwave(z, N), y, z, N)) {
+        val = cmul(val, to_c<R>(zmul(sy[y], sxz)));
+      } else {
+        val = {0, 0};
+      }
+    } else if constexpr (MODE == 1) {
+      if (y == N / 2) val = {0, 0};
+    }
+    v[j] = val;
+  }
+  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+  __syncthreads();
+  R* tre = lds;
+  R* tim = lds + B * WW;
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int p = lpad((t + T * j) * B + line);
+    tre[p] = v[j].x;
+    tim[p] = v[j].y;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < B * N; e += B * T) {
+    const int l = e % B, y = e / B;
+    const int aa = blockIdx.x * B + l;
+    if (aa < g.na) {
+      const int p = lpad(e);
+      W[widx(var, y, g.colx[aa], g.colz[aa], N, g.nxh)] = {tre[p], tim[p]};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): for kx < max_al_x
+// z c2c +i in place; z inside the cylinder cut (which always holds the z
+// Nyquist) is read as zero (kxcomm1_clean.f:238-240).  Lines beyond
+// max_al_x are zero and are never touched: the x pass reads them as zero.
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+__global__ void __launch_bounds__(256) k_z_inv(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = 256 / T;
+  constexpr int WW = LdsLine<N>::kWords;
+  using F = LineFft<R, N, E, +1>;
+  __shared__ R lds[2 * B * WW];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int var = blockIdx.y;
+  const int li = blockIdx.x * B + line;  // line = y * max_al_x + x
+  const bool act = li < N * g.max_al_x;
+  const int y = act ? li / g.max_al_x : 0, x = act ? li % g.max_al_x : 0;
+  const int c = g.cutz[x];
+  C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
+  C<R> v[E];
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int zz = t + T * j;
+    const bool kept = zz < N / 2 - c || zz > N / 2 + c;
+    v[j] = (act && kept) ? p[zz] : C<R>{0, 0};
+  }
+  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < E; j++) p[t + T * j] = v[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass building blocks (kxcomm2.f:278-333 inverse, xkcomm1.f:97,135
+// forward).  A work-group owns one y and B consecutive z (lines along kx,
+// stride n in W); LDS tile transposes kx-rows of B z values.
+// The real transform of length n runs as a complex transform of length
+// m = n/2 on packed pairs (x[2i], x[2i+1]).
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+struct XPass {
+  static constexpr int M = N / 2;
+  static constexpr int T = Team<M>::T, E = Team<M>::E;
+  static constexpr int B = TileLines<R, M>::B < N ? TileLines<R, M>::B : N;  // z per work-group
+  static constexpr int WW = LdsLine<M>::kWords;
+  static constexpr int NT = B * T;
+
+  // c2r of one W variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
+  // ignored).  Output: packed physical values in coalesced order.
+  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ W, int var, int y,
+                                             int z0, const Geo& g, R* lds, const C<R>* __restrict__ tw,
+                                             int line, int t) {
+    R* tre = lds;
+    R* tim = lds + B * WW;
+    __syncthreads();
+    for (int e = threadIdx.x; e < B * g.max_al_x; e += NT) {
+      const int l = e % B, kx = e / B;
+      const C<R> val = W[widx(var, y, kx, z0 + l, N, g.nxh)];
+      const int p = lpad(e);
+      tre[p] = val.x;
+      tim[p] = val.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int k = t + T * j;
+      const int km = M - k;
+      C<R> xk = {0, 0}, xm = {0, 0};
+      if (k < g.max_al_x) {
+        const int p = lpad(k * B + line);
+        xk = {tre[p], tim[p]};
+      }
+      if (k == 0) xk.y = 0;
+      if (km < g.max_al_x) {
+        const int p = lpad(km * B + line);
+        xm = {tre[p], tim[p]};
+      }
+      // Z = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k])
+      const C<R> s = {xk.x + xm.x, xk.y - xm.y};
+      const C<R> d = {xk.x - xm.x, xk.y + xm.y};
+      C<R> w = tw[k];
+      w.y = -w.y;
+      const C<R> wd = cmul(w, d);
+      v[j] = {s.x - wd.y, s.y + wd.x};
+    }
+    LineFft<R, M, E, +1>::template run<2>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+  }
+
+  // r2c of packed physical values v (coalesced order), scaled by `scale`,
+  // stored for kx < max_al_x into W variable var (kx Nyquist dropped).
+  static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ W, int var, int y, int z0,
+                                             const Geo& g, R* lds, const C<R>* __restrict__ tw,
+                                             int line, int t, R scale, bool act_store) {
+    LineFft<R, M, E, -1>::template run<2>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+    R* lre = lds + line * WW;
+    R* lim = lds + B * WW + line * WW;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int p = lpad(t + T * j);
+      lre[p] = v[j].x;
+      lim[p] = v[j].y;
+    }
+    __syncthreads();
+    const R half = (R)0.5 * scale;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int k = t + T * j;
+      const int p = lpad(k == 0 ? 0 : M - k);
+      const C<R> zm = {lre[p], -lim[p]};  // conj Z[m-k]
+      const C<R> zk = v[j];
+      const C<R> s = zk + zm;
+      const C<R> d = zk - zm;
+      // X = (s - i e^{-2 pi i k/n} d) / 2
+      const C<R> wd = cmul(tw[k], d);
+      v[j] = {(s.x + wd.y) * half, (s.y - wd.x) * half};
+    }
+    __syncthreads();
+    R* tre = lds;
+    R* tim = lds + B * WW;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int p = lpad((t + T * j) * B + line);
+      tre[p] = v[j].x;
+      tim[p] = v[j].y;
+    }
+    __syncthreads();
+    if (act_store) {
+      for (int e = threadIdx.x; e < B * g.max_al_x; e += NT) {
+        const int l = e % B, kx = e / B;
+        const int p = lpad(e);
+        W[widx(var, y, kx, z0 + l, N, g.nxh)] = {tre[p], tim[p]};
+      }
+    }
+  }
+};
+
+// inverse x pass alone: W (nv variables) -> physical ux
+template <typename R, int N>
+__global__ void __launch_bounds__((XPass<R, N>::NT))
+    k_x_c2r(const C<R>* __restrict__ W, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
+  using X = XPass<R, N>;
+  __shared__ R lds[2 * X::B * X::WW];
+  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
+  const int var = blockIdx.y;
+  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  C<R> v[X::E];
+  X::c2r(v, W, var, y, z0, g, lds, tw, line, t);
+  C<R>* dst = reinterpret_cast<C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
+#pragma unroll
+  for (int j = 0; j < X::E; j++) dst[t + X::T * j] = v[j];
+}
+
+// forward x pass alone: physical ux (nv variables) -> W, x 1/n
+template <typename R, int N>
+__global__ void __launch_bounds__((XPass<R, N>::NT))
+    k_x_r2c(const R* __restrict__ ux, C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw) {
+  using X = XPass<R, N>;
+  __shared__ R lds[2 * X::B * X::WW];
+  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
+  const int var = blockIdx.y;
+  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  const C<R>* src = reinterpret_cast<const C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
+  C<R> v[X::E];
+#pragma unroll
+  for (int j = 0; j < X::E; j++) v[j] = src[t + X::T * j];
+  X::r2c(v, W, var, y, z0, g, lds, tw, line, t, (R)1 / (R)N, true);
+}
+
+__device__ __forceinline__ void velmax_commit(double m, unsigned long long* velmax) {
+  // wave reduction, one atomic per wave (non-negative doubles order as integers)
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(velmax, (unsigned long long)__double_as_longlong(m));
+}
+
+// realspace_vel.f:105-113 on one packed pair; returns nx(|u|+|v|+|w|) max
+template <typename R>
+__device__ __forceinline__ double products(R u, R v, R w, R* p) {
+  const R bv2 = v * v;
+  p[4] = u * w;
+  p[3] = v * w;
+  p[2] = w * w - bv2;
+  p[1] = u * v;
+  p[0] = u * u - bv2;
+  return fabs((double)u) + fabs((double)v) + fabs((double)w);
+}
+
+// Fused x pass: c2r of u, v, w -> realspace_vel products (+ velmax) -> r2c
+// x 1/n of the five products, written back into W slots 0..4.  Physical
+// space stays in registers.
+template <typename R, int N>
+__global__ void __launch_bounds__((XPass<R, N>::NT))
+    k_x_fused(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw,
+              unsigned long long* __restrict__ velmax, int want_velmax) {
+  using X = XPass<R, N>;
+  constexpr int E = X::E;
+  __shared__ R lds[2 * X::B * X::WW];
+  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
+  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  C<R> pu[E], pv[E], pw[E];
+  X::c2r(pu, W, 0, y, z0, g, lds, tw, line, t);
+  X::c2r(pv, W, 1, y, z0, g, lds, tw, line, t);
+  X::c2r(pw, W, 2, y, z0, g, lds, tw, line, t);
+  double vm = 0.0;
+  C<R> q[E];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      R pa[5], pb[5];
+      const double ma = products(pu[j].x, pv[j].x, pw[j].x, pa);
+      const double mb = products(pu[j].y, pv[j].y, pw[j].y, pb);
+      if (s == 0) vm = fmax(vm, fmax(ma, mb));
+      q[j] = {pa[s], pb[s]};
+    }
+    X::r2c(q, W, s, y, z0, g, lds, tw, line, t, (R)1 / (R)N, true);
+  }
+  if (want_velmax) velmax_commit(vm * (double)N, velmax);
+}
+
+// realspace_vel alone on physical ux (slots 0..4)
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_realspace(R* __restrict__ ux, size_t n3, double xnorm, unsigned long long* __restrict__ velmax,
+                int want_velmax) {
+  double vm = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+    R p[5];
+    const double m = products(ux[i], ux[n3 + i], ux[2 * n3 + i], p);
+    vm = fmax(vm, m);
+#pragma unroll
+    for (int s = 0; s < 5; s++) ux[s * n3 + i] = p[s];
+  }
+  if (want_velmax) velmax_commit(vm * xnorm, velmax);
+}
+
+// ---------------------------------------------------------------------------
+// Forward z pass (transform.f:129-138) and the convective combination
+// (transform.f:169-176,229-234): C1 = i kx P1 + i kz P5, C3 = i kx P5 + i kz P3
+// (z Nyquist skipped, it is outside the cylinder).  MODE 0: transform
+// (5 slots in, C1 P2 C3 P4 out); MODE 1: xktran (nv slots, x 1/(ny nz),
+// xktran.f:121,150-157).  Only cylinder z are stored.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int MODE>
+__global__ void __launch_bounds__(256)
+    k_z_fwd(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw, int nv) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = 256 / T;
+  constexpr int WW = LdsLine<N>::kWords;
+  using F = LineFft<R, N, E, -1>;
+  __shared__ R lds[2 * B * WW];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int li = blockIdx.x * B + line;
+  const bool act = li < N * g.max_al_x;
+  const int y = act ? li / g.max_al_x : 0, x = act ? li % g.max_al_x : 0;
+  const int c = g.cutz[x];
+  R* lre = lds + line * WW;
+  R* lim = lds + B * WW + line * WW;
+  auto load = [&](C<R>* v, int var) {
+    const C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
+#pragma unroll
+    for (int j = 0; j < E; j++) v[j] = act ? p[t + T * j] : C<R>{0, 0};
+  };
+  auto store = [&](const C<R>* v, int var) {
+    C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
+    if (!act) return;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int zz = t + T * j;
+      if (zz < N / 2 - c || zz > N / 2 + c) p[zz] = v[j];
+    }
+  };
+  if constexpr (MODE == 0) {
+    C<R> p5[E], w[E];
+    load(p5, 4);
+    F::template run<1>(p5, lre, lim, tw, t);
+    load(w, 0);
+    F::template run<1>(w, lre, lim, tw, t);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const R kz = (R)kwave(t + T * j, N), kx = (R)x;
+      const C<R> a = {kx * w[j].x + kz * p5[j].x, kx * w[j].y + kz * p5[j].y};
+      w[j] = {-a.y, a.x};  // i (kx P1 + kz P5)
+    }
+    store(w, 0);
+    load(w, 2);
+    F::template run<1>(w, lre, lim, tw, t);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const R kz = (R)kwave(t + T * j, N), kx = (R)x;
+      const C<R> a = {kx * p5[j].x + kz * w[j].x, kx * p5[j].y + kz * w[j].y};
+      w[j] = {-a.y, a.x};  // i (kx P5 + kz P3)
+    }
+    store(w, 2);
+    load(w, 1);
+    F::template run<1>(w, lre, lim, tw, t);
+    store(w, 1);
+    load(w, 3);
+    F::template run<1>(w, lre, lim, tw, t);
+    store(w, 3);
+  } else {
+    const R s = (R)(1.0 / ((double)N * (double)N));
+    for (int var = 0; var < nv; var++) {
+      C<R> w[E];
+      load(w, var);
+      F::template run<1>(w, lre, lim, tw, t);
+#pragma unroll
+      for (int j = 0; j < E; j++) w[j] = cscale(w[j], s);
+      store(w, var);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward y pass (xkcomm2.f:96-125 pack, xkcomm2.f:229 y c2c -i): gather the
+// cylinder columns from W through an LDS tile, transform, store contiguous.
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+__global__ void __launch_bounds__((TileLines<R, N>::B * Team<N>::T))
+    k_y_fwd(const C<R>* __restrict__ W, C<R>* __restrict__ u, Geo g, const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = TileLines<R, N>::B;
+  constexpr int WW = LdsLine<N>::kWords;
+  using F = LineFft<R, N, E, -1>;
+  __shared__ R lds[2 * B * WW];
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int var = blockIdx.y;
+  const int a = blockIdx.x * B + line;
+  R* tre = lds;
+  R* tim = lds + B * WW;
+  for (int e = threadIdx.x; e < B * N; e += B * T) {
+    const int l = e % B, y = e / B;
+    const int aa = blockIdx.x * B + l;
+    if (aa < g.na) {
+      const C<R> val = W[widx(var, y, g.colx[aa], g.colz[aa], N, g.nxh)];
+      const int p = lpad(e);
+      tre[p] = val.x;
+      tim[p] = val.y;
+    }
+  }
+  __syncthreads();
+  C<R> v[E];
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int p = lpad((t + T * j) * B + line);
+    v[j] = {tre[p], tim[p]};
+  }
+  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+  if (a < g.na) {
+    C<R>* col = u + ((size_t)var * g.na + a) * N;
+#pragma unroll
+    for (int j = 0; j < E; j++) col[t + T * j] = v[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise cylinder kernels: one thread per (a, y).
+// ---------------------------------------------------------------------------
+struct Pt {
+  int a, y, x, z, ky, kz;
+  bool m;
+};
+__device__ __forceinline__ Pt point(size_t i, const Geo& g) {
+  Pt p;
+  p.a = (int)(i / g.n);
+  p.y = (int)(i % g.n);
+  p.x = g.colx[p.a];
+  p.z = g.colz[p.a];
+  p.ky = kwave(p.y, g.n);
+  p.kz = kwave(p.z, g.n);
+  p.m = in_mask(p.x, p.ky, p.kz, p.y, p.z, g.n);
+  return p;
+}
+
+// wavespace_vel.f:80,162-174 (b = 1): N1 = sxyz C1 + i ky sxyz P2,
+// N2 = sxyz i (kx P2 + kz P4), N3 = sxyz C3 + i ky sxyz P4,
+// sxyz = -(dt2/ny/nz) conj(sxz sy); off the mask all five slots are zeroed.
+template <typename R>
+__device__ __forceinline__ void wavespace_point(const Pt& p, C<R>* c1, C<R>* p2, C<R>* c3, C<R>* p4,
+                                                C<R>* p5, double norm, const double2* sx,
+                                                const double2* sy, const double2* sz) {
+  if (!p.m) {
+    *c1 = *p2 = *c3 = *p4 = {0, 0};
+    if (p5) *p5 = {0, 0};
+    return;
+  }
+  const double2 s = zmul(zmul(sz[p.z], sx[p.x]), sy[p.y]);
+  const C<R> sxyz = {(R)(-norm * s.x), (R)(norm * s.y)};
+  const C<R> ut = {-(R)p.ky * sxyz.y, (R)p.ky * sxyz.x};  // i ky sxyz
+  const C<R> a1 = *c1, b2 = *p2, a3 = *c3, b4 = *p4;
+  *c1 = cmul(sxyz, a1) + cmul(ut, b2);
+  const C<R> si = {-sxyz.y, sxyz.x};  // sxyz i
+  const C<R> q = {(R)p.x * b2.x + (R)p.kz * b4.x, (R)p.x * b2.y + (R)p.kz * b4.y};
+  *p2 = cmul(si, q);
+  *c3 = cmul(sxyz, a3) + cmul(ut, b4);
+}
+
+// phase1 alone (homogeneous/phshift.f:71-135, in place)
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_phshift(C<R>* __restrict__ u, Geo g, int nv, const double2* __restrict__ sx,
+              const double2* __restrict__ sy, const double2* __restrict__ sz) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (p.m) {
+      const C<R> s = to_c<R>(zmul(sy[p.y], zmul(sx[p.x], sz[p.z])));
+      for (int v = 0; v < nv; v++) u[v * tot + i] = cmul(u[v * tot + i], s);
+    } else {
+      for (int v = 0; v < nv; v++) u[v * tot + i] = {0, 0};
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_wavespace(C<R>* __restrict__ u, Geo g, double norm, const double2* __restrict__ sx,
+                const double2* __restrict__ sy, const double2* __restrict__ sz) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    wavespace_point<R>(p, u + i, u + tot + i, u + 2 * tot + i, u + 3 * tot + i, u + 4 * tot + i, norm, sx,
+                       sy, sz);
+  }
+}
+
+// proc4a.f:540-571 / proc4b.f:85-106 pressure projection (k = 0 excluded)
+template <typename R>
+__device__ __forceinline__ void project(C<R>* u, const Pt& p) {
+  if (p.x == 0 && p.z == 0 && p.y == 0) return;
+  const R kx = (R)p.x, ky = (R)p.ky, kz = (R)p.kz;
+  const R den = kx * kx + ky * ky + kz * kz;
+  const C<R> num = {kx * u[0].x + ky * u[1].x + kz * u[2].x, kx * u[0].y + ky * u[1].y + kz * u[2].y};
+  const C<R> psi = {-num.x / den, -num.y / den};
+  u[0] = {u[0].x + kx * psi.x, u[0].y + kx * psi.y};
+  u[1] = {u[1].x + ky * psi.x, u[1].y + ky * psi.y};
+  u[2] = {u[2].x + kz * psi.x, u[2].y + kz * psi.y};
+}
+
+// proc4a.f:418-487: F = xfac yfac zfac; un <- F (un + N); u <- F (un + 2N),
+// then project u.  proc4b.f:54-67,176-205: u <- un + N, project, un <- u,
+// zero mean.  STAGE 1 / 2.  Off-mask entries are left untouched.
+template <typename R, int STAGE>
+__device__ __forceinline__ void update_point(const Pt& p, C<R>* nl, C<R>* un, const double* xfac,
+                                             const double* yfac, const double* zfac) {
+  if (!p.m) return;
+  if constexpr (STAGE == 1) {
+    const R f = (R)(xfac[p.x] * yfac[p.y] * zfac[p.z]);
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      const C<R> ut = un[v];
+      un[v] = {f * (ut.x + nl[v].x), f * (ut.y + nl[v].y)};
+      nl[v] = {f * (ut.x + nl[v].x * (R)2), f * (ut.y + nl[v].y * (R)2)};
+    }
+    project(nl, p);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 3; v++) nl[v] = un[v] + nl[v];
+    project(nl, p);
+    if (p.x == 0 && p.z == 0 && p.y == 0) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) nl[v] = {0, 0};
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) un[v] = nl[v];
+  }
+}
+
+template <typename R, int STAGE>
+__global__ void __launch_bounds__(256)
+    k_update(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, const double* __restrict__ xfac,
+             const double* __restrict__ yfac, const double* __restrict__ zfac) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (!p.m) continue;
+    C<R> nl[3] = {u[i], u[tot + i], u[2 * tot + i]};
+    C<R> o[3] = {un[i], un[tot + i], un[2 * tot + i]};
+    update_point<R, STAGE>(p, nl, o, xfac, yfac, zfac);
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      u[v * tot + i] = nl[v];
+      un[v * tot + i] = o[v];
+    }
+  }
+}
+
+// wavespace_vel + proc4a/proc4b in one pass over the cylinder
+template <typename R, int STAGE>
+__global__ void __launch_bounds__(256)
+    k_wavespace_update(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
+                       const double2* __restrict__ sx, const double2* __restrict__ sy,
+                       const double2* __restrict__ sz, const double* __restrict__ xfac,
+                       const double* __restrict__ yfac, const double* __restrict__ zfac) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (!p.m) {
+      // wavespace zeroes the nonlinear slots off the mask (wavespace_vel.f:171-173)
+#pragma unroll
+      for (int v = 0; v < 3; v++) u[v * tot + i] = {0, 0};
+      continue;
+    }
+    C<R> nl[4] = {u[i], u[tot + i], u[2 * tot + i], u[3 * tot + i]};
+    wavespace_point<R>(p, nl, nl + 1, nl + 2, nl + 3, nullptr, norm, sx, sy, sz);
+    C<R> o[3] = {un[i], un[tot + i], un[2 * tot + i]};
+    update_point<R, STAGE>(p, nl, o, xfac, yfac, zfac);
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      u[v * tot + i] = nl[v];
+      un[v * tot + i] = o[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FEK forcing over a short list of (a, y) entries with |k| < kf_shell + 1.5
+// (ek_shell.f:131-168, ek_force.f:47-66), FP64 sums.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_ek_shell(const C<R>* __restrict__ un, Geo g, const int2* __restrict__ list, int nlist,
+                           int kfor, double* __restrict__ ek) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nlist; i += gridDim.x * blockDim.x) {
+    const int a = list[i].x, y = list[i].y;
+    const int x = g.colx[a], z = g.colz[a];
+    const double kx = x, ky = kwave(y, g.n), kz = kwave(z, g.n);
+    const int ik = (int)(sqrt(kx * kx + ky * ky + kz * kz) + 1.5);
+    if (ik > kfor) continue;
+    const size_t j = (size_t)a * g.n + y;
+    double s = 0;
+    for (int v = 0; v < 3; v++) {
+      const C<R> q = un[v * tot + j];
+      s += (double)q.x * q.x + (double)q.y * q.y;
+    }
+    atomicAdd(ek + ik - 1, s / 2.0 * (x == 0 ? 1.0 : 2.0));
+  }
+}
+
+template <typename R>
+__global__ void k_ek_scale(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, const int2* __restrict__ list,
+                           int nlist, int kf_shell, const double* __restrict__ ratio) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nlist; i += gridDim.x * blockDim.x) {
+    const int a = list[i].x, y = list[i].y;
+    const int x = g.colx[a], z = g.colz[a];
+    const double kx = x, ky = kwave(y, g.n), kz = kwave(z, g.n);
+    const int ik = (int)(sqrt(kx * kx + ky * ky + kz * kz) + 1.5);
+    if (ik > kf_shell) continue;
+    const size_t j = (size_t)a * g.n + y;
+    const R r = (R)ratio[ik - 1];
+    for (int v = 0; v < 3; v++) {
+      un[v * tot + j] = cscale(un[v * tot + j], r);
+      u[v * tot + j] = cscale(u[v * tot + j], r);
+    }
+  }
+}
+
+// sptr.f:139-176 style FP64 shell sums: E(ik), D(ik) = 2 nu k^2 E, per block
+// partials reduced with FP64 atomics.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_spectrum(const C<R>* __restrict__ un, Geo g, double viscos, int nshell, double* __restrict__ ek,
+               double* __restrict__ dk, double* __restrict__ tot_out) {
+  const size_t tot = (size_t)g.na * g.n;
+  double te = 0, td = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (!p.m) continue;
+    double s = 0;
+    for (int v = 0; v < 3; v++) {
+      const C<R> q = un[v * tot + i];
+      s += (double)q.x * q.x + (double)q.y * q.y;
+    }
+    const double rk2 = (double)p.x * p.x + (double)p.ky * p.ky + (double)p.kz * p.kz;
+    const double en = 0.5 * (p.x == 0 ? 1.0 : 2.0) * s;
+    const int ik = (int)(sqrt(rk2) + 1.5);
+    te += en;
+    td += 2.0 * viscos * rk2 * en;
+    if (ik <= nshell) {
+      atomicAdd(ek + ik - 1, en);
+      atomicAdd(dk + ik - 1, 2.0 * viscos * rk2 * en);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    te += __shfl_xor(te, off);
+    td += __shfl_xor(td, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(tot_out, te);
+    atomicAdd(tot_out + 1, td);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initial field (homogeneous/field.f:172-257 construction, counter-based
+// generator keyed by (seed, x, y, z)); consym on kx = 0 in a second kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t v) {
+  v += 0x9E3779B97F4A7C15ULL;
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+  return v ^ (v >> 31);
+}
+__device__ __forceinline__ double uni(uint64_t seed, int x, int y, int z, int s) {
+  const uint64_t key = ((uint64_t)x << 42) ^ ((uint64_t)y << 21) ^ (uint64_t)z;
+  const uint64_t r = mix64(mix64(seed ^ mix64(key)) + (uint64_t)s);
+  return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_field(C<R>* __restrict__ un, Geo g, unsigned long long seed, double kpeak) {
+  const size_t tot = (size_t)g.na * g.n;
+  const double twopi = 6.283185307179586476925286766559;
+  const double rpid = 1.0 / sqrt(twopi);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    C<R> o[3] = {{0, 0}, {0, 0}, {0, 0}};
+    if (p.m && !(p.x == 0 && p.z == 0 && p.y == 0)) {
+      const double k1 = p.x, k2 = p.ky, k3 = p.kz;
+      const double k = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+      const double ekx = k * k * k * k * exp(-2.0 * (k / kpeak) * (k / kpeak));
+      const double ef = sqrt(ekx) * rpid / k;
+      const double th1 = uni(seed, p.x, p.y, p.z, 0), th2 = uni(seed, p.x, p.y, p.z, 1);
+      const double ph = uni(seed, p.x, p.y, p.z, 2);
+      const double ca = cos(twopi * ph), sa = sin(twopi * ph);
+      const double2 al = make_double2(ef * cos(twopi * th1) * ca, ef * sin(twopi * th1) * ca);
+      const double2 be = make_double2(ef * cos(twopi * th2) * sa, ef * sin(twopi * th2) * sa);
+      const double k12 = sqrt(k1 * k1 + k2 * k2);
+      if (k12 == 0.0) {
+        o[0] = to_c<R>(al);
+        o[1] = to_c<R>(be);
+      } else {
+        const double rt = 1.0 / k / k12;
+        const double2 c1 = make_double2(al.x * k, al.y * k), c2 = make_double2(be.x * k3, be.y * k3);
+        o[0] = to_c<R>(make_double2((c1.x * k2 + c2.x * k1) * rt, (c1.y * k2 + c2.y * k1) * rt));
+        o[1] = to_c<R>(make_double2((c2.x * k2 - c1.x * k1) * rt, (c2.y * k2 - c1.y * k1) * rt));
+        o[2] = to_c<R>(make_double2(-be.x * k12 / k, -be.y * k12 / k));
+      }
+    }
+    for (int v = 0; v < 3; v++) un[v * tot + i] = o[v];
+  }
+}
+
+// consym.f (FRESH branch) on the kx = 0 columns a < n0 (x == 0):
+// kz = 0 column: u(-ky) = conj u(ky); kz < 0 columns: conj of (-ky, -kz).
+// colz0[z] = column index of (x = 0, z) or -1.
+template <typename R>
+__global__ void k_consym(C<R>* __restrict__ un, Geo g, const int* __restrict__ colz0) {
+  const size_t tot = (size_t)g.na * g.n;
+  const int n = g.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n * n; i += gridDim.x * blockDim.x) {
+    const int z = i / n, y = i % n;
+    const int a = colz0[z];
+    if (a < 0) continue;
+    int src_a = -1, src_y = 0;
+    if (z == 0) {
+      if (y > n / 2) { src_a = a; src_y = n - y; }
+    } else if (z > n / 2) {
+      src_a = colz0[n - z];
+      src_y = (n - y) % n;
+    }
+    for (int v = 0; v < 3; v++) {
+      C<R>* u = un + v * tot;
+      if (z == 0 && y == 0) u[(size_t)a * n].y = 0;
+      if (src_a >= 0) u[(size_t)a * n + y] = conjg(u[(size_t)src_a * n + src_y]);
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_scale(C<R>* __restrict__ u, size_t cnt, double s) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+    u[i] = cscale(u[i], (R)s);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid1d(size_t work, int per) {
+  size_t b = (work + per - 1) / per;
+  if (b > 65536 * 16) b = 65536 * 16;
+  return (unsigned)(b ? b : 1);
+}
+
+template <typename R, int N>
+struct Launch {
+  using CR = C<R>;
+  static void y_inv(const Ctx& c, int nv, int mode) {
+    constexpr int B = TileLines<R, N>::B, T = Team<N>::T;
+    dim3 grid((c.geo.na + B - 1) / B, nv);
+    if (mode == 0)
+      hipLaunchKernelGGL((k_y_inv<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
+                         c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], (const CR*)c.tw);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_y_inv<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
+                         c.sx[0], c.sy[0], c.sz[0], (const CR*)c.tw);
+    else
+      hipLaunchKernelGGL((k_y_inv<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
+                         c.sx[0], c.sy[0], c.sz[0], (const CR*)c.tw);
+  }
+  static void z_inv(const Ctx& c, int nv) {
+    constexpr int B = 256 / Team<N>::T;
+    dim3 grid((N * c.geo.max_al_x + B - 1) / B, nv);
+    hipLaunchKernelGGL((k_z_inv<R, N>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw);
+  }
+  static void x_c2r(const Ctx& c, int nv) {
+    using X = XPass<R, N>;
+    dim3 grid(N * (N / X::B), nv);
+    hipLaunchKernelGGL((k_x_c2r<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.w, (R*)c.ux, c.geo,
+                       (const CR*)c.tw);
+  }
+  static void x_r2c(const Ctx& c, int nv) {
+    using X = XPass<R, N>;
+    dim3 grid(N * (N / X::B), nv);
+    hipLaunchKernelGGL((k_x_r2c<R, N>), grid, dim3(X::NT), 0, c.stream, (const R*)c.ux, (CR*)c.w, c.geo,
+                       (const CR*)c.tw);
+  }
+  static void x_fused(const Ctx& c, int want_velmax) {
+    using X = XPass<R, N>;
+    dim3 grid(N * (N / X::B));
+    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw,
+                       c.velmax_bits, want_velmax);
+  }
+  static void z_fwd(const Ctx& c, int mode, int nv) {
+    constexpr int B = 256 / Team<N>::T;
+    dim3 grid((N * c.geo.max_al_x + B - 1) / B);
+    if (mode == 0)
+      hipLaunchKernelGGL((k_z_fwd<R, N, 0>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw, nv);
+    else
+      hipLaunchKernelGGL((k_z_fwd<R, N, 1>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw, nv);
+  }
+  static void y_fwd(const Ctx& c, int nv) {
+    constexpr int B = TileLines<R, N>::B, T = Team<N>::T;
+    dim3 grid((c.geo.na + B - 1) / B, nv);
+    hipLaunchKernelGGL((k_y_fwd<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w, (CR*)c.u, c.geo,
+                       (const CR*)c.tw);
+  }
+};
+
+template <typename R>
+struct LaunchP {
+  using CR = C<R>;
+  static void realspace(const Ctx& c, int want_velmax) {
+    size_t n3 = (size_t)c.geo.n * c.geo.n * c.geo.n;
+    hipLaunchKernelGGL((k_realspace<R>), dim3(grid1d(n3, 256 * 8)), dim3(256), 0, c.stream, (R*)c.ux, n3,
+                       (double)c.geo.n, c.velmax_bits, want_velmax);
+  }
+  static void phshift(const Ctx& c, int nv) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    hipLaunchKernelGGL((k_phshift<R>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u, c.geo, nv,
+                       c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep]);
+  }
+  static void wavespace(const Ctx& c, double norm) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    hipLaunchKernelGGL((k_wavespace<R>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u, c.geo,
+                       norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep]);
+  }
+  static void update(const Ctx& c, int stage) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    if (stage == 1)
+      hipLaunchKernelGGL((k_update<R, 1>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u,
+                         (CR*)c.un, c.geo, c.xfac, c.yfac, c.zfac);
+    else
+      hipLaunchKernelGGL((k_update<R, 2>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u,
+                         (CR*)c.un, c.geo, c.xfac, c.yfac, c.zfac);
+  }
+  static void wavespace_update(const Ctx& c, double norm, int stage) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    if (stage == 1)
+      hipLaunchKernelGGL((k_wavespace_update<R, 1>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                         (CR*)c.u, (CR*)c.un, c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac,
+                         c.yfac, c.zfac);
+    else
+      hipLaunchKernelGGL((k_wavespace_update<R, 2>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                         (CR*)c.u, (CR*)c.un, c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac,
+                         c.yfac, c.zfac);
+  }
+  static void ek_shell(const Ctx& c, int kfor, double* d_ek) {
+    hipLaunchKernelGGL((k_ek_shell<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (const CR*)c.un,
+                       c.geo, c.force_list, c.nforce, kfor, d_ek);
+  }
+  static void ek_scale(const Ctx& c, int kf_shell, const double* d_ratio) {
+    hipLaunchKernelGGL((k_ek_scale<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (CR*)c.u,
+                       (CR*)c.un, c.geo, c.force_list, c.nforce, kf_shell, d_ratio);
+  }
+  static void spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    hipLaunchKernelGGL((k_spectrum<R>), dim3(grid1d(tot, 256 * 16)), dim3(256), 0, c.stream, (const CR*)c.un,
+                       c.geo, viscos, nshell, d_ek, d_dk, d_tot);
+  }
+  static void field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    hipLaunchKernelGGL((k_field<R>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.un, c.geo, seed,
+                       kpeak);
+    hipLaunchKernelGGL((k_consym<R>), dim3(grid1d((size_t)c.geo.n * c.geo.n, 256)), dim3(256), 0, c.stream,
+                       (CR*)c.un, c.geo, d_colz0);
+  }
+  static void scale(const Ctx& c, void* p, size_t cnt, double s) {
+    hipLaunchKernelGGL((k_scale<R>), dim3(grid1d(cnt, 256 * 4)), dim3(256), 0, c.stream, (CR*)p, cnt, s);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// dispatch on (precision, n)
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+static void run_pass(const Ctx& c, Pass p, int a0, int a1) {
+  switch (p) {
+    case Pass::YInvPhase: Launch<R, N>::y_inv(c, a0, 0); break;
+    case Pass::YInvKx: Launch<R, N>::y_inv(c, a0, 1); break;
+    case Pass::YInvPlain: Launch<R, N>::y_inv(c, a0, 2); break;
+    case Pass::ZInv: Launch<R, N>::z_inv(c, a0); break;
+    case Pass::XC2R: Launch<R, N>::x_c2r(c, a0); break;
+    case Pass::XR2C: Launch<R, N>::x_r2c(c, a0); break;
+    case Pass::XFused: Launch<R, N>::x_fused(c, a0); break;
+    case Pass::ZFwdConv: Launch<R, N>::z_fwd(c, 0, 5); break;
+    case Pass::ZFwdNorm: Launch<R, N>::z_fwd(c, 1, a0); break;
+    case Pass::YFwd: Launch<R, N>::y_fwd(c, a0); break;
+  }
+  (void)a1;
+}
+
+template <typename R>
+static int run_pass_n(const Ctx& c, Pass p, int a0, int a1) {
+  switch (c.geo.n) {
+    case 32: run_pass<R, 32>(c, p, a0, a1); return 0;
+    case 64: run_pass<R, 64>(c, p, a0, a1); return 0;
+    case 128: run_pass<R, 128>(c, p, a0, a1); return 0;
+    case 256: run_pass<R, 256>(c, p, a0, a1); return 0;
+    case 512: run_pass<R, 512>(c, p, a0, a1); return 0;
+    case 1024: run_pass<R, 1024>(c, p, a0, a1); return 0;
+  }
+  return -1;
+}
+
+int launch_pass(const Ctx& c, Pass p, int a0, int a1) {
+  return c.prec == 8 ? run_pass_n<double>(c, p, a0, a1) : run_pass_n<float>(c, p, a0, a1);
+}
+
+void launch_realspace(const Ctx& c, int want_velmax) {
+  if (c.prec == 8) LaunchP<double>::realspace(c, want_velmax);
+  else LaunchP<float>::realspace(c, want_velmax);
+}
+void launch_phshift(const Ctx& c, int nv) {
+  if (c.prec == 8) LaunchP<double>::phshift(c, nv);
+  else LaunchP<float>::phshift(c, nv);
+}
+void launch_wavespace(const Ctx& c, double norm) {
+  if (c.prec == 8) LaunchP<double>::wavespace(c, norm);
+  else LaunchP<float>::wavespace(c, norm);
+}
+void launch_update(const Ctx& c, int stage) {
+  if (c.prec == 8) LaunchP<double>::update(c, stage);
+  else LaunchP<float>::update(c, stage);
+}
+void launch_wavespace_update(const Ctx& c, double norm, int stage) {
+  if (c.prec == 8) LaunchP<double>::wavespace_update(c, norm, stage);
+  else LaunchP<float>::wavespace_update(c, norm, stage);
+}
+void launch_ek_shell(const Ctx& c, int kfor, double* d_ek) {
+  if (c.prec == 8) LaunchP<double>::ek_shell(c, kfor, d_ek);
+  else LaunchP<float>::ek_shell(c, kfor, d_ek);
+}
+void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio) {
+  if (c.prec == 8) LaunchP<double>::ek_scale(c, kf_shell, d_ratio);
+  else LaunchP<float>::ek_scale(c, kf_shell, d_ratio);
+}
+void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot) {
+  if (c.prec == 8) LaunchP<double>::spectrum(c, viscos, nshell, d_ek, d_dk, d_tot);
+  else LaunchP<float>::spectrum(c, viscos, nshell, d_ek, d_dk, d_tot);
+}
+void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0) {
+  if (c.prec == 8) LaunchP<double>::field(c, seed, kpeak, d_colz0);
+  else LaunchP<float>::field(c, seed, kpeak, d_colz0);
+}
+void launch_scale(const Ctx& c, void* p, size_t cnt, double s) {
+  if (c.prec == 8) LaunchP<double>::scale(c, p, cnt, s);
+  else LaunchP<float>::scale(c, p, cnt, s);
+}
+
+}  // namespace psdns

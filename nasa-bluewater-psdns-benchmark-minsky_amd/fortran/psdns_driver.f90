@@ -1,0 +1,198 @@
+! RK2 driver over the gfx950 C ABI, mirroring the reference time loop:
+!   main.f:392-685      loop, shifts, per-step timing line in `log`
+!   timestep_module.f   two substeps: nonlinear terms, step, update
+!   step.f:64-85        dt = cfl / velmax (velmax reduced in FP64)
+!   step.f:366-385      integrating factors xfac, yfac, zfac
+!   shifts.f:40-178     phase-shift offsets (kshift rules)
+!   ek_force.f          FEK forcing after the corrector
+!
+! usage: psdns_driver key=value ...
+!   n=64 prec=8 nsteps=20 cfl=3.14159265 viscos=0.01 kshift=1 kf_shell=3
+!   seed=122929298 ic=<file> out=<file> shifts=<file> log=<file>
+! ic/out: raw uny(ny,num_al,3) complex at precision prec (stream access)
+! shifts: raw gsh(3,2) per step, real(8), overriding the kshift rule
+program psdns_driver
+  use iso_c_binding
+  use psdns_gpu
+  implicit none
+
+  integer :: n = 64, prec = 8, nsteps = 10, kshift = 1, kf_shell = 0, istep, i, k, nargs
+  integer(c_long_long) :: seed = 122929298_c_long_long
+  real(c_double) :: cfl = 3.14159265358979d0, viscos = 0.01d0, velmax, velmax2, dt, time
+  real(c_double) :: gsh(3, 2), tke, eps, r3(3), stimes(6)
+  character(len=512) :: arg, key, val, ic_file = '', out_file = '', shift_file = '', log_file = 'log'
+  type(psdns_params) :: p
+  type(c_ptr) :: ctx
+  integer(c_int) :: num_al, max_al_x, kfor
+  real(c_double), allocatable :: xfac(:), yfac(:), zfac(:), ekinf(:), ekout(:), ek(:), dk(:)
+  complex(c_double_complex), allocatable, target :: u8(:, :, :)
+  complex(c_float_complex), allocatable, target :: u4(:, :, :)
+  integer(8) :: c0, c1, crate
+  real(8) :: tcpu, tmin, tmax, tsum
+  integer :: nt, shift_unit, seedsz
+  integer, allocatable :: rseed(:)
+  logical :: first
+
+  nargs = command_argument_count()
+  do i = 1, nargs
+    call get_command_argument(i, arg)
+    k = index(arg, '=')
+    if (k == 0) cycle
+    key = arg(1:k - 1)
+    val = arg(k + 1:)
+    select case (trim(key))
+    case ('n'); read (val, *) n
+    case ('prec'); read (val, *) prec
+    case ('nsteps'); read (val, *) nsteps
+    case ('cfl'); read (val, *) cfl
+    case ('viscos'); read (val, *) viscos
+    case ('kshift'); read (val, *) kshift
+    case ('kf_shell'); read (val, *) kf_shell
+    case ('seed'); read (val, *) seed
+    case ('ic'); ic_file = val
+    case ('out'); out_file = val
+    case ('shifts'); shift_file = val
+    case ('log'); log_file = val
+    end select
+  end do
+
+  p%nx = n; p%ny = n; p%nz = n; p%nc = 0; p%prec = prec; p%kshift = kshift; p%mask_kind = 0
+  p%viscos = viscos; p%pr = 1d0; p%grad = 0d0
+  call psdns_check(psdns_create(p, ctx), 'psdns_create')
+  call psdns_check(psdns_geometry(ctx, num_al, max_al_x), 'psdns_geometry')
+  write (6, '(a,i0,a,i0,a,i0,a,i0)') 'psdns_driver: n=', n, ' prec=', prec, ' num_al=', num_al, &
+      ' max_al_x=', max_al_x
+
+  ! initial condition: file (reference cylinder layout) or the device generator
+  if (len_trim(ic_file) > 0) then
+    if (prec == 8) then
+      allocate (u8(n, num_al, 3))
+      open (11, file=trim(ic_file), access='stream', form='unformatted', status='old')
+      read (11) u8
+      close (11)
+      call psdns_check(psdns_put_state(ctx, c_loc(u8), 0), 'psdns_put_state')
+    else
+      allocate (u4(n, num_al, 3))
+      open (11, file=trim(ic_file), access='stream', form='unformatted', status='old')
+      read (11) u4
+      close (11)
+      call psdns_check(psdns_put_state(ctx, c_loc(u4), 0), 'psdns_put_state')
+    end if
+  else
+    call psdns_check(psdns_init_field(ctx, seed, 3d0, 0.5d0), 'psdns_init_field')
+  end if
+
+  kfor = 0
+  if (kf_shell > 0) then
+    kfor = kf_shell + 1
+    allocate (ekinf(kfor), ekout(kfor))
+    ! target shell energies: those of the initial field (ek_init.f role)
+    call psdns_check(psdns_ek_shell(ctx, kfor, ekinf), 'psdns_ek_shell')
+  end if
+
+  allocate (xfac(n / 2), yfac(n), zfac(n))
+  shift_unit = 0
+  if (len_trim(shift_file) > 0) then
+    shift_unit = 12
+    open (shift_unit, file=trim(shift_file), access='stream', form='unformatted', status='old')
+  end if
+  call random_seed(size=seedsz)
+  allocate (rseed(seedsz))
+  rseed = int(seed) + 12345
+  call random_seed(put=rseed)
+  gsh = 0d0
+  if (kshift == 1 .or. kshift == 3) then
+    call random_number(r3)
+    gsh(:, 2) = r3
+  end if
+
+  open (7, file=trim(log_file), status='replace')
+  call psdns_check(psdns_stage_times(ctx, 1, stimes), 'psdns_stage_times')
+  time = 0d0
+  tmin = huge(1d0); tmax = 0d0; tsum = 0d0; nt = 0
+  call system_clock(count_rate=crate)
+  do istep = 1, nsteps
+    call system_clock(c0)
+    ! shifts.f:65-84
+    if (shift_unit /= 0) then
+      read (shift_unit) gsh
+    else
+      call random_number(r3)
+      do i = 1, 3
+        select case (kshift)
+        case (1)
+          gsh(i, 1) = mod(gsh(i, 2) + 0.5d0, 1d0); gsh(i, 2) = r3(i)
+        case (2)
+          gsh(i, 1) = r3(i); gsh(i, 2) = mod(gsh(i, 1) + 0.5d0, 1d0)
+        case (3)
+          gsh(i, 1) = mod(gsh(i, 2) + 0.5d0, 1d0); gsh(i, 2) = mod(gsh(i, 1) + 0.5d0, 1d0)
+        case default
+          gsh(i, 1) = 0d0; gsh(i, 2) = 0d0
+        end select
+      end do
+    end if
+    call psdns_check(psdns_set_shifts(ctx, gsh), 'psdns_set_shifts')
+
+    ! substep 1: nonlinear terms, step (dt, integrating factors), predictor
+    call psdns_check(psdns_stage_nonlinear(ctx, 1, velmax), 'psdns_stage_nonlinear(1)')
+    if (velmax < 1d-3) then
+      write (6, *) 'velmax gotten zero at istep=', istep, velmax
+      error stop 1
+    end if
+    dt = cfl / velmax
+    if (dt > 10d0 .or. dt < 1d-7) then
+      write (6, *) 'dt gotten out of control in step, dt=', dt
+      error stop 1
+    end if
+    do i = 1, n / 2
+      xfac(i) = exp(-viscos * dble(i - 1)**2 * dt)
+    end do
+    do i = 1, n
+      k = i - 1
+      if (i > n / 2 + 1) k = i - 1 - n
+      yfac(i) = exp(-viscos * dble(k)**2 * dt)
+      zfac(i) = yfac(i)
+    end do
+    call psdns_check(psdns_stage_update(ctx, 1, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(1)')
+
+    ! substep 2: corrector
+    call psdns_check(psdns_stage_nonlinear(ctx, 2, velmax2), 'psdns_stage_nonlinear(2)')
+    call psdns_check(psdns_stage_update(ctx, 2, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(2)')
+    if (kfor > 0) call psdns_check(psdns_ek_force(ctx, kf_shell, kfor, ekinf, ekout), 'psdns_ek_force')
+    time = time + dt
+
+    call system_clock(c1)
+    tcpu = dble(c1 - c0) / dble(crate)
+    ! main.f:651-657 (one task: min = ave = max)
+    write (7, '(a,i7,a,3f11.3,a)') 'cpu at step', istep, ' :min/ave/max=', tcpu, tcpu, tcpu, '  secs.'
+    write (6, '(a,i6,1p,4e13.5)') ' istep,dt,time,cfl,velmax=', istep, dt, time, cfl, velmax
+    if (istep > 1) then
+      tmin = min(tmin, tcpu); tmax = max(tmax, tcpu); tsum = tsum + tcpu; nt = nt + 1
+    end if
+  end do
+  close (7)
+  if (shift_unit /= 0) close (shift_unit)
+
+  allocate (ek(n), dk(n))
+  call psdns_check(psdns_shell_stats(ctx, ek, dk, n, tke, eps), 'psdns_shell_stats')
+  write (6, '(a,1p,2e22.14)') 'final tke, epsilon = ', tke, eps
+  if (nt > 0) write (6, '(a,3f12.6)') 's/step min/mean/max (steps 2..n) = ', tmin, tsum / nt, tmax
+  call psdns_check(psdns_stage_times(ctx, 0, stimes), 'psdns_stage_times')
+  write (6, '(a,6f10.3)') 'pass ms (y-inv z-inv x z-fwd y-fwd+upd total) = ', stimes
+
+  if (len_trim(out_file) > 0) then
+    open (13, file=trim(out_file), access='stream', form='unformatted', status='replace')
+    if (prec == 8) then
+      if (.not. allocated(u8)) allocate (u8(n, num_al, 3))
+      call psdns_check(psdns_get_state(ctx, c_loc(u8), 0), 'psdns_get_state')
+      write (13) u8
+    else
+      if (.not. allocated(u4)) allocate (u4(n, num_al, 3))
+      call psdns_check(psdns_get_state(ctx, c_loc(u4), 0), 'psdns_get_state')
+      write (13) u4
+    end if
+    close (13)
+  end if
+  call psdns_check(psdns_destroy(ctx), 'psdns_destroy')
+  first = .true.
+end program psdns_driver

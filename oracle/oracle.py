@@ -1,0 +1,267 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/psdns_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The GPU library never uses it.
+
+Arrays follow the reference layouts (0-based, C order):
+  cylinder  (nv, num_al, n)  complex128   == uy(ny, num_al, nv)
+  physical  (nv, n, n, n)    float64      == ux(nx, zisz, yjsz, nv)  [v][y][z][x]
+"""
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "libpsdns_oracle.so"
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not _LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(_HERE)] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIB_PATH))
+        L.orc_grid_create.restype = ctypes.c_void_p
+        L.orc_grid_create.argtypes = [ctypes.c_int]
+        L.orc_grid_destroy.argtypes = [ctypes.c_void_p]
+        for f in ("orc_num_al", "orc_max_al_x"):
+            getattr(L, f).restype = ctypes.c_int
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.orc_mask_count.restype = ctypes.c_long
+        L.orc_mask_count.argtypes = [ctypes.c_void_p]
+        L.orc_mask.restype = ctypes.c_int
+        L.orc_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_columns.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.orc_set_shifts.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_shift_rule.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.orc_fft_c2c.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_fft_r2c.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_fft_c2r.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_phshift_inplace.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_itransform_vel.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_realspace_vel.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+        L.orc_transform.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_wavespace_vel.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_double]
+        L.orc_step_factors.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_proc4a.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_proc4b.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_kxtran.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_xktran.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_ek_shell.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+        L.orc_ek_force.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_spectrum.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Grid:
+    """Oracle grid for an n^3 box (one task): cylinder, mask, shifts."""
+
+    def __init__(self, n):
+        self.n = n
+        self.L = lib()
+        self.h = self.L.orc_grid_create(n)
+        self.num_al = self.L.orc_num_al(self.h)
+        self.max_al_x = self.L.orc_max_al_x(self.h)
+        cx = np.zeros(self.num_al, np.int32)
+        cz = np.zeros(self.num_al, np.int32)
+        cut = np.zeros(self.max_al_x, np.int32)
+        self.L.orc_columns(self.h, cx.ctypes.data_as(_ip), cz.ctypes.data_as(_ip), cut.ctypes.data_as(_ip))
+        self.col_x, self.col_z, self.cut_z = cx, cz, cut
+
+    def __del__(self):
+        try:
+            self.L.orc_grid_destroy(self.h)
+        except Exception:
+            pass
+
+    # geometry
+    def mask_count(self):
+        return self.L.orc_mask_count(self.h)
+
+    def mask(self):
+        n = self.n
+        kx = self.col_x.astype(np.int64)[:, None]
+        kz = np.where(self.col_z > n // 2, self.col_z - n, self.col_z).astype(np.int64)[:, None]
+        y = np.arange(n)
+        ky = np.where(y > n // 2, y - n, y)[None, :]
+        m = (9 * (kx * kx + ky * ky + kz * kz) <= 2 * n * n) & (y[None, :] != n // 2) & (self.col_z[:, None] != n // 2)
+        return m
+
+    def set_shifts(self, gsh):
+        g = np.ascontiguousarray(np.asarray(gsh, np.float64).reshape(6))
+        self.L.orc_set_shifts(self.h, _p(g))
+
+    # fields
+    def zeros_cyl(self, nv):
+        return np.zeros((nv, self.num_al, self.n), np.complex128)
+
+    def zeros_phys(self, nv):
+        return np.zeros((nv, self.n, self.n, self.n), np.float64)
+
+    def field(self, seed=122929298, kpeak=3.0, tke=0.5):
+        u = self.zeros_cyl(3)
+        self.L.orc_field(self.h, _p(u), seed, kpeak, tke)
+        return u
+
+    # hot path
+    def phshift_inplace(self, uy, nv, kstep):
+        self.L.orc_phshift_inplace(self.h, _p(uy), nv, kstep)
+
+    def itransform_vel(self, uy):
+        ux = self.zeros_phys(5)
+        self.L.orc_itransform_vel(self.h, _p(np.ascontiguousarray(uy[:3])), _p(ux))
+        return ux
+
+    def realspace_vel(self, ux, rkstep):
+        vm = np.zeros(1)
+        self.L.orc_realspace_vel(self.h, _p(ux), rkstep, _p(vm))
+        return float(vm[0])
+
+    def transform(self, ux):
+        uy = self.zeros_cyl(5)
+        self.L.orc_transform(self.h, _p(ux), _p(uy))
+        return uy
+
+    def wavespace_vel(self, uy, kstep, dt2):
+        self.L.orc_wavespace_vel(self.h, _p(uy), kstep, dt2)
+
+    def step_factors(self, viscos, dt):
+        n = self.n
+        xf, yf, zf = np.zeros(n // 2), np.zeros(n), np.zeros(n)
+        self.L.orc_step_factors(self.h, viscos, dt, _p(xf), _p(yf), _p(zf))
+        return xf, yf, zf
+
+    def proc4a(self, uy, uny, xf, yf, zf):
+        self.L.orc_proc4a(self.h, _p(uy), _p(uny), _p(xf), _p(yf), _p(zf))
+
+    def proc4b(self, uy, uny):
+        self.L.orc_proc4b(self.h, _p(uy), _p(uny))
+
+    def kxtran(self, uy, nv):
+        ux = self.zeros_phys(nv)
+        self.L.orc_kxtran(self.h, _p(np.ascontiguousarray(uy[:nv])), _p(ux), nv)
+        return ux
+
+    def xktran(self, ux, nv):
+        uy = self.zeros_cyl(nv)
+        self.L.orc_xktran(self.h, _p(np.ascontiguousarray(ux[:nv])), _p(uy), nv)
+        return uy
+
+    def ek_shell(self, uny, kfor):
+        ek = np.zeros(kfor)
+        self.L.orc_ek_shell(self.h, _p(np.ascontiguousarray(uny[:3])), kfor, _p(ek))
+        return ek
+
+    def ek_force(self, uy, uny, kf_shell, kfor, ekinf):
+        e = np.ascontiguousarray(ekinf, np.float64)
+        self.L.orc_ek_force(self.h, _p(uy), _p(uny), kf_shell, kfor, _p(e))
+
+    def spectrum(self, uny, viscos, nshell=None):
+        nshell = nshell or self.n
+        ek, dk = np.zeros(nshell), np.zeros(nshell)
+        t, e = np.zeros(1), np.zeros(1)
+        self.L.orc_spectrum(self.h, _p(np.ascontiguousarray(uny[:3])), viscos, nshell, _p(ek), _p(dk), _p(t), _p(e))
+        return ek, dk, float(t[0]), float(e[0])
+
+
+def fft_c2c(x, sign):
+    a = np.ascontiguousarray(x, np.complex128).copy()
+    lib().orc_fft_c2c(_p(a), a.shape[-1], a.size // a.shape[-1], sign)
+    return a
+
+
+def fft_r2c(x):
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[-1]
+    out = np.zeros(x.shape[:-1] + (n // 2 + 1,), np.complex128)
+    lib().orc_fft_r2c(_p(x), _p(out), n, x.size // n)
+    return out
+
+
+def fft_c2r(X, n):
+    X = np.ascontiguousarray(X, np.complex128)
+    out = np.zeros(X.shape[:-1] + (n,), np.float64)
+    lib().orc_fft_c2r(_p(X), _p(out), n, X.size // X.shape[-1])
+    return out
+
+
+class Run:
+    """RK2 time loop on the oracle, mirroring main.f/timestep_module.f:
+    shifts -> [phase1, itransform_vel, realspace_vel, transform,
+    (stage 1: step -> dt, factors), wavespace_vel, proc4a | proc4b] x 2,
+    then FEK forcing (ek_force) after the corrector."""
+
+    def __init__(self, grid, uny, viscos, cfl, kshift=4, kf_shell=0, shift_seed=7, ekinf=None, dt_fixed=None):
+        self.g = grid
+        self.uny = np.ascontiguousarray(uny, np.complex128).copy()
+        self.uy = grid.zeros_cyl(5)
+        self.uy[:3] = self.uny
+        self.viscos, self.cfl, self.kshift = viscos, cfl, kshift
+        self.dt_fixed = dt_fixed
+        self.kf_shell = kf_shell
+        self.kfor = kf_shell + 1 if kf_shell > 0 else 0
+        self.ekinf = ekinf if ekinf is not None else (grid.ek_shell(self.uny, self.kfor) if self.kfor else None)
+        self.gsh = np.zeros(6)
+        self.rng = np.random.default_rng(shift_seed)
+        self.istep = 0
+        self.time = 0.0
+        self.dts = []
+        if kshift in (1, 3):
+            r = np.ascontiguousarray(self.rng.random(3))
+            lib().orc_shift_rule(kshift, _p(self.gsh), _p(r), 1)
+
+    def shift_sequence(self, nsteps):
+        """the gsh(3,2) values the next nsteps steps will use (consumes them)"""
+        return np.array([self.next_shifts() for _ in range(nsteps)])
+
+    def next_shifts(self):
+        r = np.ascontiguousarray(self.rng.random(3))
+        lib().orc_shift_rule(self.kshift, _p(self.gsh), _p(r), 0)
+        return self.gsh.copy()
+
+    def step(self, gsh=None):
+        g = self.g
+        g.set_shifts(self.next_shifts() if gsh is None else gsh)
+        for k in (1, 2):
+            g.phshift_inplace(self.uy, 3, k)
+            ux = g.itransform_vel(self.uy)
+            vm = g.realspace_vel(ux, k)
+            nl = g.transform(ux)
+            if k == 1:
+                self.velmax = vm
+                # step.f:77-78 (cfl > 0), else a constant dt
+                self.dt = self.cfl / vm if self.dt_fixed is None else self.dt_fixed
+                self.factors = g.step_factors(self.viscos, self.dt)
+            g.wavespace_vel(nl, k, self.dt / 2.0)
+            self.uy = nl
+            if k == 1:
+                g.proc4a(self.uy, self.uny, *self.factors)
+            else:
+                g.proc4b(self.uy, self.uny)
+        if self.kfor:
+            g.ek_force(self.uy, self.uny, self.kf_shell, self.kfor, self.ekinf)
+        self.istep += 1
+        self.time += self.dt
+        self.dts.append(self.dt)
+        return self.uny

@@ -1,0 +1,189 @@
+"""Multi-step GPU runs vs the FP64 oracle from the same seeded field, the
+reference's own run-time invariants, size-independent properties at 256^3,
+and the Fortran ISO_C_BINDING driver as a drop-in.
+
+Tolerances (relative L2 of the spectral velocity over the mask):
+  FP64 GPU vs FP64 oracle: 1e-11 after 20 steps at 64^3;
+  FP32 GPU vs FP64 oracle: 1e-5 after 10 steps (the stated target).
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import kgrid, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(*a):
+    print("[gpu case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(O, P, n, prec, nsteps, kshift=1, kf_shell=3, seed=122929298, viscos=0.01):
+    g = O.Grid(n)
+    u0 = g.field(seed=seed)
+    ro = O.Run(g, u0, viscos, cfl=np.pi, kshift=kshift, kf_shell=kf_shell, shift_seed=seed)
+    s = P.Solver(n, prec, viscos=viscos)
+    s.put_state(u0)
+    rg = P.Run(s, viscos, np.pi, kf_shell=kf_shell, ekinf=ro.ekinf)
+    errs = []
+    m = g.mask()[None]
+    for _ in range(nsteps):
+        gsh = ro.next_shifts()
+        ro.step(gsh)
+        rg.step(gsh)
+        errs.append(rel(s.get_state(), ro.uny, m))
+    return g, s, ro, rg, errs
+
+
+def test_fp64_20_steps_64(O, P):
+    banner("fp64 20 steps 64^3")
+    g, s, ro, rg, errs = run_pair(O, P, 64, 8, 20)
+    print("rel L2 per step:", ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-11
+    assert np.allclose(rg.dts, ro.dts, rtol=1e-12)
+    ek, dk, tke, eps = s.shell_stats(64)
+    rek, rdk, rtke, reps = g.spectrum(ro.uny, 0.01)
+    assert tke == pytest.approx(rtke, rel=1e-12)
+    assert eps == pytest.approx(reps, rel=1e-12)
+    s.close()
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_fp32_10_steps_vs_fp64_oracle(O, P, n):
+    banner("fp32 10 steps", n)
+    g, s, ro, rg, errs = run_pair(O, P, n, 4, 10)
+    print("rel L2 per step:", ["%.2e" % e for e in errs])
+    assert errs[-1] < 1e-5
+    ek, dk, tke, eps = s.shell_stats(n)
+    rek, rdk, rtke, reps = g.spectrum(ro.uny, 0.01)
+    assert tke == pytest.approx(rtke, rel=1e-5)
+    assert eps == pytest.approx(reps, rel=1e-5)
+    sel = rek > 1e-8 * rek.max()
+    assert np.max(np.abs(ek[sel] / rek[sel] - 1)) < 1e-4
+    s.close()
+
+
+def test_fek_forcing_on_gpu(O, P):
+    banner("fek forcing 64^3 fp64")
+    n = 64
+    s = P.Solver(n, 8, viscos=0.01)
+    s.init_field()
+    target = s.ek_shell(4)
+    rg = P.Run(s, 0.01, np.pi, kf_shell=3, ekinf=target)
+    for _ in range(3):
+        rg.step(np.array([0.1, 0.2, 0.3, 0.6, 0.7, 0.8]))
+        ek = s.ek_shell(4)
+        assert np.allclose(ek[1:3], target[1:3], rtol=1e-12)
+        assert ek[0] == 0.0
+    s.close()
+
+
+def test_reference_invariants_sinusoidal_ic(O, P):
+    # README.spp:101-105 on the GPU: TKE does not increase, rms u = rms v
+    banner("sinusoidal IC invariants 64^3 fp64")
+    n = 64
+    g = O.Grid(n)
+    x = np.arange(n) * 2 * np.pi / n
+    Y, Z, X = np.meshgrid(x, x, x, indexing="ij")
+    ux = np.stack([np.sin(X) * np.cos(Y) * np.cos(Z), np.cos(X) * np.sin(Y) * np.cos(Z),
+                   -2 * np.cos(X) * np.cos(Y) * np.sin(Z)])
+    u0 = g.xktran(ux, 3)
+    s = P.Solver(n, 8, viscos=0.01)
+    s.put_state(u0)
+    rg = P.Run(s, 0.01, np.pi)
+    prev = s.shell_stats(0)[2]
+    tf = np.where(g.col_x == 0, 1.0, 2.0)[:, None]
+    for _ in range(6):
+        rg.step(np.zeros(6))
+        t = s.shell_stats(0)[2]
+        assert t <= prev * (1 + 1e-13)
+        prev = t
+        un = s.get_state()
+        eu, ev = np.sum(tf * np.abs(un[0]) ** 2), np.sum(tf * np.abs(un[1]) ** 2)
+        assert abs(eu - ev) < 1e-12 * eu
+    s.close()
+
+
+def test_stokes_decay_beltrami_on_gpu(O, P):
+    banner("beltrami decay 64^3 fp64/fp32")
+    n, nu = 64, 0.05
+    g = O.Grid(n)
+    x = np.arange(n) * 2 * np.pi / n
+    Y, Z, X = np.meshgrid(x, x, x, indexing="ij")
+    ux = np.stack([np.sin(Z) + 0.4 * np.cos(Y), 0.7 * np.sin(X) + np.cos(Z), 0.4 * np.sin(Y) + 0.7 * np.cos(X)])
+    u0 = g.xktran(ux, 3)
+    for prec, tol in ((8, 1e-12), (4, 2e-6)):
+        s = P.Solver(n, prec, viscos=nu)
+        s.put_state(u0)
+        t = 0.0
+        for _ in range(5):
+            s.set_shifts(np.array([0.3, 0.1, 0.7, 0.8, 0.6, 0.2]))
+            dt = 0.02
+            xf, yf, zf = s.step_factors(dt)
+            s.stage_nonlinear(1)
+            s.stage_update(1, dt, xf, yf, zf)
+            s.stage_nonlinear(2)
+            s.stage_update(2, dt, xf, yf, zf)
+            t += dt
+        assert rel(s.get_state(), u0 * np.exp(-nu * t), g.mask()[None]) < tol
+        s.close()
+
+
+def test_size_independent_properties_256(O, P):
+    """256^3 FP32, 3 steps: divergence-free state, zero mean, Hermitian kx=0
+    plane (real physical field), TKE within the unforced decay budget."""
+    banner("256^3 fp32 properties")
+    n = 256
+    s = P.Solver(n, 4, viscos=0.01)
+    s.init_field()
+    cx, cz, _ = s.columns()
+    rg = P.Run(s, 0.01, np.pi)
+    t0 = s.shell_stats(0)[2]
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        rg.step(rng.random(6))
+    un = s.get_state().astype(np.complex128)
+    kx, ky, kz = kgrid(n, cx, cz)
+    div = kx * un[0] + ky * un[1] + kz * un[2]
+    kmag = np.sqrt(kx * kx + ky * ky + kz * kz)
+    assert np.abs(div).max() < 1e-5 * np.max(kmag * np.abs(un).max(axis=0))
+    assert np.abs(un[:, 0, 0]).max() == 0.0
+    # kx = 0, kz = 0 column: u(-ky) = conj u(ky) up to FP32 round-off
+    a0 = int(np.nonzero((cx == 0) & (cz == 0))[0][0])
+    col = un[:, a0, :]
+    mirror = np.conj(col[:, (n - np.arange(n)) % n])
+    assert np.abs(col - mirror).max() < 1e-5 * np.abs(col).max()
+    ek, dk, t1, eps = s.shell_stats(n)
+    assert 0 < t1 < t0
+    assert t0 - t1 < 1.2 * eps * sum(rg.dts) + 1e-3 * t0
+    s.close()
+
+
+def test_fortran_driver_matches_oracle(O, P, tmp_path):
+    banner("fortran driver 64^3 fp64 5 steps")
+    n, nsteps = 64, 5
+    g = O.Grid(n)
+    u0 = g.field(seed=5)
+    ro = O.Run(g, u0, 0.01, cfl=np.pi, kshift=1, kf_shell=3, shift_seed=5)
+    shifts = ro.shift_sequence(nsteps)
+    ro2 = O.Run(g, u0, 0.01, cfl=np.pi, kshift=1, kf_shell=3, shift_seed=5)
+    for k in range(nsteps):
+        ro2.step(shifts[k])
+    # Fortran column-major uy(ny, num_al, 3) == C-order (3, num_al, ny)
+    np.ascontiguousarray(u0, np.complex128).tofile(tmp_path / "ic.bin")
+    np.ascontiguousarray(shifts, np.float64).tofile(tmp_path / "shifts.bin")
+    cmd = [str(P.DRIVER_PATH), f"n={n}", "prec=8", f"nsteps={nsteps}", "cfl=3.14159265358979",
+           "viscos=0.01", "kshift=1", "kf_shell=3", f"ic={tmp_path / 'ic.bin'}",
+           f"shifts={tmp_path / 'shifts.bin'}", f"out={tmp_path / 'out.bin'}", f"log={tmp_path / 'log'}"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = np.fromfile(tmp_path / "out.bin", np.complex128).reshape(3, g.num_al, n)
+    assert rel(out, ro2.uny, g.mask()[None]) < 1e-11
+    log = (tmp_path / "log").read_text().splitlines()
+    assert len([l for l in log if l.startswith("cpu at step")]) == nsteps
