@@ -87,6 +87,24 @@ def cpu_baseline(n_target, budget_s=15.0):
     }
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of nbytes (the runtime's copy kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the pass kernels."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +113,12 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--prec", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)

@@ -14,8 +14,11 @@ import pathlib
 
 import numpy as np
 
+import os
+
 PKG = pathlib.Path(__file__).resolve().parent
-LIB_PATH = PKG / "lib" / "libpsdns_gpu.so"
+# PSDNS_GPU_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = pathlib.Path(os.environ.get("PSDNS_GPU_LIB", PKG / "lib" / "libpsdns_gpu.so"))
 DRIVER_PATH = PKG / "bin" / "psdns_driver"
 
 FIELD_U, FIELD_UN, FIELD_UX = 0, 1, 2

@@ -1,13 +1,20 @@
-// Line FFTs for gfx950: a power-of-two complex transform of length L done by
-// a team of T = L/E threads, E points per thread, Stockham auto-sort stages
-// exchanged through a padded structure-of-arrays LDS line.
+// Line FFTs for gfx950: B power-of-two complex transforms of length L per
+// work-group, each done by a team of T = L/E threads holding E points,
+// Stockham auto-sort stages exchanged through a padded structure-of-arrays
+// LDS line per transform.
 //
-// Register layout in and out ("coalesced order"): thread t of the team holds
-// points t + T*j, j = 0..E-1, so that a team reading or writing a contiguous
-// line touches consecutive addresses with consecutive lanes.  The first
-// Stockham stage (radix E) reads its inputs straight from that layout and
-// the last stage leaves its outputs in it, so a transform costs
-// (number of stages - 1) LDS round trips.
+// A thread works on one (line, q) pair per stage and holds points
+// q + T*j, j = 0..E-1, of that line.  Two mappings of the B*T threads are
+// used:
+//   natural     line = tid / T, q = tid % T   (lanes run along a line:
+//               coalesced when the line is contiguous in memory)
+//   transposed  line = tid % B, q = tid / B   (lanes run across lines:
+//               coalesced when lines are adjacent in memory)
+// The first Stockham stage (radix E) reads its inputs straight from the
+// registers of either mapping and the last stage leaves its outputs in the
+// registers of either mapping, or in LDS in natural order; so an FFT whose
+// input and output sit in memories with different fast axes does the
+// transpose inside the first or last stage instead of in an extra LDS pass.
 //
 // Sign convention (FFTW, epfftw.f): SIGN = -1 is FFTW_FORWARD (exp(-i...)),
 // SIGN = +1 is FFTW_BACKWARD; transforms are unnormalised.
@@ -40,7 +47,7 @@ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
   else return {a.y, -a.x};
 }
 
-// cos / sin of 2*pi*k/16, k = 0..15 (exact decimal expansions rounded by the
+// cos / sin of 2*pi*k/16, k = 0..15 (decimal expansions rounded by the
 // compiler to the target type)
 __device__ constexpr double kCos16[16] = {
     1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
@@ -119,13 +126,36 @@ struct StagePlan {
   static constexpr int kRad = kRem >= E ? E : kRem;
 };
 
-template <typename R, int L, int E, int SIGN>
-struct LineFft {
+// One thread's view of a transform: its line in LDS and its q.
+template <typename R>
+struct Slot {
+  R* re;
+  R* im;
+  int q;
+};
+
+// B lines of length L, E points per thread, LDS planes re [B][W], im [B][W]
+template <typename R, int L, int E, int B, int SIGN>
+struct TeamFft {
+  static_assert(L > E, "at least two Stockham stages");
   static constexpr int T = L / E;
+  static constexpr int NT = B * T;
   static constexpr int W = LdsLine<L>::kWords;
+  static constexpr int kLdsReals = 2 * B * W;
+
+  static __device__ __forceinline__ Slot<R> natural(R* lds, int tid) {
+    const int line = tid / T;
+    return {lds + line * W, lds + B * W + line * W, tid % T};
+  }
+  static __device__ __forceinline__ Slot<R> transposed(R* lds, int tid) {
+    const int line = tid % B;
+    return {lds + line * W, lds + B * W + line * W, tid / B};
+  }
+  static __device__ __forceinline__ int natural_line(int tid) { return tid / T; }
+  static __device__ __forceinline__ int transposed_line(int tid) { return tid % B; }
 
   // Twiddle exp(SIGN 2 pi i k / L) from the table tw[k] = exp(-2 pi i k / TL),
-  // TL = L * TSTRIDE (the table may be longer than this line).
+  // TL = L * TSTRIDE (the table may belong to a longer transform).
   template <int TSTRIDE>
   static __device__ __forceinline__ cpx<R> tw_get(const cpx<R>* __restrict__ tw, int k) {
     cpx<R> w = tw[k * TSTRIDE];
@@ -133,28 +163,29 @@ struct LineFft {
     return w;
   }
 
-  // one middle/last stage: radix RAD, span NS (product of earlier radices)
-  template <int RAD, int NS, int TSTRIDE, bool LAST>
-  static __device__ __forceinline__ void stage(cpx<R>* v, R* lre, R* lim,
-                                               const cpx<R>* __restrict__ tw, int t) {
+  // Stockham stage of radix RAD after span NS.  Reads the stage input from
+  // LDS line s.  OUT 0: results back to LDS (Stockham positions);
+  // OUT 1: final results into registers (v[m] = X[q + T m]).
+  template <int RAD, int NS, int TSTRIDE, int OUT>
+  static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* __restrict__ tw) {
     constexpr int NB = E / RAD;  // butterflies per thread
     cpx<R> w[E];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NB; i++) {
-      const int j = t + T * i;
+      const int j = s.q + T * i;
       const int jm = j & (NS - 1);
 #pragma unroll
       for (int r = 0; r < RAD; r++) {
         const int p = lpad(j + r * (L / RAD));
-        w[i * RAD + r] = {lre[p], lim[p]};
+        w[i * RAD + r] = {s.re[p], s.im[p]};
       }
 #pragma unroll
       for (int r = 1; r < RAD; r++)
         w[i * RAD + r] = cmul(w[i * RAD + r], tw_get<TSTRIDE>(tw, jm * r * (L / (NS * RAD))));
       Dft<R, RAD, SIGN>::run(w + i * RAD);
     }
-    if constexpr (LAST) {
+    if constexpr (OUT == 1) {
 #pragma unroll
       for (int i = 0; i < NB; i++)
 #pragma unroll
@@ -163,50 +194,53 @@ struct LineFft {
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < NB; i++) {
-        const int j = t + T * i;
+        const int j = s.q + T * i;
         const int jm = j & (NS - 1);
         const int base = (j / NS) * NS * RAD + jm;
 #pragma unroll
         for (int r = 0; r < RAD; r++) {
           const int p = lpad(base + r * NS);
-          lre[p] = w[i * RAD + r].x;
-          lim[p] = w[i * RAD + r].y;
+          s.re[p] = w[i * RAD + r].x;
+          s.im[p] = w[i * RAD + r].y;
         }
       }
     }
   }
 
-  template <int NS, int TSTRIDE>
-  static __device__ __forceinline__ void stages(cpx<R>* v, R* lre, R* lim,
-                                                const cpx<R>* __restrict__ tw, int t) {
+  template <int NS, int TSTRIDE, bool TO_LDS>
+  static __device__ __forceinline__ void stages(cpx<R>* v, Slot<R> mid, Slot<R> out,
+                                                const cpx<R>* __restrict__ tw) {
     if constexpr (NS < L) {
       constexpr int RAD = StagePlan<L, E, NS>::kRad;
       constexpr bool LAST = (NS * RAD == L);
-      stage<RAD, NS, TSTRIDE, LAST>(v, lre, lim, tw, t);
-      stages<NS * RAD, TSTRIDE>(v, lre, lim, tw, t);
+      if constexpr (!LAST) {
+        stage<RAD, NS, TSTRIDE, 0>(v, mid, tw);
+      } else if constexpr (TO_LDS) {
+        stage<RAD, NS, TSTRIDE, 0>(v, out, tw);
+      } else {
+        stage<RAD, NS, TSTRIDE, 1>(v, out, tw);
+      }
+      stages<NS * RAD, TSTRIDE, TO_LDS>(v, mid, out, tw);
     }
   }
 
-  // Transform v (coalesced order) in place.  lre/lim: this team's LDS line
-  // (W words each).  Every thread of the work-group must call it: it starts
-  // with a barrier (the LDS line may still be read by an earlier step) and
-  // leaves the LDS in use, so a caller writing LDS afterwards barriers first.
-  template <int TSTRIDE>
-  static __device__ __forceinline__ void run(cpx<R>* v, R* lre, R* lim,
-                                             const cpx<R>* __restrict__ tw, int t) {
+  // v holds points in.q + T*j of line `in`; on return either v holds
+  // X[out.q + T*m] of line `out` (TO_LDS = false) or the line in LDS holds
+  // X in natural order at lpad(k) (TO_LDS = true; read after a barrier).
+  // Middle stages use the natural mapping.  Every thread of the work-group
+  // calls it (barriers inside; the first one protects earlier LDS reads).
+  template <int TSTRIDE, bool TO_LDS = false>
+  static __device__ __forceinline__ void run(cpx<R>* v, R* lds, Slot<R> in, Slot<R> out,
+                                             const cpx<R>* __restrict__ tw) {
     Dft<R, E, SIGN>::run(v);
-    if constexpr (L == E) {
-      return;
-    } else {
-      __syncthreads();
+    __syncthreads();
 #pragma unroll
-      for (int r = 0; r < E; r++) {
-        const int p = lpad(t * E + r);
-        lre[p] = v[r].x;
-        lim[p] = v[r].y;
-      }
-      stages<E, TSTRIDE>(v, lre, lim, tw, t);
+    for (int r = 0; r < E; r++) {
+      const int p = lpad(in.q * E + r);
+      in.re[p] = v[r].x;
+      in.im[p] = v[r].y;
     }
+    stages<E, TSTRIDE, TO_LDS>(v, natural(lds, threadIdx.x), out, tw);
   }
 };
 

@@ -50,8 +50,11 @@ struct psdns_ctx {
   std::vector<int> colx, colz, cutz;
   // device allocations
   int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr;
-  void *d_u = nullptr, *d_un = nullptr, *d_w = nullptr, *d_ux = nullptr, *d_tw = nullptr;
+  int2* d_grp = nullptr;
+  int ngrp = 0;
+  void *d_u = nullptr, *d_un = nullptr, *d_w1 = nullptr, *d_xs = nullptr, *d_ux = nullptr, *d_tw = nullptr;
   double2* d_shift = nullptr;  // sx[2][nxh], sy[2][n], sz[2][n]
+  void* d_shiftr = nullptr;    // per stage: sx[nxh] | sy[n] | sz[n], precision prec
   double* d_fac = nullptr;     // xfac[nxh], yfac[n], zfac[n]
   double* d_red = nullptr;     // reductions scratch: [2 + 2*n + n]
   unsigned long long* d_velmax = nullptr;
@@ -190,6 +193,19 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   c->na = (int)c->colx.size();
   std::vector<int> colz0(n, -1);
   for (int a = 0; a < c->na && c->colx[a] == 0; a++) colz0[c->colz[a]] = a;
+  // y-pass column groups: consecutive columns of one kx with consecutive z
+  // inside one aligned block of yb z values
+  std::vector<int2> grp;
+  {
+    const int yb = psdns::y_group_lines(c->prec);
+    for (int a = 0; a < c->na; a++) {
+      const bool extend = !grp.empty() && c->colx[a] == c->colx[a - 1] && c->colz[a] == c->colz[a - 1] + 1 &&
+                          c->colz[a] / yb == c->colz[grp.back().x] / yb;
+      if (extend) grp.back().y++;
+      else grp.push_back(make_int2(a, 1));
+    }
+  }
+  c->ngrp = (int)grp.size();
 
   auto bail = [&](int rc) {
     psdns_destroy(c);
@@ -212,11 +228,15 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_colz, sizeof(int) * c->na);
   ALLOC(c->d_cutz, sizeof(int) * c->max_al_x);
   ALLOC(c->d_colz0, sizeof(int) * n);
+  ALLOC(c->d_grp, sizeof(int2) * c->ngrp);
   ALLOC(c->d_u, c->csize() * 5 * cyl);
   ALLOC(c->d_un, c->csize() * 3 * cyl);
-  ALLOC(c->d_w, c->csize() * 5 * (size_t)n * c->nxh * n);
+  const size_t w1_elems = (size_t)5 * c->max_al_x * n * n, xs_elems = (size_t)5 * n * n * c->nxh;
+  ALLOC(c->d_w1, c->csize() * w1_elems);
+  ALLOC(c->d_xs, c->csize() * xs_elems);
   ALLOC(c->d_tw, c->csize() * n);
   ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
+  ALLOC(c->d_shiftr, c->csize() * 2 * (c->nxh + 2 * n));
   ALLOC(c->d_fac, sizeof(double) * (c->nxh + 2 * n));
   ALLOC(c->d_red, sizeof(double) * (2 + 3 * (size_t)n + 64));
   ALLOC(c->d_velmax, sizeof(unsigned long long));
@@ -224,9 +244,11 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   CHK(hipMemcpy(c->d_colz, c->colz.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_cutz, c->cutz.data(), sizeof(int) * c->max_al_x, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_colz0, colz0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  CHK(hipMemcpy(c->d_grp, grp.data(), sizeof(int2) * c->ngrp, hipMemcpyHostToDevice));
   CHK(hipMemsetAsync(c->d_u, 0, c->csize() * 5 * cyl, c->c.stream));
   CHK(hipMemsetAsync(c->d_un, 0, c->csize() * 3 * cyl, c->c.stream));
-  CHK(hipMemsetAsync(c->d_w, 0, c->csize() * 5 * (size_t)n * c->nxh * n, c->c.stream));
+  CHK(hipMemsetAsync(c->d_w1, 0, c->csize() * w1_elems, c->c.stream));
+  CHK(hipMemsetAsync(c->d_xs, 0, c->csize() * xs_elems, c->c.stream));
   // twiddles exp(-2 pi i k / n) formed in FP64
   {
     std::vector<double> twd(2 * n);
@@ -244,10 +266,11 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   }
   Ctx& k = c->c;
   k.prec = c->prec;
-  k.geo = psdns::Geo{n, c->nxh, c->na, c->max_al_x, c->d_colx, c->d_colz, c->d_cutz};
+  k.geo = psdns::Geo{n, c->nxh, c->na, c->max_al_x, c->d_colx, c->d_colz, c->d_cutz, c->d_grp, c->ngrp};
   k.u = c->d_u;
   k.un = c->d_un;
-  k.w = c->d_w;
+  k.w1 = c->d_w1;
+  k.xs = c->d_xs;
   k.ux = nullptr;
   k.tw = c->d_tw;
   k.kstep = 0;
@@ -255,6 +278,7 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
     k.sx[s] = c->d_shift + s * c->nxh;
     k.sy[s] = c->d_shift + 2 * c->nxh + s * n;
     k.sz[s] = c->d_shift + 2 * c->nxh + 2 * n + s * n;
+    k.sr[s] = (char*)c->d_shiftr + c->csize() * s * (c->nxh + 2 * n);
   }
   k.xfac = c->d_fac;
   k.yfac = c->d_fac + c->nxh;
@@ -276,8 +300,8 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
 int psdns_destroy(psdns_ctx* c) {
   if (!c) return PSDNS_OK;
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
-  void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_colz0, c->d_u, c->d_un, c->d_w, c->d_ux,
-                  c->d_tw, c->d_shift, c->d_fac, c->d_red, c->d_velmax, c->d_force};
+  void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_colz0, c->d_grp, c->d_u, c->d_un, c->d_w1, c->d_xs, c->d_ux,
+                  c->d_tw,   c->d_shift, c->d_shiftr, c->d_fac, c->d_red, c->d_velmax, c->d_force};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -323,6 +347,24 @@ int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
     }
   }
   HIPCHK(hipMemcpyAsync(c->d_shift, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->c.stream));
+  // the same factors rounded to the state precision, stage-major sx | sy | sz
+  std::vector<double> hr(2 * h.size());
+  size_t o = 0;
+  for (int s = 0; s < 2; s++) {
+    const double2* parts[3] = {h.data() + s * nxh, h.data() + 2 * nxh + s * n, h.data() + 2 * nxh + 2 * n + s * n};
+    const int len[3] = {nxh, n, n};
+    for (int d = 0; d < 3; d++)
+      for (int i = 0; i < len[d]; i++) {
+        hr[o++] = parts[d][i].x;
+        hr[o++] = parts[d][i].y;
+      }
+  }
+  if (c->prec == 8) {
+    HIPCHK(hipMemcpyAsync(c->d_shiftr, hr.data(), sizeof(double) * hr.size(), hipMemcpyHostToDevice, c->c.stream));
+    return sync(c);
+  }
+  std::vector<float> hf(hr.begin(), hr.end());
+  HIPCHK(hipMemcpyAsync(c->d_shiftr, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, c->c.stream));
   return sync(c);
 }
 
@@ -389,10 +431,9 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
   }
   c->c.kstep = kstep - 1;
   if ((rc = mark(c, 5))) return rc;
-  if ((rc = pass(c, Pass::YFwd, 4))) return rc;
   const double dt2 = dt / 2.0;
   const double norm = dt2 / c->n / c->n;  // wavespace_vel.f:80
-  psdns::launch_wavespace_update(c->c, norm, kstep);
+  if (psdns::launch_y_update(c->c, kstep, norm) != 0) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   HIPCHK(hipGetLastError());
   if ((rc = mark(c, 6))) return rc;
   if ((rc = sync(c))) return rc;

@@ -11,13 +11,21 @@ namespace psdns {
 // Cylinder geometry on the device (mpisetup.f:311-354, one task)
 struct Geo {
   int n;         // nx = ny = nz
-  int nxh;       // n/2 = xisz (kx slots in the z-pencil array)
+  int nxh;       // n/2 = xisz (kx stride of the X work array)
   int na;        // num_al: cylinder columns
   int max_al_x;  // kx < max_al_x hold cylinder columns
   const int* colx;  // [na]   kx of column a
   const int* colz;  // [na]   z index of column a
   const int* cutz;  // [max_al_x]
+  // column groups of the y passes: consecutive columns of one kx whose z
+  // lie in one aligned block of y_group_lines(prec) entries, so that a group
+  // covers whole aligned runs of a z-pencil row; (first column, count)
+  const int2* grp;
+  int ngrp;
 };
+
+// columns per y-pass group (64-byte runs of complex values in W)
+inline int y_group_lines(int prec) { return prec == 8 ? 4 : 8; }
 
 // Everything a launcher needs; owned by psdns_ctx.
 struct Ctx {
@@ -26,13 +34,17 @@ struct Ctx {
   hipStream_t stream;
   void* u;   // cylinder, 5 slots  [5][na][n] complex
   void* un;  // cylinder, 3 slots  [3][na][n] complex
-  void* w;   // z-pencils, 5 slots [5][n][n/2][n] complex
+  void* w1;  // y<->z work, 5 slots [5][max_al_x][n(y)][n(z)] complex
+  void* xs;  // z<->x work, 5 slots [5][n(y)][n(z)][n/2(kx)] complex
   void* ux;  // physical, 5 slots  [5][n][n][n] real (per-routine path only)
   void* tw;  // twiddles exp(-2 pi i k / n), k < n, precision prec
   int kstep; // 0-based RK stage for the shift tables
   const double2* sx[2];
   const double2* sy[2];
   const double2* sz[2];
+  // the same shift factors rounded to the state precision (prec), used by
+  // the fused y-forward pass: sr[stage] = sx[nxh] | sy[n] | sz[n] complex
+  const void* sr[2];
   const double* xfac;
   const double* yfac;
   const double* zfac;
@@ -55,6 +67,8 @@ enum class Pass {
 };
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1);
+// fused y-forward + wavespace_vel + proc4a (stage 1) / proc4b (stage 2)
+int launch_y_update(const Ctx& c, int stage, double norm);
 void launch_phshift(const Ctx& c, int nv);
 void launch_realspace(const Ctx& c, int want_velmax);
 void launch_wavespace(const Ctx& c, double norm);

@@ -1,13 +1,20 @@
 // HIP kernels of the PSDNS RK2 hot path for gfx950 (MI355X).
 //
 // Device data (one variable = one slot; complex = interleaved re, im):
-//   cylinder  u[v][a][y]       reference uy(ny, num_al, nv)      (state)
-//   z-pencil  W[v][y][kx][z]   reference uz(nz, xisz, yjsz, nv)  (work)
-//   physical  ux[v][y][z][x]   reference ux(nx, zisz, yjsz, nv)  (only for
-//             the per-routine entry points; the fused step never writes it)
-// Only cylinder columns (kx < max_al_x, z outside the cut) are ever stored
-// in W by the inverse y pass and read by the forward y pass; z entries in
-// the cut are synthesised as zeros by the z pass instead of being stored.
+//   cylinder  u[v][a][y]        reference uy(ny, num_al, nv)  (state, y fastest)
+//   W1        w1[v][kx][y][z]   between the y and z passes (kx < max_al_x):
+//             a cylinder column's y-run stays inside one kx slab, the z
+//             lines are contiguous
+//   X         xs[v][y][z][kx]   between the z and x passes (kx < max_al_x
+//             used, stride n/2): the x lines are contiguous
+//   physical  ux[v][y][z][x]    reference ux(nx, zisz, yjsz, nv) (only for the
+//             per-routine entry points; the fused step keeps physical space
+//             in registers)
+// The reference's z-pencil array uz(nz, xisz, yjsz) (kxcomm1_clean.f:28) is
+// split into W1 and X so that every pass reads or writes one side along a
+// contiguous line and the other side in 64-byte runs inside a few pages.
+// Only cylinder columns (kx < max_al_x, z outside the cut) are stored in W1;
+// z entries in the cut are synthesised as zeros by the inverse z pass.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -21,21 +28,18 @@ template <typename R>
 using C = cpx<R>;
 
 // thread-team shape of a length-L line: E points per thread, T threads
-template <int L>
+template <int L, int EMAX = 16>
 struct Team {
-  static constexpr int E = L >= 1024 ? 16 : (L >= 512 ? 8 : 4);
+  static constexpr int E0 = L >= 1024 ? 16 : (L >= 512 ? 8 : 4);
+  static constexpr int E = E0 < EMAX ? E0 : EMAX;
   static constexpr int T = L / E;
 };
 
-// lines per work-group for passes that transpose through an LDS tile:
-// as many as fit about 80 KB of LDS, at most 16, at least one 256-thread WG
-template <typename R, int L>
-struct TileLines {
-  static constexpr int kLineBytes = 2 * LdsLine<L>::kWords * (int)sizeof(R);
-  static constexpr int kFit = 82000 / kLineBytes;
-  static constexpr int kCap = kFit >= 16 ? 16 : (kFit >= 8 ? 8 : (kFit >= 4 ? 4 : (kFit >= 2 ? 2 : 1)));
-  static constexpr int kMin = 256 / Team<L>::T;
-  static constexpr int B = kCap > kMin ? kCap : kMin;
+// lines per work-group of the passes with a transposed side: one aligned
+// 64-byte run of complex values
+template <typename R>
+struct RunLines {
+  static constexpr int B = sizeof(R) == 8 ? 4 : 8;
 };
 
 __device__ __forceinline__ int kwave(int i, int n) { return i <= n / 2 ? i : i - n; }
@@ -55,244 +59,236 @@ __device__ __forceinline__ C<R> to_c(double2 a) {
   return {(R)a.x, (R)a.y};
 }
 
-// W index of (v, y, kx, z)
-__device__ __forceinline__ size_t widx(int v, int y, int kx, int z, int n, int nxh) {
-  return (((size_t)v * n + y) * nxh + kx) * n + z;
+// Work-groups are handed to the 8 XCDs round-robin by linear block id, and
+// each XCD has its own L2.  Give every XCD a contiguous range of logical
+// blocks so that neighbouring tiles (which share cache lines and DRAM pages)
+// are worked on by the same L2.  Bijective for any grid size.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  const int q = nblocks / kXcd, r = nblocks % kXcd;
+  const int x = bid % kXcd, k = bid / kXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+// logical (block.x, block.y) of a 2-D grid after the XCD remap
+__device__ __forceinline__ int2 xcd_block2() {
+  const int n = gridDim.x * gridDim.y;
+  const int l = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, n);
+  return make_int2(l % gridDim.x, l / gridDim.x);
+}
+
+// W1 index of (v, kx, y, z) and X index of (v, y, z, kx)
+__device__ __forceinline__ size_t w1idx(int v, int kx, int y, int z, const Geo& g) {
+  return (((size_t)v * g.max_al_x + kx) * g.n + y) * g.n + z;
+}
+__device__ __forceinline__ size_t xidx(int v, int y, int z, int kx, const Geo& g) {
+  return (((size_t)v * g.n + y) * g.n + z) * g.nxh + kx;
 }
 
 // ---------------------------------------------------------------------------
-// Inverse y pass: cylinder -> z-pencils.
+// Inverse y pass: cylinder -> W1.
 // MODE 0: phase1 (phshift_inplace, homogeneous/phshift.f:71-135) fused on
-//         load: u *= sy(y) (sx(x) sz(z)) on the mask, else 0.  The shifted
-//         field is not written back: in the step the next writer of u is
-//         the forward transform (itransform_vel overwrites u in place).
+//         load: u *= sy(y) (sx(x) sz(z)) on the mask, else 0 (off-mask
+//         entries are not read).  The shifted field is not written back: in
+//         the step the next writer of u is the forward transform
+//         (itransform_vel overwrites u in place).
 // MODE 1: kxtran (kxtran.f:36): zero the y Nyquist row.
 // MODE 2: plain inverse transform (itransform_vel on an already shifted u).
-// Then y c2c +i (kxcomm1_clean.f:118) and scatter of the column into W
-// (kxcomm1_clean.f:205-255) through an LDS tile so stores run along z.
+// Then y c2c +i (kxcomm1_clean.f:118); loads run along y (natural mapping),
+// the last Stockham stage hands the outputs over in the transposed mapping so
+// that the scatter (kxcomm1_clean.f:205-255) writes 64-byte runs along z.
 // ---------------------------------------------------------------------------
 template <typename R, int N, int MODE>
-__global__ void __launch_bounds__((TileLines<R, N>::B * Team<N>::T))
-    k_y_inv(const C<R>* __restrict__ u, C<R>* __restrict__ W, Geo g, const double2* __restrict__ sx,
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
+    k_y_inv(const C<R>* __restrict__ u, C<R>* __restrict__ w1, Geo g, const double2* __restrict__ sx,
             const double2* __restrict__ sy, const double2* __restrict__ sz,
             const C<R>* __restrict__ tw) {
-  constexpr int T = Team<N>::T, E = Team<N>::E, B = TileLines<R, N>::B;
-  constexpr int WW = LdsLine<N>::kWords;
-  using F = LineFft<R, N, E, +1>;
-  __shared__ R lds[2 * B * WW];
-  const int line = threadIdx.x / T, t = threadIdx.x % T;
-  const int var = blockIdx.y;
-  const int a = blockIdx.x * B + line;
-  const bool act = a < g.na;
-  const int x = act ? g.colx[a] : 0, z = act ? g.colz[a] : 0;
-  const C<R>* col = u + ((size_t)var * g.na + (act ? a : 0)) * N;
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, +1>;
+  __shared__ R lds[F::kLdsReals];
+  const int tid = threadIdx.x;
+  const int2 blk = xcd_block2();
+  const int var = blk.y;
+  const int2 grp = g.grp[blk.x];
+  const Slot<R> in = F::natural(lds, tid);
+  const int li = F::natural_line(tid);
+  const bool act = li < grp.y;
+  const int a = grp.x + (act ? li : 0);
+  const int x = g.colx[a], z = g.colz[a];
+  const C<R>* col = u + ((size_t)var * g.na + a) * N;
   C<R> v[E];
   double2 sxz = make_double2(1.0, 0.0);
-  if (MODE == 0 && act) sxz = zmul(sx[x], sz[z]);
+  if (MODE == 0) sxz = zmul(sx[x], sz[z]);
 #pragma unroll
   for (int j = 0; j < E; j++) {
-    const int y = t + T * j;
-    C<R> val = act ? col[y] : C<R>{0, 0};
+    const int y = in.q + T * j;
+    C<R> val = {0, 0};
     if constexpr (MODE == 0) {
-      if (in_mask(x, kwave(y, N), kwave(z, N), y, z, N)) {
-        val = cmul(val, to_c<R>(zmul(sy[y], sxz)));
-      } else {
-        val = {0, 0};
-      }
+      if (act && in_mask(x, kwave(y, N), kwave(z, N), y, z, N)) val = cmul(col[y], to_c<R>(zmul(sy[y], sxz)));
     } else if constexpr (MODE == 1) {
-      if (y == N / 2) val = {0, 0};
+      if (act && y != N / 2) val = col[y];
+    } else {
+      if (act) val = col[y];
     }
     v[j] = val;
   }
-  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
-  __syncthreads();
-  R* tre = lds;
-  R* tim = lds + B * WW;
+  const Slot<R> out = F::transposed(lds, tid);
+  F::template run<1>(v, lds, in, out, tw);
+  const int lo = F::transposed_line(tid);
+  if (lo < grp.y) {
+    // a group's columns are consecutive z of one kx
+    C<R>* p = w1 + w1idx(var, g.colx[grp.x], 0, g.colz[grp.x] + lo, g);
 #pragma unroll
-  for (int j = 0; j < E; j++) {
-    const int p = lpad((t + T * j) * B + line);
-    tre[p] = v[j].x;
-    tim[p] = v[j].y;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < B * N; e += B * T) {
-    const int l = e % B, y = e / B;
-    const int aa = blockIdx.x * B + l;
-    if (aa < g.na) {
-      const int p = lpad(e);
-      W[widx(var, y, g.colx[aa], g.colz[aa], N, g.nxh)] = {tre[p], tim[p]};
-    }
+    for (int m = 0; m < E; m++) p[(size_t)(out.q + T * m) * N] = v[m];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): for kx < max_al_x
-// z c2c +i in place; z inside the cylinder cut (which always holds the z
-// Nyquist) is read as zero (kxcomm1_clean.f:238-240).  Lines beyond
-// max_al_x are zero and are never touched: the x pass reads them as zero.
+// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): W1 -> X.  A
+// work-group owns one y and B consecutive kx (< max_al_x); the z lines are
+// read along z (natural mapping) with the cylinder cut, which always holds
+// the z Nyquist, read as zero (kxcomm1_clean.f:238-240); z c2c +i; the last
+// stage hands over in the transposed mapping so that X is written in 64-byte
+// runs along kx.  kx >= max_al_x is zero and is never stored.
 // ---------------------------------------------------------------------------
 template <typename R, int N>
-__global__ void __launch_bounds__(256) k_z_inv(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw) {
-  constexpr int T = Team<N>::T, E = Team<N>::E, B = 256 / T;
-  constexpr int WW = LdsLine<N>::kWords;
-  using F = LineFft<R, N, E, +1>;
-  __shared__ R lds[2 * B * WW];
-  const int line = threadIdx.x / T, t = threadIdx.x % T;
-  const int var = blockIdx.y;
-  const int li = blockIdx.x * B + line;  // line = y * max_al_x + x
-  const bool act = li < N * g.max_al_x;
-  const int y = act ? li / g.max_al_x : 0, x = act ? li % g.max_al_x : 0;
-  const int c = g.cutz[x];
-  C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
+    k_z_inv(const C<R>* __restrict__ w1, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, +1>;
+  __shared__ R lds[F::kLdsReals];
+  const int tid = threadIdx.x;
+  const int2 blk = xcd_block2();
+  const int var = blk.y;
+  const int nkb = (g.max_al_x + B - 1) / B;
+  const int y = blk.x / nkb, kx0 = (blk.x % nkb) * B;
+  const Slot<R> in = F::natural(lds, tid);
+  const int kx = kx0 + F::natural_line(tid);
+  const bool act = kx < g.max_al_x;
+  const int c = g.cutz[act ? kx : 0];
+  const C<R>* p = w1 + w1idx(var, act ? kx : 0, y, 0, g);
   C<R> v[E];
 #pragma unroll
   for (int j = 0; j < E; j++) {
-    const int zz = t + T * j;
+    const int zz = in.q + T * j;
     const bool kept = zz < N / 2 - c || zz > N / 2 + c;
     v[j] = (act && kept) ? p[zz] : C<R>{0, 0};
   }
-  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
-  if (act) {
+  const Slot<R> out = F::transposed(lds, tid);
+  F::template run<1>(v, lds, in, out, tw);
+  const int ko = kx0 + F::transposed_line(tid);
+  if (ko < g.max_al_x) {
+    C<R>* q = xs + xidx(var, y, 0, ko, g);
 #pragma unroll
-    for (int j = 0; j < E; j++) p[t + T * j] = v[j];
+    for (int m = 0; m < E; m++) q[(size_t)(out.q + T * m) * g.nxh] = v[m];
   }
 }
 
 // ---------------------------------------------------------------------------
 // x pass building blocks (kxcomm2.f:278-333 inverse, xkcomm1.f:97,135
-// forward).  A work-group owns one y and B consecutive z (lines along kx,
-// stride n in W); LDS tile transposes kx-rows of B z values.
-// The real transform of length n runs as a complex transform of length
-// m = n/2 on packed pairs (x[2i], x[2i+1]).
+// forward) on contiguous X lines: a work-group owns one y and B consecutive
+// z, natural mapping throughout.  The real transform of length n runs as a
+// complex transform of length m = n/2 on packed pairs (x[2i], x[2i+1]).
 // ---------------------------------------------------------------------------
+#ifndef PSDNS_XPASS_E
+#define PSDNS_XPASS_E 8
+#endif
 template <typename R, int N>
 struct XPass {
   static constexpr int M = N / 2;
-  static constexpr int T = Team<M>::T, E = Team<M>::E;
-  static constexpr int B = TileLines<R, M>::B < N ? TileLines<R, M>::B : N;  // z per work-group
-  static constexpr int WW = LdsLine<M>::kWords;
+  static constexpr int T = Team<M, PSDNS_XPASS_E>::T, E = Team<M, PSDNS_XPASS_E>::E;
+  static constexpr int B0 = 256 / T > 0 ? 256 / T : 1;
+  static constexpr int B = B0 < N ? B0 : N;  // z lines per work-group
   static constexpr int NT = B * T;
+  using FI = TeamFft<R, M, E, B, +1>;
+  using FF = TeamFft<R, M, E, B, -1>;
+  static constexpr int kLdsReals = FI::kLdsReals;
 
-  // c2r of one W variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
-  // ignored).  Output: packed physical values in coalesced order.
-  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ W, int var, int y,
-                                             int z0, const Geo& g, R* lds, const C<R>* __restrict__ tw,
-                                             int line, int t) {
-    R* tre = lds;
-    R* tim = lds + B * WW;
-    __syncthreads();
-    for (int e = threadIdx.x; e < B * g.max_al_x; e += NT) {
-      const int l = e % B, kx = e / B;
-      const C<R> val = W[widx(var, y, kx, z0 + l, N, g.nxh)];
-      const int p = lpad(e);
-      tre[p] = val.x;
-      tim[p] = val.y;
-    }
-    __syncthreads();
+  // c2r of one X variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
+  // ignored): Z[k] = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k]).
+  // Output: packed physical values (x[2i], x[2i+1]) of this thread's line.
+  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xs, int var, int y, int z0,
+                                             const Geo& g, R* lds, const C<R>* __restrict__ tw) {
+    const int tid = threadIdx.x;
+    const Slot<R> s = FI::natural(lds, tid);
+    const C<R>* line = xs + xidx(var, y, z0, 0, g) + FI::natural_line(tid) * g.nxh;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = t + T * j;
+      const int k = s.q + T * j;
       const int km = M - k;
       C<R> xk = {0, 0}, xm = {0, 0};
-      if (k < g.max_al_x) {
-        const int p = lpad(k * B + line);
-        xk = {tre[p], tim[p]};
-      }
+      if (k < g.max_al_x) xk = line[k];
       if (k == 0) xk.y = 0;
-      if (km < g.max_al_x) {
-        const int p = lpad(km * B + line);
-        xm = {tre[p], tim[p]};
-      }
-      // Z = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k])
-      const C<R> s = {xk.x + xm.x, xk.y - xm.y};
+      if (km < g.max_al_x) xm = line[km];
+      const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
       C<R> w = tw[k];
       w.y = -w.y;
       const C<R> wd = cmul(w, d);
-      v[j] = {s.x - wd.y, s.y + wd.x};
+      v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
-    LineFft<R, M, E, +1>::template run<2>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
+    FI::template run<2>(v, lds, s, s, tw);
   }
 
-  // r2c of packed physical values v (coalesced order), scaled by `scale`,
-  // stored for kx < max_al_x into W variable var (kx Nyquist dropped).
-  static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ W, int var, int y, int z0,
-                                             const Geo& g, R* lds, const C<R>* __restrict__ tw,
-                                             int line, int t, R scale, bool act_store) {
-    LineFft<R, M, E, -1>::template run<2>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
-    R* lre = lds + line * WW;
-    R* lim = lds + B * WW + line * WW;
+  // r2c of packed physical values v of this thread's line, scaled by
+  // `scale`: the last stage leaves Z in LDS; X[k] = (Z[k] + conj Z[m-k]) / 2
+  // - i e^{-2 pi i k/n} (Z[k] - conj Z[m-k]) / 2 for kx < max_al_x (kx
+  // Nyquist dropped).
+  static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ xs, int var, int y, int z0,
+                                             const Geo& g, R* lds, const C<R>* __restrict__ tw, R scale) {
+    const int tid = threadIdx.x;
+    const Slot<R> s = FF::natural(lds, tid);
+    FF::template run<2, true>(v, lds, s, s, tw);
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const int p = lpad(t + T * j);
-      lre[p] = v[j].x;
-      lim[p] = v[j].y;
-    }
-    __syncthreads();
+    C<R>* line = xs + xidx(var, y, z0, 0, g) + FF::natural_line(tid) * g.nxh;
     const R half = (R)0.5 * scale;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = t + T * j;
-      const int p = lpad(k == 0 ? 0 : M - k);
-      const C<R> zm = {lre[p], -lim[p]};  // conj Z[m-k]
-      const C<R> zk = v[j];
-      const C<R> s = zk + zm;
-      const C<R> d = zk - zm;
-      // X = (s - i e^{-2 pi i k/n} d) / 2
-      const C<R> wd = cmul(tw[k], d);
-      v[j] = {(s.x + wd.y) * half, (s.y - wd.x) * half};
-    }
-    __syncthreads();
-    R* tre = lds;
-    R* tim = lds + B * WW;
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const int p = lpad((t + T * j) * B + line);
-      tre[p] = v[j].x;
-      tim[p] = v[j].y;
-    }
-    __syncthreads();
-    if (act_store) {
-      for (int e = threadIdx.x; e < B * g.max_al_x; e += NT) {
-        const int l = e % B, kx = e / B;
-        const int p = lpad(e);
-        W[widx(var, y, kx, z0 + l, N, g.nxh)] = {tre[p], tim[p]};
+      const int k = s.q + T * j;
+      if (k < g.max_al_x) {
+        const int pk = lpad(k), pm = lpad(k == 0 ? 0 : M - k);
+        const C<R> zk = {s.re[pk], s.im[pk]};
+        const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
+        const C<R> sm = zk + zm;
+        const C<R> d = zk - zm;
+        const C<R> wd = cmul(tw[k], d);
+        line[k] = {(sm.x + wd.y) * half, (sm.y - wd.x) * half};
       }
     }
   }
 };
 
-// inverse x pass alone: W (nv variables) -> physical ux
+// inverse x pass alone: X (nv variables) -> physical ux
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT))
-    k_x_c2r(const C<R>* __restrict__ W, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
+    k_x_c2r(const C<R>* __restrict__ xs, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
   using X = XPass<R, N>;
-  __shared__ R lds[2 * X::B * X::WW];
-  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
-  const int var = blockIdx.y;
-  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  __shared__ R lds[X::kLdsReals];
+  const int2 blk = xcd_block2();
+  const int var = blk.y;
+  const int y = blk.x / (N / X::B), z0 = (blk.x % (N / X::B)) * X::B;
   C<R> v[X::E];
-  X::c2r(v, W, var, y, z0, g, lds, tw, line, t);
+  X::c2r(v, xs, var, y, z0, g, lds, tw);
+  const int line = X::FI::natural_line(threadIdx.x), q = threadIdx.x % X::T;
   C<R>* dst = reinterpret_cast<C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
 #pragma unroll
-  for (int j = 0; j < X::E; j++) dst[t + X::T * j] = v[j];
+  for (int j = 0; j < X::E; j++) dst[q + X::T * j] = v[j];
 }
 
-// forward x pass alone: physical ux (nv variables) -> W, x 1/n
+// forward x pass alone: physical ux (nv variables) -> X, x 1/n
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT))
-    k_x_r2c(const R* __restrict__ ux, C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw) {
+    k_x_r2c(const R* __restrict__ ux, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw) {
   using X = XPass<R, N>;
-  __shared__ R lds[2 * X::B * X::WW];
-  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
-  const int var = blockIdx.y;
-  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  __shared__ R lds[X::kLdsReals];
+  const int2 blk = xcd_block2();
+  const int var = blk.y;
+  const int y = blk.x / (N / X::B), z0 = (blk.x % (N / X::B)) * X::B;
+  const int line = X::FF::natural_line(threadIdx.x), q = threadIdx.x % X::T;
   const C<R>* src = reinterpret_cast<const C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
   C<R> v[X::E];
 #pragma unroll
-  for (int j = 0; j < X::E; j++) v[j] = src[t + X::T * j];
-  X::r2c(v, W, var, y, z0, g, lds, tw, line, t, (R)1 / (R)N, true);
+  for (int j = 0; j < X::E; j++) v[j] = src[q + X::T * j];
+  X::r2c(v, xs, var, y, z0, g, lds, tw, (R)1 / (R)N);
 }
 
 __device__ __forceinline__ void velmax_commit(double m, unsigned long long* velmax) {
@@ -301,49 +297,51 @@ __device__ __forceinline__ void velmax_commit(double m, unsigned long long* velm
   if ((threadIdx.x & 63) == 0) atomicMax(velmax, (unsigned long long)__double_as_longlong(m));
 }
 
-// realspace_vel.f:105-113 on one packed pair; returns nx(|u|+|v|+|w|) max
+// realspace_vel.f:105-113 (b = 1): slot s of u^2-v^2, uv, w^2-v^2, vw, uw
 template <typename R>
-__device__ __forceinline__ double products(R u, R v, R w, R* p) {
-  const R bv2 = v * v;
-  p[4] = u * w;
-  p[3] = v * w;
-  p[2] = w * w - bv2;
-  p[1] = u * v;
-  p[0] = u * u - bv2;
-  return fabs((double)u) + fabs((double)v) + fabs((double)w);
+__device__ __forceinline__ R product(int s, R u, R v, R w) {
+  switch (s) {
+    case 0: return u * u - v * v;
+    case 1: return u * v;
+    case 2: return w * w - v * v;
+    case 3: return v * w;
+    default: return u * w;
+  }
 }
 
-// Fused x pass: c2r of u, v, w -> realspace_vel products (+ velmax) -> r2c
-// x 1/n of the five products, written back into W slots 0..4.  Physical
-// space stays in registers.
+// Fused x pass: c2r of u, v, w -> realspace_vel products (+ velmax, FP64,
+// realspace_vel.f:108-110) -> r2c x 1/n of the five products, written back
+// into X slots 0..4.  Physical space stays in registers.
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT))
-    k_x_fused(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw,
+    k_x_fused(C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
               unsigned long long* __restrict__ velmax, int want_velmax) {
   using X = XPass<R, N>;
   constexpr int E = X::E;
-  __shared__ R lds[2 * X::B * X::WW];
-  const int line = threadIdx.x / X::T, t = threadIdx.x % X::T;
-  const int y = blockIdx.x / (N / X::B), z0 = (blockIdx.x % (N / X::B)) * X::B;
+  __shared__ R lds[X::kLdsReals];
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const int y = bx / (N / X::B), z0 = (bx % (N / X::B)) * X::B;
   C<R> pu[E], pv[E], pw[E];
-  X::c2r(pu, W, 0, y, z0, g, lds, tw, line, t);
-  X::c2r(pv, W, 1, y, z0, g, lds, tw, line, t);
-  X::c2r(pw, W, 2, y, z0, g, lds, tw, line, t);
-  double vm = 0.0;
-  C<R> q[E];
-#pragma unroll
-  for (int s = 0; s < 5; s++) {
+  X::c2r(pu, xs, 0, y, z0, g, lds, tw);
+  X::c2r(pv, xs, 1, y, z0, g, lds, tw);
+  X::c2r(pw, xs, 2, y, z0, g, lds, tw);
+  if (want_velmax) {
+    double vm = 0.0;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      R pa[5], pb[5];
-      const double ma = products(pu[j].x, pv[j].x, pw[j].x, pa);
-      const double mb = products(pu[j].y, pv[j].y, pw[j].y, pb);
-      if (s == 0) vm = fmax(vm, fmax(ma, mb));
-      q[j] = {pa[s], pb[s]};
+      vm = fmax(vm, fabs((double)pu[j].x) + fabs((double)pv[j].x) + fabs((double)pw[j].x));
+      vm = fmax(vm, fabs((double)pu[j].y) + fabs((double)pv[j].y) + fabs((double)pw[j].y));
     }
-    X::r2c(q, W, s, y, z0, g, lds, tw, line, t, (R)1 / (R)N, true);
+    velmax_commit(vm * (double)N, velmax);
   }
-  if (want_velmax) velmax_commit(vm * (double)N, velmax);
+#pragma unroll 1
+  for (int s = 0; s < 5; s++) {
+    C<R> q[E];
+#pragma unroll
+    for (int j = 0; j < E; j++)
+      q[j] = {product(s, pu[j].x, pv[j].x, pw[j].x), product(s, pu[j].y, pv[j].y, pw[j].y)};
+    X::r2c(q, xs, s, y, z0, g, lds, tw, (R)1 / (R)N);
+  }
 }
 
 // realspace_vel alone on physical ux (slots 0..4)
@@ -353,11 +351,10 @@ __global__ void __launch_bounds__(256)
                 int want_velmax) {
   double vm = 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
-    R p[5];
-    const double m = products(ux[i], ux[n3 + i], ux[2 * n3 + i], p);
-    vm = fmax(vm, m);
+    const R u = ux[i], v = ux[n3 + i], w = ux[2 * n3 + i];
+    vm = fmax(vm, fabs((double)u) + fabs((double)v) + fabs((double)w));
 #pragma unroll
-    for (int s = 0; s < 5; s++) ux[s * n3 + i] = p[s];
+    for (int s = 0; s < 5; s++) ux[s * n3 + i] = product(s, u, v, w);
   }
   if (want_velmax) velmax_commit(vm * xnorm, velmax);
 }
@@ -365,117 +362,132 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Forward z pass (transform.f:129-138) and the convective combination
 // (transform.f:169-176,229-234): C1 = i kx P1 + i kz P5, C3 = i kx P5 + i kz P3
-// (z Nyquist skipped, it is outside the cylinder).  MODE 0: transform
-// (5 slots in, C1 P2 C3 P4 out); MODE 1: xktran (nv slots, x 1/(ny nz),
-// xktran.f:121,150-157).  Only cylinder z are stored.
+// (z Nyquist skipped, it is outside the cylinder).  X -> W1: a work-group
+// owns one y and B consecutive kx; the first stage gathers X in 64-byte runs
+// along kx (transposed mapping), the last stage leaves z lines in the
+// natural mapping for contiguous stores of the cylinder z.  MODE 0:
+// transform (5 slots in, C1 P2 C3 P4 out); MODE 1: xktran (nv slots,
+// x 1/(ny nz), xktran.f:121,150-157).
 // ---------------------------------------------------------------------------
+#ifndef PSDNS_ZFWD_E
+#define PSDNS_ZFWD_E 16
+#endif
 template <typename R, int N, int MODE>
-__global__ void __launch_bounds__(256)
-    k_z_fwd(C<R>* __restrict__ W, Geo g, const C<R>* __restrict__ tw, int nv) {
-  constexpr int T = Team<N>::T, E = Team<N>::E, B = 256 / T;
-  constexpr int WW = LdsLine<N>::kWords;
-  using F = LineFft<R, N, E, -1>;
-  __shared__ R lds[2 * B * WW];
-  const int line = threadIdx.x / T, t = threadIdx.x % T;
-  const int li = blockIdx.x * B + line;
-  const bool act = li < N * g.max_al_x;
-  const int y = act ? li / g.max_al_x : 0, x = act ? li % g.max_al_x : 0;
-  const int c = g.cutz[x];
-  R* lre = lds + line * WW;
-  R* lim = lds + B * WW + line * WW;
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
+    k_z_fwd(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw, int nv) {
+  constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, -1>;
+  __shared__ R lds[F::kLdsReals];
+  const int tid = threadIdx.x;
+  const int nkb = (g.max_al_x + B - 1) / B;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const int y = bx / nkb, kx0 = (bx % nkb) * B;
+  const Slot<R> in = F::transposed(lds, tid);
+  const Slot<R> out = F::natural(lds, tid);
+  const int ki = kx0 + F::transposed_line(tid);
+  const int kx = kx0 + F::natural_line(tid);
+  const bool act = kx < g.max_al_x;
+  const int c = g.cutz[act ? kx : 0];
   auto load = [&](C<R>* v, int var) {
-    const C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
+    if (ki < g.max_al_x) {
+      const C<R>* p = xs + xidx(var, y, 0, ki, g);
 #pragma unroll
-    for (int j = 0; j < E; j++) v[j] = act ? p[t + T * j] : C<R>{0, 0};
+      for (int r = 0; r < E; r++) v[r] = p[(size_t)(in.q + T * r) * g.nxh];
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; r++) v[r] = {0, 0};
+    }
   };
   auto store = [&](const C<R>* v, int var) {
-    C<R>* p = W + widx(var, y, x, 0, N, g.nxh);
     if (!act) return;
+    C<R>* p = w1 + w1idx(var, kx, y, 0, g);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int zz = t + T * j;
+      const int zz = out.q + T * j;
       if (zz < N / 2 - c || zz > N / 2 + c) p[zz] = v[j];
     }
   };
   if constexpr (MODE == 0) {
     C<R> p5[E], w[E];
     load(p5, 4);
-    F::template run<1>(p5, lre, lim, tw, t);
+    F::template run<1>(p5, lds, in, out, tw);
     load(w, 0);
-    F::template run<1>(w, lre, lim, tw, t);
+    F::template run<1>(w, lds, in, out, tw);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const R kz = (R)kwave(t + T * j, N), kx = (R)x;
-      const C<R> a = {kx * w[j].x + kz * p5[j].x, kx * w[j].y + kz * p5[j].y};
+      const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
+      const C<R> a = {kxr * w[j].x + kz * p5[j].x, kxr * w[j].y + kz * p5[j].y};
       w[j] = {-a.y, a.x};  // i (kx P1 + kz P5)
     }
     store(w, 0);
     load(w, 2);
-    F::template run<1>(w, lre, lim, tw, t);
+    F::template run<1>(w, lds, in, out, tw);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const R kz = (R)kwave(t + T * j, N), kx = (R)x;
-      const C<R> a = {kx * p5[j].x + kz * w[j].x, kx * p5[j].y + kz * w[j].y};
+      const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
+      const C<R> a = {kxr * p5[j].x + kz * w[j].x, kxr * p5[j].y + kz * w[j].y};
       w[j] = {-a.y, a.x};  // i (kx P5 + kz P3)
     }
     store(w, 2);
     load(w, 1);
-    F::template run<1>(w, lre, lim, tw, t);
+    F::template run<1>(w, lds, in, out, tw);
     store(w, 1);
     load(w, 3);
-    F::template run<1>(w, lre, lim, tw, t);
+    F::template run<1>(w, lds, in, out, tw);
     store(w, 3);
   } else {
-    const R s = (R)(1.0 / ((double)N * (double)N));
+    const R sc = (R)(1.0 / ((double)N * (double)N));
     for (int var = 0; var < nv; var++) {
       C<R> w[E];
       load(w, var);
-      F::template run<1>(w, lre, lim, tw, t);
+      F::template run<1>(w, lds, in, out, tw);
 #pragma unroll
-      for (int j = 0; j < E; j++) w[j] = cscale(w[j], s);
+      for (int j = 0; j < E; j++) w[j] = cscale(w[j], sc);
       store(w, var);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Forward y pass (xkcomm2.f:96-125 pack, xkcomm2.f:229 y c2c -i): gather the
-// cylinder columns from W through an LDS tile, transform, store contiguous.
+// Forward y pass (xkcomm2.f:96-125 pack, xkcomm2.f:229 y c2c -i): the first
+// stage gathers a group's cylinder columns from W1 in 64-byte runs along z
+// (transposed mapping), the last stage leaves them in the natural mapping
+// for contiguous stores along y.
 // ---------------------------------------------------------------------------
+template <typename R, int N, int EMAX>
+__device__ __forceinline__ void y_gather(C<R>* v, const C<R>* __restrict__ w1, int var, const Geo& g, int2 grp,
+                                         int l, int q) {
+  constexpr int T = Team<N, EMAX>::T, E = Team<N, EMAX>::E;
+  if (l < grp.y) {
+    const C<R>* p = w1 + w1idx(var, g.colx[grp.x], 0, g.colz[grp.x] + l, g);
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = p[(size_t)(q + T * r) * N];
+  } else {
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = {0, 0};
+  }
+}
+
 template <typename R, int N>
-__global__ void __launch_bounds__((TileLines<R, N>::B * Team<N>::T))
-    k_y_fwd(const C<R>* __restrict__ W, C<R>* __restrict__ u, Geo g, const C<R>* __restrict__ tw) {
-  constexpr int T = Team<N>::T, E = Team<N>::E, B = TileLines<R, N>::B;
-  constexpr int WW = LdsLine<N>::kWords;
-  using F = LineFft<R, N, E, -1>;
-  __shared__ R lds[2 * B * WW];
-  const int line = threadIdx.x / T, t = threadIdx.x % T;
-  const int var = blockIdx.y;
-  const int a = blockIdx.x * B + line;
-  R* tre = lds;
-  R* tim = lds + B * WW;
-  for (int e = threadIdx.x; e < B * N; e += B * T) {
-    const int l = e % B, y = e / B;
-    const int aa = blockIdx.x * B + l;
-    if (aa < g.na) {
-      const C<R> val = W[widx(var, y, g.colx[aa], g.colz[aa], N, g.nxh)];
-      const int p = lpad(e);
-      tre[p] = val.x;
-      tim[p] = val.y;
-    }
-  }
-  __syncthreads();
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
+    k_y_fwd(const C<R>* __restrict__ w1, C<R>* __restrict__ u, Geo g, const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, -1>;
+  __shared__ R lds[F::kLdsReals];
+  const int tid = threadIdx.x;
+  const int2 blk = xcd_block2();
+  const int var = blk.y;
+  const int2 grp = g.grp[blk.x];
+  const Slot<R> in = F::transposed(lds, tid);
   C<R> v[E];
+  y_gather<R, N, 16>(v, w1, var, g, grp, F::transposed_line(tid), in.q);
+  const Slot<R> out = F::natural(lds, tid);
+  F::template run<1>(v, lds, in, out, tw);
+  const int l = F::natural_line(tid);
+  if (l < grp.y) {
+    C<R>* col = u + ((size_t)var * g.na + grp.x + l) * N;
 #pragma unroll
-  for (int j = 0; j < E; j++) {
-    const int p = lpad((t + T * j) * B + line);
-    v[j] = {tre[p], tim[p]};
-  }
-  F::template run<1>(v, lds + line * WW, lds + B * WW + line * WW, tw, t);
-  if (a < g.na) {
-    C<R>* col = u + ((size_t)var * g.na + a) * N;
-#pragma unroll
-    for (int j = 0; j < E; j++) col[t + T * j] = v[j];
+    for (int j = 0; j < E; j++) col[out.q + T * j] = v[j];
   }
 }
 
@@ -635,6 +647,96 @@ __global__ void __launch_bounds__(256)
     for (int v = 0; v < 3; v++) {
       u[v * tot + i] = nl[v];
       un[v * tot + i] = o[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2):
+// gathers P2, C1, P4, C3 from W (transposed mapping), y c2c -i each
+// (xkcomm2.f:229), accumulates N1, N2, N3 (wavespace_vel.f:162-174) in
+// registers (natural mapping), then the integrating-factor update and
+// projection with un (proc4a.f:418-571, proc4b.f:54-205).  Writes u slots
+// 0..2 (zero off the mask, as wavespace_vel.f:171-173) and un on the mask.
+// Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n].
+// ---------------------------------------------------------------------------
+template <typename R, int N, int STAGE>
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
+    k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
+                   const C<R>* __restrict__ sr, const double* __restrict__ xfac, const double* __restrict__ yfac,
+                   const double* __restrict__ zfac, const C<R>* __restrict__ tw) {
+  constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, -1>;
+  __shared__ R lds[F::kLdsReals];
+  const int tid = threadIdx.x;
+  const Slot<R> in = F::transposed(lds, tid);
+  const Slot<R> out = F::natural(lds, tid);
+  const int2 grp = g.grp[xcd_block(blockIdx.x, gridDim.x)];
+  const int lin = F::transposed_line(tid);
+  const int ln = F::natural_line(tid);
+  const bool act = ln < grp.y;
+  const int a = grp.x + (act ? ln : 0);
+  const int x = g.colx[a], z = g.colz[a], kz = kwave(z, N);
+  // shift-free sums (wavespace_vel.f:165-170 with sxyz factored out):
+  // A1 = C1 + i ky P2, A2 = kx P2 + kz P4, A3 = C3 + i ky P4;
+  // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3
+  C<R> a1[E], a2[E], a3[E], v[E];
+  y_gather<R, N, 8>(v, w1, 1, g, grp, lin, in.q);  // P2
+  F::template run<1>(v, lds, in, out, tw);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const R ky = (R)kwave(out.q + T * m, N);
+    a1[m] = {-ky * v[m].y, ky * v[m].x};
+    a2[m] = cscale(v[m], (R)x);
+  }
+  y_gather<R, N, 8>(v, w1, 0, g, grp, lin, in.q);  // C1
+  F::template run<1>(v, lds, in, out, tw);
+#pragma unroll
+  for (int m = 0; m < E; m++) a1[m] = a1[m] + v[m];
+  y_gather<R, N, 8>(v, w1, 3, g, grp, lin, in.q);  // P4
+  F::template run<1>(v, lds, in, out, tw);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const R ky = (R)kwave(out.q + T * m, N);
+    a2[m] = a2[m] + cscale(v[m], (R)kz);
+    a3[m] = {-ky * v[m].y, ky * v[m].x};
+  }
+  y_gather<R, N, 8>(v, w1, 2, g, grp, lin, in.q);  // C3
+  F::template run<1>(v, lds, in, out, tw);
+#pragma unroll
+  for (int m = 0; m < E; m++) a3[m] = a3[m] + v[m];
+  const C<R>* sxr = sr;
+  const C<R>* syr = sr + g.nxh;
+  const C<R>* szr = sr + g.nxh + N;
+  const C<R> sxz = cmul(szr[z], sxr[x]);
+  const R rn = (R)norm;
+  if (!act) return;
+  const size_t tot = (size_t)g.na * N;
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    Pt p;
+    p.a = a;
+    p.y = out.q + T * m;
+    p.x = x;
+    p.z = z;
+    p.ky = kwave(p.y, N);
+    p.kz = kz;
+    p.m = in_mask(x, p.ky, kz, p.y, z, N);
+    const size_t i = (size_t)a * N + p.y;
+    if (!p.m) {
+#pragma unroll
+      for (int vv = 0; vv < 3; vv++) u[vv * tot + i] = {0, 0};
+      continue;
+    }
+    const C<R> sp = cmul(sxz, syr[p.y]);
+    const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
+    C<R> nl[3] = {cmul(sxyz, a1[m]), cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]), cmul(sxyz, a3[m])};
+    C<R> o[3] = {un[i], un[tot + i], un[2 * tot + i]};
+    update_point<R, STAGE>(p, nl, o, xfac, yfac, zfac);
+#pragma unroll
+    for (int vv = 0; vv < 3; vv++) {
+      u[vv * tot + i] = nl[vv];
+      un[vv * tot + i] = o[vv];
     }
   }
 }
@@ -814,54 +916,71 @@ template <typename R, int N>
 struct Launch {
   using CR = C<R>;
   static void y_inv(const Ctx& c, int nv, int mode) {
-    constexpr int B = TileLines<R, N>::B, T = Team<N>::T;
-    dim3 grid((c.geo.na + B - 1) / B, nv);
+    constexpr int B = RunLines<R>::B, T = Team<N>::T;
+    dim3 grid(c.geo.ngrp, nv);
+    const int k = mode == 0 ? c.kstep : 0;
     if (mode == 0)
-      hipLaunchKernelGGL((k_y_inv<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
-                         c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], (const CR*)c.tw);
+      hipLaunchKernelGGL((k_y_inv<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
+                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
     else if (mode == 1)
-      hipLaunchKernelGGL((k_y_inv<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
-                         c.sx[0], c.sy[0], c.sz[0], (const CR*)c.tw);
+      hipLaunchKernelGGL((k_y_inv<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
+                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
     else
-      hipLaunchKernelGGL((k_y_inv<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w, c.geo,
-                         c.sx[0], c.sy[0], c.sz[0], (const CR*)c.tw);
+      hipLaunchKernelGGL((k_y_inv<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
+                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
   }
   static void z_inv(const Ctx& c, int nv) {
-    constexpr int B = 256 / Team<N>::T;
-    dim3 grid((N * c.geo.max_al_x + B - 1) / B, nv);
-    hipLaunchKernelGGL((k_z_inv<R, N>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw);
+    constexpr int B = RunLines<R>::B, T = Team<N>::T;
+    const int nkb = (c.geo.max_al_x + B - 1) / B;
+    dim3 grid(N * nkb, nv);
+    hipLaunchKernelGGL((k_z_inv<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xs, c.geo,
+                       (const CR*)c.tw);
   }
   static void x_c2r(const Ctx& c, int nv) {
     using X = XPass<R, N>;
     dim3 grid(N * (N / X::B), nv);
-    hipLaunchKernelGGL((k_x_c2r<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.w, (R*)c.ux, c.geo,
+    hipLaunchKernelGGL((k_x_c2r<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.xs, (R*)c.ux, c.geo,
                        (const CR*)c.tw);
   }
   static void x_r2c(const Ctx& c, int nv) {
     using X = XPass<R, N>;
     dim3 grid(N * (N / X::B), nv);
-    hipLaunchKernelGGL((k_x_r2c<R, N>), grid, dim3(X::NT), 0, c.stream, (const R*)c.ux, (CR*)c.w, c.geo,
+    hipLaunchKernelGGL((k_x_r2c<R, N>), grid, dim3(X::NT), 0, c.stream, (const R*)c.ux, (CR*)c.xs, c.geo,
                        (const CR*)c.tw);
   }
   static void x_fused(const Ctx& c, int want_velmax) {
     using X = XPass<R, N>;
     dim3 grid(N * (N / X::B));
-    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw,
+    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (CR*)c.xs, c.geo, (const CR*)c.tw,
                        c.velmax_bits, want_velmax);
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
-    constexpr int B = 256 / Team<N>::T;
-    dim3 grid((N * c.geo.max_al_x + B - 1) / B);
+    constexpr int B = RunLines<R>::B, T = Team<N, PSDNS_ZFWD_E>::T;
+    const int nkb = (c.geo.max_al_x + B - 1) / B;
+    dim3 grid(N * nkb);
     if (mode == 0)
-      hipLaunchKernelGGL((k_z_fwd<R, N, 0>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw, nv);
+      hipLaunchKernelGGL((k_z_fwd<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
+                         (const CR*)c.tw, nv);
     else
-      hipLaunchKernelGGL((k_z_fwd<R, N, 1>), grid, dim3(256), 0, c.stream, (CR*)c.w, c.geo, (const CR*)c.tw, nv);
+      hipLaunchKernelGGL((k_z_fwd<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
+                         (const CR*)c.tw, nv);
   }
   static void y_fwd(const Ctx& c, int nv) {
-    constexpr int B = TileLines<R, N>::B, T = Team<N>::T;
-    dim3 grid((c.geo.na + B - 1) / B, nv);
-    hipLaunchKernelGGL((k_y_fwd<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w, (CR*)c.u, c.geo,
+    constexpr int B = RunLines<R>::B, T = Team<N>::T;
+    dim3 grid(c.geo.ngrp, nv);
+    hipLaunchKernelGGL((k_y_fwd<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u, c.geo,
                        (const CR*)c.tw);
+  }
+  static void y_update(const Ctx& c, int stage, double norm) {
+    constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
+    dim3 grid(c.geo.ngrp);
+    const CR* sr = (const CR*)c.sr[c.kstep];
+    if (stage == 1)
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
+                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw);
+    else
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
+                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw);
   }
 };
 
@@ -959,6 +1078,23 @@ static int run_pass_n(const Ctx& c, Pass p, int a0, int a1) {
     case 1024: run_pass<R, 1024>(c, p, a0, a1); return 0;
   }
   return -1;
+}
+
+template <typename R>
+static int y_update_n(const Ctx& c, int stage, double norm) {
+  switch (c.geo.n) {
+    case 32: Launch<R, 32>::y_update(c, stage, norm); return 0;
+    case 64: Launch<R, 64>::y_update(c, stage, norm); return 0;
+    case 128: Launch<R, 128>::y_update(c, stage, norm); return 0;
+    case 256: Launch<R, 256>::y_update(c, stage, norm); return 0;
+    case 512: Launch<R, 512>::y_update(c, stage, norm); return 0;
+    case 1024: Launch<R, 1024>::y_update(c, stage, norm); return 0;
+  }
+  return -1;
+}
+
+int launch_y_update(const Ctx& c, int stage, double norm) {
+  return c.prec == 8 ? y_update_n<double>(c, stage, norm) : y_update_n<float>(c, stage, norm);
 }
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1) {
