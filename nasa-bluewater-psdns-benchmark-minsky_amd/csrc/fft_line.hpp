@@ -109,8 +109,9 @@ struct Dft<R, 1, SIGN> {
   static __device__ __forceinline__ void run(cpx<R>*) {}
 };
 
-// LDS line: real and imaginary planes, one pad word every 32 entries.
-// kWords also bounds lpad() of a B-line tile: B * kWords >= lpad(B * L - 1) + 1.
+// LDS line: real and imaginary planes.  lpad (one pad word every 32
+// entries) is the layout of plain LDS tiles; TeamFft pads each Stockham
+// buffer for the stage that writes it (see TeamFft::pos).
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
 template <int L>
 struct LdsLine {
@@ -134,14 +135,55 @@ struct Slot {
   int q;
 };
 
-// B lines of length L, E points per thread, LDS planes re [B][W], im [B][W]
-template <typename R, int L, int E, int B, int SIGN>
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// B lines of length L, E points per thread, LDS planes re [B][S], im [B][S].
+// Padding of the buffer written by the stage of span NSW (product of the
+// earlier radices), which writes chunks of NSW consecutive points with chunk
+// stride NSW*RAD:  stage 0 (NSW = 1, stride-E writes): one pad word every
+// 32; NSW < 32: NSW pad words every NSW*RAD points, so that the chunks of a
+// wave start on different banks; NSW >= 32: none.  SX extra words per line
+// stride shift the banks of neighbouring lines for the transposed mapping.
+template <typename R, int L, int E, int B, int SIGN, int SX = 0>
 struct TeamFft {
   static_assert(L > E, "at least two Stockham stages");
   static constexpr int T = L / E;
   static constexpr int NT = B * T;
-  static constexpr int W = LdsLine<L>::kWords;
+
+  template <int NSW>
+  static __device__ __forceinline__ constexpr int pos(int i) {
+    constexpr int RAD = StagePlan<L, E, NSW>::kRad;
+    if constexpr (NSW == 1) {
+      return i + (i >> 5);
+    } else if constexpr (NSW < 32) {
+      return i + (i >> ilog2(NSW * RAD)) * NSW;
+    } else {
+      return i;
+    }
+  }
+  template <int NSW>
+  static constexpr int words() {
+    constexpr int RAD = StagePlan<L, E, NSW>::kRad;
+    constexpr int last = NSW == 1 ? (L - 1) + ((L - 1) >> 5)
+                                  : (NSW < 32 ? (L - 1) + ((L - 1) >> ilog2(NSW * RAD)) * NSW : L - 1);
+    if constexpr (NSW * RAD < L) {
+      constexpr int rest = words<NSW * RAD>();
+      return (last + 1) > rest ? (last + 1) : rest;
+    } else {
+      return last + 1;
+    }
+  }
+  static constexpr int W = words<1>() + SX;  // line stride (words)
   static constexpr int kLdsReals = 2 * B * W;
+  // span of the last stage: its buffer holds the result for run<..., true>
+  template <int NS>
+  static constexpr int last_span() {
+    constexpr int RAD = StagePlan<L, E, NS>::kRad;
+    if constexpr (NS * RAD == L) return NS;
+    else return last_span<NS * RAD>();
+  }
+  // LDS position of output point k after run<TSTRIDE, true>
+  static __device__ __forceinline__ int out_pos(int k) { return pos<last_span<1>()>(k); }
 
   static __device__ __forceinline__ Slot<R> natural(R* lds, int tid) {
     const int line = tid / T;
@@ -177,7 +219,7 @@ struct TeamFft {
       const int jm = j & (NS - 1);
 #pragma unroll
       for (int r = 0; r < RAD; r++) {
-        const int p = lpad(j + r * (L / RAD));
+        const int p = pos<NS / E>(j + r * (L / RAD));  // buffer written by the stage of span NS/E
         w[i * RAD + r] = {s.re[p], s.im[p]};
       }
 #pragma unroll
@@ -199,7 +241,7 @@ struct TeamFft {
         const int base = (j / NS) * NS * RAD + jm;
 #pragma unroll
         for (int r = 0; r < RAD; r++) {
-          const int p = lpad(base + r * NS);
+          const int p = pos<NS>(base + r * NS);
           s.re[p] = w[i * RAD + r].x;
           s.im[p] = w[i * RAD + r].y;
         }
@@ -226,7 +268,7 @@ struct TeamFft {
 
   // v holds points in.q + T*j of line `in`; on return either v holds
   // X[out.q + T*m] of line `out` (TO_LDS = false) or the line in LDS holds
-  // X in natural order at lpad(k) (TO_LDS = true; read after a barrier).
+  // X[k] at out_pos(k) (TO_LDS = true; read after a barrier).
   // Middle stages use the natural mapping.  Every thread of the work-group
   // calls it (barriers inside; the first one protects earlier LDS reads).
   template <int TSTRIDE, bool TO_LDS = false>
@@ -236,7 +278,7 @@ struct TeamFft {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < E; r++) {
-      const int p = lpad(in.q * E + r);
+      const int p = pos<1>(in.q * E + r);
       in.re[p] = v[r].x;
       in.im[p] = v[r].y;
     }

@@ -42,6 +42,15 @@ struct RunLines {
   static constexpr int B = sizeof(R) == 8 ? 4 : 8;
 };
 
+// extra LDS words per line (bank offset between lines) for a transposed
+// last stage (OUT) or a transposed first stage (IN), from a bank-conflict
+// model of the stage access patterns for 32 and 64 banks
+template <int E, int B>
+struct LineShift {
+  static constexpr int OUT = B >= 8 ? 4 : 8;
+  static constexpr int IN = B >= 8 ? (E >= 16 ? 2 : 1) : 4;
+};
+
 __device__ __forceinline__ int kwave(int i, int n) { return i <= n / 2 ? i : i - n; }
 
 // masks.f:67-77 in exact integer form: (k/n)^2 summed <= 2/9 <=> 9 k^2 <= 2 n^2
@@ -103,7 +112,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
             const double2* __restrict__ sy, const double2* __restrict__ sz,
             const C<R>* __restrict__ tw) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, +1>;
+  using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
   __shared__ R lds[F::kLdsReals];
   const int tid = threadIdx.x;
   const int2 blk = xcd_block2();
@@ -154,7 +163,7 @@ template <typename R, int N>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
     k_z_inv(const C<R>* __restrict__ w1, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, +1>;
+  using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
   __shared__ R lds[F::kLdsReals];
   const int tid = threadIdx.x;
   const int2 blk = xcd_block2();
@@ -245,7 +254,7 @@ struct XPass {
     for (int j = 0; j < E; j++) {
       const int k = s.q + T * j;
       if (k < g.max_al_x) {
-        const int pk = lpad(k), pm = lpad(k == 0 ? 0 : M - k);
+        const int pk = FF::out_pos(k), pm = FF::out_pos(k == 0 ? 0 : M - k);
         const C<R> zk = {s.re[pk], s.im[pk]};
         const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
         const C<R> sm = zk + zm;
@@ -376,7 +385,7 @@ template <typename R, int N, int MODE>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
     k_z_fwd(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw, int nv) {
   constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, -1>;
+  using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
   const int tid = threadIdx.x;
   const int nkb = (g.max_al_x + B - 1) / B;
@@ -472,7 +481,7 @@ template <typename R, int N>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
     k_y_fwd(const C<R>* __restrict__ w1, C<R>* __restrict__ u, Geo g, const C<R>* __restrict__ tw) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, -1>;
+  using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
   const int tid = threadIdx.x;
   const int2 blk = xcd_block2();
@@ -666,7 +675,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
                    const C<R>* __restrict__ sr, const double* __restrict__ xfac, const double* __restrict__ yfac,
                    const double* __restrict__ zfac, const C<R>* __restrict__ tw) {
   constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, -1>;
+  using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
   const int tid = threadIdx.x;
   const Slot<R> in = F::transposed(lds, tid);
