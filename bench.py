@@ -176,10 +176,13 @@ def main():
     dom_ms = times[names.index(dominant)] / launches
     achieved = nb[dominant] / (dom_ms * 1e-3) / 1e9
 
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE counter
+    # passes over this same workload (scaled by a plain copy's readings)
     traffic = None
-    tfile = REPO / "profiles" / f"traffic_{dominant}_{n}_fp{8 * s_bytes}.json"
+    tfile = REPO / "profiles" / f"traffic_{n}_fp{8 * s_bytes}.json"
     if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        kern = json.loads(tfile.read_text()).get("kernels", {})
+        traffic = kern.get(dominant, {}).get("bytes_per_launch")
 
     line = {
         "metric": "s/RK2 step & FFT HBM-roofline % at 1024^3 FP32; LES-closure speedup vs DNS",
