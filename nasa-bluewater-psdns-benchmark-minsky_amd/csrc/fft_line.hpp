@@ -182,7 +182,7 @@ struct TeamFft {
     if constexpr (NS * RAD == L) return NS;
     else return last_span<NS * RAD>();
   }
-  // LDS position of output point k after run<TSTRIDE, true>
+  // LDS position of output point k after run<true>
   static __device__ __forceinline__ int out_pos(int k) { return pos<last_span<1>()>(k); }
 
   static __device__ __forceinline__ Slot<R> natural(R* lds, int tid) {
@@ -196,11 +196,59 @@ struct TeamFft {
   static __device__ __forceinline__ int natural_line(int tid) { return tid / T; }
   static __device__ __forceinline__ int transposed_line(int tid) { return tid % B; }
 
-  // Twiddle exp(SIGN 2 pi i k / L) from the table tw[k] = exp(-2 pi i k / TL),
-  // TL = L * TSTRIDE (the table may belong to a longer transform).
+  // Twiddles come from a table in LDS that depends on (L, E) only, so that
+  // the transforms of both signs share it: entries hold exp(-2 pi i m / L)
+  // and SIGN = +1 reads the conjugate (the negation folds into the multiply).
+  // For each stage after the first (span NS, radix RAD) the table holds
+  // (RAD-1) x NS entries, entry (r-1) NS + jm = exp(-2 pi i jm r / (NS RAD)),
+  // so that the lanes of a stage read consecutive entries.  A radix-2 stage
+  // with NS >= 512 keeps a two-level table of NS/16 + 16 entries instead:
+  // exp(-2 pi i jm / (2 NS)) = A[jm / 16] B[jm % 16].
+  static constexpr int rad_at(int ns) { return L / ns >= E ? E : L / ns; }
+  static constexpr bool two_level(int ns) { return ns >= 512; }
+  static constexpr int tw_entries(int ns) { return two_level(ns) ? ns / 16 + 16 : (rad_at(ns) - 1) * ns; }
+  static constexpr int tw_off(int ns) {
+    int off = 0;
+    for (int s = E; s < ns; s *= rad_at(s)) off += tw_entries(s);
+    return off;
+  }
+  static constexpr int kTwEntries = tw_off(L);
+
+  // Fill the LDS table from tw[k] = exp(-2 pi i k / (L TSTRIDE)), k < L
+  // TSTRIDE (a table that may belong to a longer transform); all threads
+  // tid < nthr of the work-group take part.  Readers must pass a barrier
+  // first (every run() does before its first twiddle).
   template <int TSTRIDE>
-  static __device__ __forceinline__ cpx<R> tw_get(const cpx<R>* __restrict__ tw, int k) {
-    cpx<R> w = tw[k * TSTRIDE];
+  static __device__ __forceinline__ void fill_twiddles(cpx<R>* stw, const cpx<R>* __restrict__ tw, int tid,
+                                                       int nthr) {
+    for (int e = tid; e < kTwEntries; e += nthr) {
+      int s = E, off = 0;
+      while (e >= off + tw_entries(s)) {
+        off += tw_entries(s);
+        s *= rad_at(s);
+      }
+      const int k = e - off, rad = rad_at(s);
+      int m;
+      if (two_level(s)) {
+        m = (k < s / 16 ? 16 * k : k - s / 16) * (L / (2 * s));
+      } else {
+        m = (k % s) * (k / s + 1) * (L / (s * rad));
+      }
+      stw[e] = tw[m * TSTRIDE];
+    }
+  }
+
+  // twiddle exp(SIGN 2 pi i jm r / (NS RAD)) of the stage of span NS
+  template <int NS, int RAD>
+  static __device__ __forceinline__ cpx<R> tw_at(const cpx<R>* stw, int jm, int r) {
+    cpx<R> w;
+    if constexpr (two_level(NS)) {
+      static_assert(RAD == 2, "two-level twiddles only for a radix-2 stage");
+      constexpr int off = tw_off(NS);
+      w = cmul(stw[off + (jm >> 4)], stw[off + NS / 16 + (jm & 15)]);
+    } else {
+      w = stw[tw_off(NS) + (r - 1) * NS + jm];
+    }
     if constexpr (SIGN > 0) w.y = -w.y;
     return w;
   }
@@ -208,8 +256,8 @@ struct TeamFft {
   // Stockham stage of radix RAD after span NS.  Reads the stage input from
   // LDS line s.  OUT 0: results back to LDS (Stockham positions);
   // OUT 1: final results into registers (v[m] = X[q + T m]).
-  template <int RAD, int NS, int TSTRIDE, int OUT>
-  static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* __restrict__ tw) {
+  template <int RAD, int NS, int OUT>
+  static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* stw) {
     constexpr int NB = E / RAD;  // butterflies per thread
     cpx<R> w[E];
     __syncthreads();
@@ -224,7 +272,7 @@ struct TeamFft {
       }
 #pragma unroll
       for (int r = 1; r < RAD; r++)
-        w[i * RAD + r] = cmul(w[i * RAD + r], tw_get<TSTRIDE>(tw, jm * r * (L / (NS * RAD))));
+        w[i * RAD + r] = cmul(w[i * RAD + r], tw_at<NS, RAD>(stw, jm, r));
       Dft<R, RAD, SIGN>::run(w + i * RAD);
     }
     if constexpr (OUT == 1) {
@@ -249,20 +297,19 @@ struct TeamFft {
     }
   }
 
-  template <int NS, int TSTRIDE, bool TO_LDS>
-  static __device__ __forceinline__ void stages(cpx<R>* v, Slot<R> mid, Slot<R> out,
-                                                const cpx<R>* __restrict__ tw) {
+  template <int NS, bool TO_LDS>
+  static __device__ __forceinline__ void stages(cpx<R>* v, Slot<R> mid, Slot<R> out, const cpx<R>* stw) {
     if constexpr (NS < L) {
       constexpr int RAD = StagePlan<L, E, NS>::kRad;
       constexpr bool LAST = (NS * RAD == L);
       if constexpr (!LAST) {
-        stage<RAD, NS, TSTRIDE, 0>(v, mid, tw);
+        stage<RAD, NS, 0>(v, mid, stw);
       } else if constexpr (TO_LDS) {
-        stage<RAD, NS, TSTRIDE, 0>(v, out, tw);
+        stage<RAD, NS, 0>(v, out, stw);
       } else {
-        stage<RAD, NS, TSTRIDE, 1>(v, out, tw);
+        stage<RAD, NS, 1>(v, out, stw);
       }
-      stages<NS * RAD, TSTRIDE, TO_LDS>(v, mid, out, tw);
+      stages<NS * RAD, TO_LDS>(v, mid, out, stw);
     }
   }
 
@@ -271,9 +318,18 @@ struct TeamFft {
   // X[k] at out_pos(k) (TO_LDS = true; read after a barrier).
   // Middle stages use the natural mapping.  Every thread of the work-group
   // calls it (barriers inside; the first one protects earlier LDS reads).
-  template <int TSTRIDE, bool TO_LDS = false>
-  static __device__ __forceinline__ void run(cpx<R>* v, R* lds, Slot<R> in, Slot<R> out,
-                                             const cpx<R>* __restrict__ tw) {
+  //
+  // The thread's slot coordinates pass through an empty asm statement, so
+  // that in a kernel looping over tiles the LDS addresses and twiddle loads
+  // of the stages are formed inside each call instead of being hoisted out
+  // of the loop and held in registers across it.
+  // stw is the LDS twiddle table (fill_twiddles).
+  template <bool TO_LDS = false>
+  static __device__ __forceinline__ void run(cpx<R>* v, R* lds, Slot<R> in, Slot<R> out, const cpx<R>* stw) {
+    int oi = (int)(in.re - lds), qi = in.q, oo = (int)(out.re - lds), qo = out.q, tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(oi), "+v"(qi), "+v"(oo), "+v"(qo), "+v"(tid));
+    in = {lds + oi, lds + oi + B * W, qi};
+    out = {lds + oo, lds + oo + B * W, qo};
     Dft<R, E, SIGN>::run(v);
     __syncthreads();
 #pragma unroll
@@ -282,7 +338,7 @@ struct TeamFft {
       in.re[p] = v[r].x;
       in.im[p] = v[r].y;
     }
-    stages<E, TSTRIDE, TO_LDS>(v, natural(lds, threadIdx.x), out, tw);
+    stages<E, TO_LDS>(v, natural(lds, tid), out, stw);
   }
 };
 

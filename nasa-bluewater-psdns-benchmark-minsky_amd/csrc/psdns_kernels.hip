@@ -114,6 +114,8 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
   __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   const int tid = threadIdx.x;
   const int2 blk = xcd_block2();
   const int var = blk.y;
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
     v[j] = val;
   }
   const Slot<R> out = F::transposed(lds, tid);
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
   const int lo = F::transposed_line(tid);
   if (lo < grp.y) {
     // a group's columns are consecutive z of one kx
@@ -151,45 +153,159 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
   }
 }
 
+// Persistent work-groups: the grid holds as many work-groups as fit on the
+// device at once (a multiple of 8), and each walks a list of tiles.  XCD x
+// (work-groups with blockIdx.x % 8 == x) owns one contiguous range of tiles,
+// which its work-groups take in turn, so tiles that share cache lines and
+// DRAM pages go through one L2 at about the same time.  The tile loop is
+// unrolled by two with two register buffers: the loads of the next tile are
+// issued before the transform of the current one, so that every work-group
+// keeps its next tile's reads in flight during its LDS stages.
+struct TileWalk {
+  int t, end, step;
+};
+__device__ __forceinline__ TileWalk tile_walk(int ntiles) {
+  constexpr int kXcd = 8;
+  const int x = blockIdx.x % kXcd, wpx = gridDim.x / kXcd;
+  const int q = ntiles / kXcd, r = ntiles % kXcd;
+  const int start = x * q + (x < r ? x : r);
+  return {start + (int)blockIdx.x / kXcd, start + q + (x < r ? 1 : 0), wpx};
+}
+
+// Run load(buf, t, valid) / finish(buf, t) over the work-group's tiles with
+// the next tile's loads issued before finish of the current one.  load is
+// called with valid = false past the last tile and must then issue the same
+// instructions with every lane out of range (no branch around the loads).
+template <typename Buf, typename Load, typename Finish>
+__device__ __forceinline__ void tile_loop(int ntiles, Buf& ba, Buf& bb, Load&& load, Finish&& finish) {
+  const TileWalk w = tile_walk(ntiles);
+  int t = w.t;
+  if (t >= w.end) return;
+  load(ba, t, true);
+  for (;;) {
+    const int t2 = t + w.step;
+    load(bb, t2 < w.end ? t2 : t, t2 < w.end);
+    finish(ba, t);
+    if (t2 >= w.end) break;
+    t = t2 + w.step;
+    load(ba, t < w.end ? t : t2, t < w.end);
+    finish(bb, t2);
+    if (t >= w.end) break;
+  }
+}
+
+template <typename R, int E>
+struct Regs {
+  C<R> v[E];
+};
+
+// An opaque copy of a per-lane value: offsets formed from it inside a tile
+// loop are recomputed per tile rather than hoisted and held in registers.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// base + a 32-bit byte offset: with a uniform base the compiler addresses
+// through a scalar base register and one 32-bit offset register per lane
+template <typename T>
+__device__ __forceinline__ T* at_bytes(T* base, unsigned off) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ const T* at_bytes(const T* base, unsigned off) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
+// Raw buffer access (a base and a 32-bit byte offset per lane).  A lane
+// whose offset is kOob, or every lane of a view with no records, reads zeros
+// and writes nothing without touching memory, while the instruction is still
+// issued: the number of memory operations in flight then does not depend on
+// the data, so a tile loop can wait for exactly the loads it needs.
+constexpr unsigned kOob = 0x80000000u;
+template <typename R>
+struct BufView {
+  __amdgpu_buffer_rsrc_t r;
+  // bytes: 0 (no records: every access out of range) or kOob (offsets below
+  // kOob in range)
+  __device__ __forceinline__ BufView(const void* base, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ C<R> load(unsigned off) const {
+    if constexpr (sizeof(R) == 4) {
+      const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+      return {__uint_as_float(d[0]), __uint_as_float(d[1])};
+    } else {
+      const auto d = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+      return {__hiloint2double((int)d[1], (int)d[0]), __hiloint2double((int)d[3], (int)d[2])};
+    }
+  }
+  __device__ __forceinline__ void store(unsigned off, C<R> v) const {
+    if constexpr (sizeof(R) == 4) {
+      using V2 = unsigned __attribute__((ext_vector_type(2)));
+      V2 d = {__float_as_uint(v.x), __float_as_uint(v.y)};
+      __builtin_amdgcn_raw_buffer_store_b64(d, r, off, 0, 0);
+    } else {
+      using V4 = unsigned __attribute__((ext_vector_type(4)));
+      const unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
+      const unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
+      V4 d = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+      __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
-// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): W1 -> X.  A
-// work-group owns one y and B consecutive kx (< max_al_x); the z lines are
+// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): W1 -> X.  A tile
+// is one variable, one y and B consecutive kx (< max_al_x); the z lines are
 // read along z (natural mapping) with the cylinder cut, which always holds
 // the z Nyquist, read as zero (kxcomm1_clean.f:238-240); z c2c +i; the last
 // stage hands over in the transposed mapping so that X is written in 64-byte
 // runs along kx.  kx >= max_al_x is zero and is never stored.
 // ---------------------------------------------------------------------------
 template <typename R, int N>
-__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
-    k_z_inv(const C<R>* __restrict__ w1, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw) {
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_z_inv(const C<R>* __restrict__ w1, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw, int nv) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
   __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  __shared__ int scut[N / 2];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  for (int k = threadIdx.x; k < g.max_al_x; k += F::NT) scut[k] = g.cutz[k];
   const int tid = threadIdx.x;
-  const int2 blk = xcd_block2();
-  const int var = blk.y;
   const int nkb = (g.max_al_x + B - 1) / B;
-  const int y = blk.x / nkb, kx0 = (blk.x % nkb) * B;
   const Slot<R> in = F::natural(lds, tid);
-  const int kx = kx0 + F::natural_line(tid);
-  const bool act = kx < g.max_al_x;
-  const int c = g.cutz[act ? kx : 0];
-  const C<R>* p = w1 + w1idx(var, act ? kx : 0, y, 0, g);
-  C<R> v[E];
-#pragma unroll
-  for (int j = 0; j < E; j++) {
-    const int zz = in.q + T * j;
-    const bool kept = zz < N / 2 - c || zz > N / 2 + c;
-    v[j] = (act && kept) ? p[zz] : C<R>{0, 0};
-  }
   const Slot<R> out = F::transposed(lds, tid);
-  F::template run<1>(v, lds, in, out, tw);
-  const int ko = kx0 + F::transposed_line(tid);
-  if (ko < g.max_al_x) {
-    C<R>* q = xs + xidx(var, y, 0, ko, g);
+  const int li0 = F::natural_line(tid), lo0 = F::transposed_line(tid);
+  __syncthreads();
+  // tile t = (var * N + y) * nkb + kx block; per-tile bases are uniform,
+  // per-lane offsets are 32-bit byte offsets
+  auto load = [&](Regs<R, E>& b, int t, bool valid) {
+    const int var = t / (N * nkb), y = (t / nkb) % N, kx0 = (t % nkb) * B;
+    const BufView<R> src(w1 + w1idx(var, kx0, y, 0, g), valid ? kOob : 0u);
+    const int li = opaque(li0), q = opaque(in.q);
+    const int kx = kx0 + li;
+    const bool act = kx < g.max_al_x;
+    const int c = scut[act ? kx : 0];
 #pragma unroll
-    for (int m = 0; m < E; m++) q[(size_t)(out.q + T * m) * g.nxh] = v[m];
-  }
+    for (int j = 0; j < E; j++) {
+      const int zz = q + T * j;
+      const bool kept = act && (zz < N / 2 - c || zz > N / 2 + c);
+      b.v[j] = src.load(kept ? ((unsigned)li * N * N + zz) * (unsigned)sizeof(C<R>) : kOob);
+    }
+  };
+  auto finish = [&](Regs<R, E>& b, int t) {
+    F::template run<>(b.v, lds, in, out, stw);
+    const int var = t / (N * nkb), y = (t / nkb) % N, kx0 = (t % nkb) * B;
+    const int lo = opaque(lo0), q = opaque(out.q);
+    const BufView<R> dst(xs + xidx(var, y, 0, kx0, g), kOob);
+    const bool act = kx0 + lo < g.max_al_x;
+#pragma unroll
+    for (int m = 0; m < E; m++)
+      dst.store(act ? ((unsigned)(q + T * m) * g.nxh + lo) * (unsigned)sizeof(C<R>) : kOob, b.v[m]);
+  };
+  Regs<R, E> ba, bb;
+  tile_loop(nv * N * nkb, ba, bb, load, finish);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,12 +327,23 @@ struct XPass {
   using FI = TeamFft<R, M, E, B, +1>;
   using FF = TeamFft<R, M, E, B, -1>;
   static constexpr int kLdsReals = FI::kLdsReals;
+  static constexpr int kTw = FI::kTwEntries;  // shared by FI and FF
+
+  // LDS tables of a work-group: stage twiddles (from tw of length N, every
+  // second entry) and xtw[k] = exp(-2 pi i k / N), k < M, for the real-data
+  // pre/post-processing (read before the first barrier of a transform, so
+  // fill ends with one).
+  static __device__ __forceinline__ void fill(C<R>* stw, C<R>* xtw, const C<R>* __restrict__ tw) {
+    FI::template fill_twiddles<2>(stw, tw, threadIdx.x, NT);
+    for (int k = threadIdx.x; k < M; k += NT) xtw[k] = tw[k];
+    __syncthreads();
+  }
 
   // c2r of one X variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
   // ignored): Z[k] = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k]).
   // Output: packed physical values (x[2i], x[2i+1]) of this thread's line.
   static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xs, int var, int y, int z0,
-                                             const Geo& g, R* lds, const C<R>* __restrict__ tw) {
+                                             const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
     const int tid = threadIdx.x;
     const Slot<R> s = FI::natural(lds, tid);
     const C<R>* line = xs + xidx(var, y, z0, 0, g) + FI::natural_line(tid) * g.nxh;
@@ -230,12 +357,12 @@ struct XPass {
       if (km < g.max_al_x) xm = line[km];
       const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
-      C<R> w = tw[k];
+      C<R> w = xtw[k];
       w.y = -w.y;
       const C<R> wd = cmul(w, d);
       v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
-    FI::template run<2>(v, lds, s, s, tw);
+    FI::template run<>(v, lds, s, s, stw);
   }
 
   // r2c of packed physical values v of this thread's line, scaled by
@@ -243,10 +370,10 @@ struct XPass {
   // - i e^{-2 pi i k/n} (Z[k] - conj Z[m-k]) / 2 for kx < max_al_x (kx
   // Nyquist dropped).
   static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ xs, int var, int y, int z0,
-                                             const Geo& g, R* lds, const C<R>* __restrict__ tw, R scale) {
+                                             const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw, R scale) {
     const int tid = threadIdx.x;
     const Slot<R> s = FF::natural(lds, tid);
-    FF::template run<2, true>(v, lds, s, s, tw);
+    FF::template run<true>(v, lds, s, s, stw);
     __syncthreads();
     C<R>* line = xs + xidx(var, y, z0, 0, g) + FF::natural_line(tid) * g.nxh;
     const R half = (R)0.5 * scale;
@@ -259,7 +386,7 @@ struct XPass {
         const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
         const C<R> sm = zk + zm;
         const C<R> d = zk - zm;
-        const C<R> wd = cmul(tw[k], d);
+        const C<R> wd = cmul(xtw[k], d);
         line[k] = {(sm.x + wd.y) * half, (sm.y - wd.x) * half};
       }
     }
@@ -272,11 +399,13 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
     k_x_c2r(const C<R>* __restrict__ xs, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
   using X = XPass<R, N>;
   __shared__ R lds[X::kLdsReals];
+  __shared__ C<R> stw[X::kTw], xtw[X::M];
+  X::fill(stw, xtw, tw);
   const int2 blk = xcd_block2();
   const int var = blk.y;
   const int y = blk.x / (N / X::B), z0 = (blk.x % (N / X::B)) * X::B;
   C<R> v[X::E];
-  X::c2r(v, xs, var, y, z0, g, lds, tw);
+  X::c2r(v, xs, var, y, z0, g, lds, stw, xtw);
   const int line = X::FI::natural_line(threadIdx.x), q = threadIdx.x % X::T;
   C<R>* dst = reinterpret_cast<C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
 #pragma unroll
@@ -289,6 +418,8 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
     k_x_r2c(const R* __restrict__ ux, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw) {
   using X = XPass<R, N>;
   __shared__ R lds[X::kLdsReals];
+  __shared__ C<R> stw[X::kTw], xtw[X::M];
+  X::fill(stw, xtw, tw);
   const int2 blk = xcd_block2();
   const int var = blk.y;
   const int y = blk.x / (N / X::B), z0 = (blk.x % (N / X::B)) * X::B;
@@ -297,7 +428,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   C<R> v[X::E];
 #pragma unroll
   for (int j = 0; j < X::E; j++) v[j] = src[q + X::T * j];
-  X::r2c(v, xs, var, y, z0, g, lds, tw, (R)1 / (R)N);
+  X::r2c(v, xs, var, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
 }
 
 __device__ __forceinline__ void velmax_commit(double m, unsigned long long* velmax) {
@@ -328,12 +459,14 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   using X = XPass<R, N>;
   constexpr int E = X::E;
   __shared__ R lds[X::kLdsReals];
+  __shared__ C<R> stw[X::kTw], xtw[X::M];
+  X::fill(stw, xtw, tw);
   const int bx = xcd_block(blockIdx.x, gridDim.x);
   const int y = bx / (N / X::B), z0 = (bx % (N / X::B)) * X::B;
   C<R> pu[E], pv[E], pw[E];
-  X::c2r(pu, xs, 0, y, z0, g, lds, tw);
-  X::c2r(pv, xs, 1, y, z0, g, lds, tw);
-  X::c2r(pw, xs, 2, y, z0, g, lds, tw);
+  X::c2r(pu, xs, 0, y, z0, g, lds, stw, xtw);
+  X::c2r(pv, xs, 1, y, z0, g, lds, stw, xtw);
+  X::c2r(pw, xs, 2, y, z0, g, lds, stw, xtw);
   if (want_velmax) {
     double vm = 0.0;
 #pragma unroll
@@ -349,7 +482,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
 #pragma unroll
     for (int j = 0; j < E; j++)
       q[j] = {product(s, pu[j].x, pv[j].x, pw[j].x), product(s, pu[j].y, pv[j].y, pw[j].y)};
-    X::r2c(q, xs, s, y, z0, g, lds, tw, (R)1 / (R)N);
+    X::r2c(q, xs, s, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
   }
 }
 
@@ -387,6 +520,8 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
   constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   const int tid = threadIdx.x;
   const int nkb = (g.max_al_x + B - 1) / B;
   const int bx = xcd_block(blockIdx.x, gridDim.x);
@@ -419,9 +554,9 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
   if constexpr (MODE == 0) {
     C<R> p5[E], w[E];
     load(p5, 4);
-    F::template run<1>(p5, lds, in, out, tw);
+    F::template run<>(p5, lds, in, out, stw);
     load(w, 0);
-    F::template run<1>(w, lds, in, out, tw);
+    F::template run<>(w, lds, in, out, stw);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
@@ -430,7 +565,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
     }
     store(w, 0);
     load(w, 2);
-    F::template run<1>(w, lds, in, out, tw);
+    F::template run<>(w, lds, in, out, stw);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
@@ -439,17 +574,17 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
     }
     store(w, 2);
     load(w, 1);
-    F::template run<1>(w, lds, in, out, tw);
+    F::template run<>(w, lds, in, out, stw);
     store(w, 1);
     load(w, 3);
-    F::template run<1>(w, lds, in, out, tw);
+    F::template run<>(w, lds, in, out, stw);
     store(w, 3);
   } else {
     const R sc = (R)(1.0 / ((double)N * (double)N));
     for (int var = 0; var < nv; var++) {
       C<R> w[E];
       load(w, var);
-      F::template run<1>(w, lds, in, out, tw);
+      F::template run<>(w, lds, in, out, stw);
 #pragma unroll
       for (int j = 0; j < E; j++) w[j] = cscale(w[j], sc);
       store(w, var);
@@ -483,6 +618,8 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   const int tid = threadIdx.x;
   const int2 blk = xcd_block2();
   const int var = blk.y;
@@ -491,7 +628,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
   C<R> v[E];
   y_gather<R, N, 16>(v, w1, var, g, grp, F::transposed_line(tid), in.q);
   const Slot<R> out = F::natural(lds, tid);
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
   const int l = F::natural_line(tid);
   if (l < grp.y) {
     C<R>* col = u + ((size_t)var * g.na + grp.x + l) * N;
@@ -677,6 +814,8 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   const int tid = threadIdx.x;
   const Slot<R> in = F::transposed(lds, tid);
   const Slot<R> out = F::natural(lds, tid);
@@ -691,7 +830,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3
   C<R> a1[E], a2[E], a3[E], v[E];
   y_gather<R, N, 8>(v, w1, 1, g, grp, lin, in.q);  // P2
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
@@ -699,11 +838,11 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     a2[m] = cscale(v[m], (R)x);
   }
   y_gather<R, N, 8>(v, w1, 0, g, grp, lin, in.q);  // C1
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) a1[m] = a1[m] + v[m];
   y_gather<R, N, 8>(v, w1, 3, g, grp, lin, in.q);  // P4
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
@@ -711,7 +850,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     a3[m] = {-ky * v[m].y, ky * v[m].x};
   }
   y_gather<R, N, 8>(v, w1, 2, g, grp, lin, in.q);  // C3
-  F::template run<1>(v, lds, in, out, tw);
+  F::template run<>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) a3[m] = a3[m] + v[m];
   const C<R>* sxr = sr;
@@ -921,6 +1060,19 @@ static inline unsigned grid1d(size_t work, int per) {
   return (unsigned)(b ? b : 1);
 }
 
+// Work-groups of a persistent kernel: as many as are resident at once on
+// the device (occupancy x CUs), rounded down to a multiple of the 8 XCDs.
+static unsigned persistent_grid(const void* kernel, int threads) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  if (cus < 8) cus = 8;
+  const unsigned g = (unsigned)(cus * per_cu);
+  return g >= 8 ? g / 8 * 8 : 8;
+}
+
 template <typename R, int N>
 struct Launch {
   using CR = C<R>;
@@ -940,10 +1092,10 @@ struct Launch {
   }
   static void z_inv(const Ctx& c, int nv) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
-    const int nkb = (c.geo.max_al_x + B - 1) / B;
-    dim3 grid(N * nkb, nv);
-    hipLaunchKernelGGL((k_z_inv<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xs, c.geo,
-                       (const CR*)c.tw);
+    const auto kern = k_z_inv<R, N>;
+    static const unsigned grid = persistent_grid((const void*)kern, B * T);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream,
+                       (const CR*)c.w1, (CR*)c.xs, c.geo, (const CR*)c.tw, nv);
   }
   static void x_c2r(const Ctx& c, int nv) {
     using X = XPass<R, N>;
