@@ -49,10 +49,11 @@ struct psdns_ctx {
   double viscos = 0.0;
   std::vector<int> colx, colz, cutz;
   // device allocations
-  int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr;
+  int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr, *d_col0 = nullptr;
   int2* d_grp = nullptr;
   int ngrp = 0;
-  void *d_u = nullptr, *d_un = nullptr, *d_w1 = nullptr, *d_xs = nullptr, *d_ux = nullptr, *d_tw = nullptr;
+  void *d_u = nullptr, *d_un = nullptr, *d_w1 = nullptr, *d_xi = nullptr, *d_xs = nullptr, *d_ux = nullptr,
+       *d_tw = nullptr;
   double2* d_shift = nullptr;  // sx[2][nxh], sy[2][n], sz[2][n]
   void* d_shiftr = nullptr;    // per stage: sx[nxh] | sy[n] | sz[n], precision prec
   double* d_fac = nullptr;     // xfac[nxh], yfac[n], zfac[n]
@@ -227,12 +228,14 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_colx, sizeof(int) * c->na);
   ALLOC(c->d_colz, sizeof(int) * c->na);
   ALLOC(c->d_cutz, sizeof(int) * c->max_al_x);
+  ALLOC(c->d_col0, sizeof(int) * c->max_al_x);
   ALLOC(c->d_colz0, sizeof(int) * n);
   ALLOC(c->d_grp, sizeof(int2) * c->ngrp);
   ALLOC(c->d_u, c->csize() * 5 * cyl);
   ALLOC(c->d_un, c->csize() * 3 * cyl);
   const size_t w1_elems = (size_t)5 * c->max_al_x * n * n, xs_elems = (size_t)5 * n * n * c->nxh;
   ALLOC(c->d_w1, c->csize() * w1_elems);
+  ALLOC(c->d_xi, c->csize() * w1_elems);
   ALLOC(c->d_xs, c->csize() * xs_elems);
   ALLOC(c->d_tw, c->csize() * n);
   ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
@@ -243,11 +246,17 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   CHK(hipMemcpy(c->d_colx, c->colx.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_colz, c->colz.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_cutz, c->cutz.data(), sizeof(int) * c->max_al_x, hipMemcpyHostToDevice));
+  {
+    std::vector<int> col0(c->max_al_x, 0);
+    for (int a = c->na - 1; a >= 0; a--) col0[c->colx[a]] = a;
+    CHK(hipMemcpy(c->d_col0, col0.data(), sizeof(int) * c->max_al_x, hipMemcpyHostToDevice));
+  }
   CHK(hipMemcpy(c->d_colz0, colz0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_grp, grp.data(), sizeof(int2) * c->ngrp, hipMemcpyHostToDevice));
   CHK(hipMemsetAsync(c->d_u, 0, c->csize() * 5 * cyl, c->c.stream));
   CHK(hipMemsetAsync(c->d_un, 0, c->csize() * 3 * cyl, c->c.stream));
   CHK(hipMemsetAsync(c->d_w1, 0, c->csize() * w1_elems, c->c.stream));
+  CHK(hipMemsetAsync(c->d_xi, 0, c->csize() * w1_elems, c->c.stream));
   CHK(hipMemsetAsync(c->d_xs, 0, c->csize() * xs_elems, c->c.stream));
   // twiddles exp(-2 pi i k / n) formed in FP64
   {
@@ -266,10 +275,11 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   }
   Ctx& k = c->c;
   k.prec = c->prec;
-  k.geo = psdns::Geo{n, c->nxh, c->na, c->max_al_x, c->d_colx, c->d_colz, c->d_cutz, c->d_grp, c->ngrp};
+  k.geo = psdns::Geo{n, c->nxh, c->na, c->max_al_x, c->d_colx, c->d_colz, c->d_cutz, c->d_col0, c->d_grp, c->ngrp};
   k.u = c->d_u;
   k.un = c->d_un;
   k.w1 = c->d_w1;
+  k.xi = c->d_xi;
   k.xs = c->d_xs;
   k.ux = nullptr;
   k.tw = c->d_tw;
@@ -300,8 +310,9 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
 int psdns_destroy(psdns_ctx* c) {
   if (!c) return PSDNS_OK;
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
-  void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_colz0, c->d_grp, c->d_u, c->d_un, c->d_w1, c->d_xs, c->d_ux,
-                  c->d_tw,   c->d_shift, c->d_shiftr, c->d_fac, c->d_red, c->d_velmax, c->d_force};
+  void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
+                  c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
+                  c->d_red,  c->d_velmax, c->d_force};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)

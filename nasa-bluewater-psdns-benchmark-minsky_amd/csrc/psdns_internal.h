@@ -17,6 +17,7 @@ struct Geo {
   const int* colx;  // [na]   kx of column a
   const int* colz;  // [na]   z index of column a
   const int* cutz;  // [max_al_x]
+  const int* col0;  // [max_al_x]  first cylinder column of kx
   // column groups of the y passes: consecutive columns of one kx whose z
   // lie in one aligned block of y_group_lines(prec) entries, so that a group
   // covers whole aligned runs of a z-pencil row; (first column, count)
@@ -34,8 +35,10 @@ struct Ctx {
   hipStream_t stream;
   void* u;   // cylinder, 5 slots  [5][na][n] complex
   void* un;  // cylinder, 3 slots  [3][na][n] complex
-  void* w1;  // y<->z work, 5 slots [5][max_al_x][n(y)][n(z)] complex
-  void* xs;  // z<->x work, 5 slots [5][n(y)][n(z)][n/2(kx)] complex
+  void* w1;  // y<->z work, 5 slots [5][max_al_x][n(y)][n(z)] complex (also
+             // the y-transformed cylinder [5][na][n] of the inverse path)
+  void* xi;  // z->x work of the inverse path, 5 slots [5][n(y)][max_al_x][n(z)]
+  void* xs;  // x->z work, 5 slots [5][n(y)][n(z)][n/2(kx)] complex
   void* ux;  // physical, 5 slots  [5][n][n][n] real (per-routine path only)
   void* tw;  // twiddles exp(-2 pi i k / n), k < n, precision prec
   int kstep; // 0-based RK stage for the shift tables
@@ -54,13 +57,13 @@ struct Ctx {
 };
 
 enum class Pass {
-  YInvPhase,  // phase1 + y c2c +i + scatter (arg: number of variables)
-  YInvKx,     // kxtran: zero y Nyquist + y c2c +i + scatter
-  YInvPlain,  // y c2c +i + scatter
-  ZInv,       // z c2c +i (arg: number of variables)
-  XC2R,       // x c2r -> physical (arg: number of variables)
+  YInvPhase,  // phase1 + y c2c +i, u -> yw (arg: number of variables)
+  YInvKx,     // kxtran: zero y Nyquist + y c2c +i, u -> yw
+  YInvPlain,  // y c2c +i, u -> yw
+  ZInv,       // gather yw + z c2c +i -> xi (arg: number of variables)
+  XC2R,       // gather xi + x c2r -> physical (arg: number of variables)
   XR2C,       // physical -> x r2c / n (arg: number of variables)
-  XFused,     // c2r u,v,w -> products (+velmax, arg) -> r2c / n of 5
+  XFused,     // c2r u,v,w (xi) -> products (+velmax, arg) -> r2c / n of 5 (X)
   ZFwdConv,   // z c2c -i of 5 + convective combination -> 4
   ZFwdNorm,   // z c2c -i / (ny nz) (arg: number of variables)
   YFwd,       // gather cylinder + y c2c -i (arg: number of variables)

@@ -2,19 +2,24 @@
 //
 // Device data (one variable = one slot; complex = interleaved re, im):
 //   cylinder  u[v][a][y]        reference uy(ny, num_al, nv)  (state, y fastest)
-//   W1        w1[v][kx][y][z]   between the y and z passes (kx < max_al_x):
-//             a cylinder column's y-run stays inside one kx slab, the z
-//             lines are contiguous
-//   X         xs[v][y][z][kx]   between the z and x passes (kx < max_al_x
-//             used, stride n/2): the x lines are contiguous
+//   inverse transform:
+//     yw      w1 as [v][a][y]   y-transformed cylinder (y pass output)
+//     xi      xi[v][y][kx][z]   z-transformed (z pass output, kx < max_al_x):
+//             the z lines are contiguous
+//   forward transform:
+//     X       xs[v][y][z][kx]   x pass output (kx < max_al_x used, stride
+//             n/2): the x lines are contiguous
+//     W1      w1[v][kx][y][z]   z pass output (cylinder z only): the z lines
+//             are contiguous, a cylinder column's y-run stays in one kx slab
 //   physical  ux[v][y][z][x]    reference ux(nx, zisz, yjsz, nv) (only for the
 //             per-routine entry points; the fused step keeps physical space
 //             in registers)
 // The reference's z-pencil array uz(nz, xisz, yjsz) (kxcomm1_clean.f:28) is
-// split into W1 and X so that every pass reads or writes one side along a
-// contiguous line and the other side in 64-byte runs inside a few pages.
-// Only cylinder columns (kx < max_al_x, z outside the cut) are stored in W1;
-// z entries in the cut are synthesised as zeros by the inverse z pass.
+// replaced by these layouts so that every pass writes whole lines and the
+// transposes happen on the read side, in 64-byte runs inside a few pages
+// (scattered 64-byte writes reach about two thirds of the bandwidth of
+// scattered 64-byte reads on this memory system).  The cylinder cut is never
+// stored: the z passes read it as zeros and skip it on output.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -92,65 +97,9 @@ __device__ __forceinline__ size_t w1idx(int v, int kx, int y, int z, const Geo& 
 __device__ __forceinline__ size_t xidx(int v, int y, int z, int kx, const Geo& g) {
   return (((size_t)v * g.n + y) * g.n + z) * g.nxh + kx;
 }
-
-// ---------------------------------------------------------------------------
-// Inverse y pass: cylinder -> W1.
-// MODE 0: phase1 (phshift_inplace, homogeneous/phshift.f:71-135) fused on
-//         load: u *= sy(y) (sx(x) sz(z)) on the mask, else 0 (off-mask
-//         entries are not read).  The shifted field is not written back: in
-//         the step the next writer of u is the forward transform
-//         (itransform_vel overwrites u in place).
-// MODE 1: kxtran (kxtran.f:36): zero the y Nyquist row.
-// MODE 2: plain inverse transform (itransform_vel on an already shifted u).
-// Then y c2c +i (kxcomm1_clean.f:118); loads run along y (natural mapping),
-// the last Stockham stage hands the outputs over in the transposed mapping so
-// that the scatter (kxcomm1_clean.f:205-255) writes 64-byte runs along z.
-// ---------------------------------------------------------------------------
-template <typename R, int N, int MODE>
-__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
-    k_y_inv(const C<R>* __restrict__ u, C<R>* __restrict__ w1, Geo g, const double2* __restrict__ sx,
-            const double2* __restrict__ sy, const double2* __restrict__ sz,
-            const C<R>* __restrict__ tw) {
-  constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
-  __shared__ R lds[F::kLdsReals];
-  __shared__ C<R> stw[F::kTwEntries];
-  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
-  const int tid = threadIdx.x;
-  const int2 blk = xcd_block2();
-  const int var = blk.y;
-  const int2 grp = g.grp[blk.x];
-  const Slot<R> in = F::natural(lds, tid);
-  const int li = F::natural_line(tid);
-  const bool act = li < grp.y;
-  const int a = grp.x + (act ? li : 0);
-  const int x = g.colx[a], z = g.colz[a];
-  const C<R>* col = u + ((size_t)var * g.na + a) * N;
-  C<R> v[E];
-  double2 sxz = make_double2(1.0, 0.0);
-  if (MODE == 0) sxz = zmul(sx[x], sz[z]);
-#pragma unroll
-  for (int j = 0; j < E; j++) {
-    const int y = in.q + T * j;
-    C<R> val = {0, 0};
-    if constexpr (MODE == 0) {
-      if (act && in_mask(x, kwave(y, N), kwave(z, N), y, z, N)) val = cmul(col[y], to_c<R>(zmul(sy[y], sxz)));
-    } else if constexpr (MODE == 1) {
-      if (act && y != N / 2) val = col[y];
-    } else {
-      if (act) val = col[y];
-    }
-    v[j] = val;
-  }
-  const Slot<R> out = F::transposed(lds, tid);
-  F::template run<>(v, lds, in, out, stw);
-  const int lo = F::transposed_line(tid);
-  if (lo < grp.y) {
-    // a group's columns are consecutive z of one kx
-    C<R>* p = w1 + w1idx(var, g.colx[grp.x], 0, g.colz[grp.x] + lo, g);
-#pragma unroll
-    for (int m = 0; m < E; m++) p[(size_t)(out.q + T * m) * N] = v[m];
-  }
+// xi index of (v, y, kx, z): inverse z pass output, x pass input
+__device__ __forceinline__ size_t xiidx(int v, int y, int kx, int z, const Geo& g) {
+  return (((size_t)v * g.n + y) * g.max_al_x + kx) * g.n + z;
 }
 
 // Persistent work-groups: the grid holds as many work-groups as fit on the
@@ -255,64 +204,146 @@ struct BufView {
 };
 
 // ---------------------------------------------------------------------------
-// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): W1 -> X.  A tile
-// is one variable, one y and B consecutive kx (< max_al_x); the z lines are
-// read along z (natural mapping) with the cylinder cut, which always holds
-// the z Nyquist, read as zero (kxcomm1_clean.f:238-240); z c2c +i; the last
-// stage hands over in the transposed mapping so that X is written in 64-byte
-// runs along kx.  kx >= max_al_x is zero and is never stored.
+// Inverse y pass: cylinder u -> cylinder yw (same [v][a][y] layout, in the
+// W work array).
+// MODE 0: phase1 (phshift_inplace, homogeneous/phshift.f:71-135) fused on
+//         load: u sy(y) (sx(x) sz(z)) on the mask, else 0 (off-mask entries
+//         are not read).  The shifted field is not written back: in the step
+//         the next writer of u is the forward transform (itransform_vel
+//         overwrites u in place).
+// MODE 1: kxtran (kxtran.f:36): zero the y Nyquist row.
+// MODE 2: plain inverse transform (itransform_vel on an already shifted u).
+// Then y c2c +i (kxcomm1_clean.f:118).  A tile is B consecutive columns of
+// one variable; loads and stores both run along y (natural mapping), so the
+// pass streams whole columns in and out.  The transpose towards z lines
+// (kxcomm1_clean.f:205-255) is done by the gathered loads of the inverse z
+// pass: 64-byte runs are cheap to read but not to write.
+// Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n].
+// ---------------------------------------------------------------------------
+template <typename R>
+struct YInv {
+  static constexpr int B = 4;  // columns per tile
+};
+
+template <typename R, int N, int MODE>
+__global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
+    k_y_inv(const C<R>* __restrict__ u, C<R>* __restrict__ yw, Geo g, const C<R>* __restrict__ sr,
+            const C<R>* __restrict__ tw, int nv, const int* __restrict__ colx, const int* __restrict__ colz) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = YInv<R>::B;
+  using F = TeamFft<R, N, E, B, +1>;
+  __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  __shared__ C<R> ssy[MODE == 0 ? N : 1];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  if constexpr (MODE == 0)
+    for (int k = threadIdx.x; k < N; k += F::NT) ssy[k] = sr[g.nxh + k];
+  const int tid = threadIdx.x;
+  const Slot<R> io = F::natural(lds, tid);
+  const int li0 = F::natural_line(tid);
+  const int nab = (g.na + B - 1) / B;  // tiles per variable
+  __syncthreads();
+  // the line of a lane (wave-uniform when a line spans whole waves)
+  auto line_of = [&]() { return T >= 64 ? __builtin_amdgcn_readfirstlane(opaque(li0)) : opaque(li0); };
+  // the loads only issue (off-mask lanes read nothing); the shift factors
+  // are applied in finish, so that the next tile's loads stay in flight
+  auto load = [&](Regs<R, E>& b, int t, bool valid) {
+    const int var = t / nab, a0 = (t % nab) * B;
+    const BufView<R> src(u + ((size_t)var * g.na + a0) * N, valid ? kOob : 0u);
+    // lines past the last column re-read the last column (never stored), so
+    // that no branch depends on the tile
+    const int li = line_of(), q = opaque(io.q);
+    const int lc = li < g.na - a0 ? li : g.na - 1 - a0;
+    const int x = colx[a0 + lc], z = colz[a0 + lc], kz = kwave(z, N);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int y = q + T * j;
+      bool keep = true;
+      if constexpr (MODE == 0) keep = in_mask(x, kwave(y, N), kz, y, z, N);
+      if constexpr (MODE == 1) keep = y != N / 2;
+      b.v[j] = src.load(keep ? ((unsigned)lc * N + y) * (unsigned)sizeof(C<R>) : kOob);
+    }
+  };
+  auto finish = [&](Regs<R, E>& b, int t) {
+    const int var = t / nab, a0 = (t % nab) * B;
+    if constexpr (MODE == 0) {
+      const int li = line_of(), q = opaque(io.q);
+      const int a = a0 + li < g.na ? a0 + li : g.na - 1;
+      const C<R> sxz = cmul(sr[colx[a]], sr[g.nxh + N + colz[a]]);
+#pragma unroll
+      for (int j = 0; j < E; j++) b.v[j] = cmul(b.v[j], cmul(ssy[q + T * j], sxz));
+    }
+    F::template run<>(b.v, lds, io, io, stw);
+    const int li = line_of(), q = opaque(io.q);
+    const BufView<R> dst(yw + ((size_t)var * g.na + a0) * N, kOob);
+    const bool act = a0 + li < g.na;
+#pragma unroll
+    for (int m = 0; m < E; m++)
+      dst.store(act ? ((unsigned)li * N + q + T * m) * (unsigned)sizeof(C<R>) : kOob, b.v[m]);
+  };
+  Regs<R, E> ba, bb;
+  tile_loop(nv * nab, ba, bb, load, finish);
+}
+
+// ---------------------------------------------------------------------------
+// Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): y-transformed
+// cylinder yw -> xi[v][y][kx][z].  A tile is one variable, one kx
+// (< max_al_x) and B consecutive y.  Its z lines are gathered from the kx's
+// cylinder columns in 64-byte runs along y (transposed mapping), the cut
+// (which always holds the z Nyquist) read as zero (kxcomm1_clean.f:238-240);
+// z c2c +i; the last stage leaves the lines in the natural mapping for
+// contiguous stores along z.
 // ---------------------------------------------------------------------------
 template <typename R, int N>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((amdgpu_waves_per_eu(4)))
-    k_z_inv(const C<R>* __restrict__ w1, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw, int nv) {
+    k_z_inv(const C<R>* __restrict__ yw, C<R>* __restrict__ xi, Geo g, const C<R>* __restrict__ tw, int nv,
+            const int* __restrict__ cutz, const int* __restrict__ col0) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
-  using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::OUT>;
+  constexpr int NYB = N / B;  // y blocks
+  using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
-  __shared__ int scut[N / 2];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
-  for (int k = threadIdx.x; k < g.max_al_x; k += F::NT) scut[k] = g.cutz[k];
   const int tid = threadIdx.x;
-  const int nkb = (g.max_al_x + B - 1) / B;
-  const Slot<R> in = F::natural(lds, tid);
-  const Slot<R> out = F::transposed(lds, tid);
-  const int li0 = F::natural_line(tid), lo0 = F::transposed_line(tid);
+  const Slot<R> in = F::transposed(lds, tid);
+  const Slot<R> out = F::natural(lds, tid);
+  const int lt0 = F::transposed_line(tid), ln0 = F::natural_line(tid);
   __syncthreads();
-  // tile t = (var * N + y) * nkb + kx block; per-tile bases are uniform,
-  // per-lane offsets are 32-bit byte offsets
+  // tile t = (var * max_al_x + kx) * NYB + y block
   auto load = [&](Regs<R, E>& b, int t, bool valid) {
-    const int var = t / (N * nkb), y = (t / nkb) % N, kx0 = (t % nkb) * B;
-    const BufView<R> src(w1 + w1idx(var, kx0, y, 0, g), valid ? kOob : 0u);
-    const int li = opaque(li0), q = opaque(in.q);
-    const int kx = kx0 + li;
-    const bool act = kx < g.max_al_x;
-    const int c = scut[act ? kx : 0];
+    const int vk = t / NYB, y0 = (t % NYB) * B;
+    const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
+    const int c = cutz[kx];
+    const BufView<R> src(yw + ((size_t)var * g.na + col0[kx]) * N + y0, valid ? kOob : 0u);
+    const int l = opaque(lt0), q = opaque(in.q);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const int zz = q + T * j;
-      const bool kept = act && (zz < N / 2 - c || zz > N / 2 + c);
-      b.v[j] = src.load(kept ? ((unsigned)li * N * N + zz) * (unsigned)sizeof(C<R>) : kOob);
+      const bool lo = zz < N / 2 - c, hi = zz > N / 2 + c;
+      const int ai = lo ? zz : zz - 2 * c - 1;  // column of z within kx
+      b.v[j] = src.load(lo || hi ? ((unsigned)ai * N + l) * (unsigned)sizeof(C<R>) : kOob);
     }
   };
   auto finish = [&](Regs<R, E>& b, int t) {
     F::template run<>(b.v, lds, in, out, stw);
-    const int var = t / (N * nkb), y = (t / nkb) % N, kx0 = (t % nkb) * B;
-    const int lo = opaque(lo0), q = opaque(out.q);
-    const BufView<R> dst(xs + xidx(var, y, 0, kx0, g), kOob);
-    const bool act = kx0 + lo < g.max_al_x;
+    const int vk = t / NYB, y0 = (t % NYB) * B;
+    const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
+    const int l = opaque(ln0), q = opaque(out.q);
+    const BufView<R> dst(xi + xiidx(var, y0, kx, 0, g), kOob);
 #pragma unroll
     for (int m = 0; m < E; m++)
-      dst.store(act ? ((unsigned)(q + T * m) * g.nxh + lo) * (unsigned)sizeof(C<R>) : kOob, b.v[m]);
+      dst.store(((unsigned)l * g.max_al_x * N + q + T * m) * (unsigned)sizeof(C<R>), b.v[m]);
   };
   Regs<R, E> ba, bb;
-  tile_loop(nv * N * nkb, ba, bb, load, finish);
+  tile_loop(nv * g.max_al_x * NYB, ba, bb, load, finish);
 }
 
 // ---------------------------------------------------------------------------
 // x pass building blocks (kxcomm2.f:278-333 inverse, xkcomm1.f:97,135
-// forward) on contiguous X lines: a work-group owns one y and B consecutive
-// z, natural mapping throughout.  The real transform of length n runs as a
-// complex transform of length m = n/2 on packed pairs (x[2i], x[2i+1]).
+// forward): a tile is one y and B consecutive z.  The inverse transform
+// gathers its x lines from xi[v][y][kx][z] in 64-byte runs along z
+// (transposed mapping); the forward transform writes contiguous x lines of
+// X[v][y][z][kx] (natural mapping).  The real transform of length n runs as
+// a complex transform of length m = n/2 on packed pairs (x[2i], x[2i+1]).
 // ---------------------------------------------------------------------------
 #ifndef PSDNS_XPASS_E
 #define PSDNS_XPASS_E 8
@@ -321,11 +352,10 @@ template <typename R, int N>
 struct XPass {
   static constexpr int M = N / 2;
   static constexpr int T = Team<M, PSDNS_XPASS_E>::T, E = Team<M, PSDNS_XPASS_E>::E;
-  static constexpr int B0 = 256 / T > 0 ? 256 / T : 1;
-  static constexpr int B = B0 < N ? B0 : N;  // z lines per work-group
+  static constexpr int B = RunLines<R>::B;  // z lines per tile
   static constexpr int NT = B * T;
-  using FI = TeamFft<R, M, E, B, +1>;
-  using FF = TeamFft<R, M, E, B, -1>;
+  using FI = TeamFft<R, M, E, B, +1, LineShift<E, B>::IN>;
+  using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN>;
   static constexpr int kLdsReals = FI::kLdsReals;
   static constexpr int kTw = FI::kTwEntries;  // shared by FI and FF
 
@@ -339,22 +369,24 @@ struct XPass {
     __syncthreads();
   }
 
-  // c2r of one X variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
+  // c2r of one xi variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
   // ignored): Z[k] = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k]).
-  // Output: packed physical values (x[2i], x[2i+1]) of this thread's line.
-  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xs, int var, int y, int z0,
+  // Output: packed physical values (x[2i], x[2i+1]) of this thread's line
+  // in the natural mapping.
+  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xi, int var, int y, int z0,
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
     const int tid = threadIdx.x;
-    const Slot<R> s = FI::natural(lds, tid);
-    const C<R>* line = xs + xidx(var, y, z0, 0, g) + FI::natural_line(tid) * g.nxh;
+    const Slot<R> in = FI::transposed(lds, tid);
+    const Slot<R> out = FI::natural(lds, tid);
+    const BufView<R> src(xi + xiidx(var, y, 0, z0, g), kOob);
+    const int l = FI::transposed_line(tid);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = s.q + T * j;
+      const int k = in.q + T * j;
       const int km = M - k;
-      C<R> xk = {0, 0}, xm = {0, 0};
-      if (k < g.max_al_x) xk = line[k];
+      C<R> xk = src.load(k < g.max_al_x ? ((unsigned)k * N + l) * (unsigned)sizeof(C<R>) : kOob);
       if (k == 0) xk.y = 0;
-      if (km < g.max_al_x) xm = line[km];
+      const C<R> xm = src.load(km < g.max_al_x ? ((unsigned)km * N + l) * (unsigned)sizeof(C<R>) : kOob);
       const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
       C<R> w = xtw[k];
@@ -362,7 +394,7 @@ struct XPass {
       const C<R> wd = cmul(w, d);
       v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
-    FI::template run<>(v, lds, s, s, stw);
+    FI::template run<>(v, lds, in, out, stw);
   }
 
   // r2c of packed physical values v of this thread's line, scaled by
@@ -393,10 +425,10 @@ struct XPass {
   }
 };
 
-// inverse x pass alone: X (nv variables) -> physical ux
+// inverse x pass alone: xi (nv variables) -> physical ux
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT))
-    k_x_c2r(const C<R>* __restrict__ xs, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
+    k_x_c2r(const C<R>* __restrict__ xi, R* __restrict__ ux, Geo g, const C<R>* __restrict__ tw) {
   using X = XPass<R, N>;
   __shared__ R lds[X::kLdsReals];
   __shared__ C<R> stw[X::kTw], xtw[X::M];
@@ -405,7 +437,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   const int var = blk.y;
   const int y = blk.x / (N / X::B), z0 = (blk.x % (N / X::B)) * X::B;
   C<R> v[X::E];
-  X::c2r(v, xs, var, y, z0, g, lds, stw, xtw);
+  X::c2r(v, xi, var, y, z0, g, lds, stw, xtw);
   const int line = X::FI::natural_line(threadIdx.x), q = threadIdx.x % X::T;
   C<R>* dst = reinterpret_cast<C<R>*>(ux + (((size_t)var * N + y) * N + z0 + line) * N);
 #pragma unroll
@@ -449,12 +481,12 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
   }
 }
 
-// Fused x pass: c2r of u, v, w -> realspace_vel products (+ velmax, FP64,
-// realspace_vel.f:108-110) -> r2c x 1/n of the five products, written back
-// into X slots 0..4.  Physical space stays in registers.
+// Fused x pass: c2r of u, v, w (from xi) -> realspace_vel products
+// (+ velmax, FP64, realspace_vel.f:108-110) -> r2c x 1/n of the five
+// products into X slots 0..4.  Physical space stays in registers.
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT))
-    k_x_fused(C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
+    k_x_fused(const C<R>* __restrict__ xi, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
               unsigned long long* __restrict__ velmax, int want_velmax) {
   using X = XPass<R, N>;
   constexpr int E = X::E;
@@ -464,9 +496,9 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   const int bx = xcd_block(blockIdx.x, gridDim.x);
   const int y = bx / (N / X::B), z0 = (bx % (N / X::B)) * X::B;
   C<R> pu[E], pv[E], pw[E];
-  X::c2r(pu, xs, 0, y, z0, g, lds, stw, xtw);
-  X::c2r(pv, xs, 1, y, z0, g, lds, stw, xtw);
-  X::c2r(pw, xs, 2, y, z0, g, lds, stw, xtw);
+  X::c2r(pu, xi, 0, y, z0, g, lds, stw, xtw);
+  X::c2r(pv, xi, 1, y, z0, g, lds, stw, xtw);
+  X::c2r(pw, xi, 2, y, z0, g, lds, stw, xtw);
   if (want_velmax) {
     double vm = 0.0;
 #pragma unroll
@@ -1063,13 +1095,19 @@ static inline unsigned grid1d(size_t work, int per) {
 // Work-groups of a persistent kernel: as many as are resident at once on
 // the device (occupancy x CUs), rounded down to a multiple of the 8 XCDs.
 static unsigned persistent_grid(const void* kernel, int threads) {
+#ifdef PSDNS_RESIDENT_SCALE
+  // tuning: a larger grid than fits at once (work-groups take fewer tiles)
+  constexpr int kScale = PSDNS_RESIDENT_SCALE;
+#else
+  constexpr int kScale = 1;
+#endif
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
   if (cus < 8) cus = 8;
-  const unsigned g = (unsigned)(cus * per_cu);
+  const unsigned g = (unsigned)(cus * per_cu * kScale);
   return g >= 8 ? g / 8 * 8 : 8;
 }
 
@@ -1077,30 +1115,36 @@ template <typename R, int N>
 struct Launch {
   using CR = C<R>;
   static void y_inv(const Ctx& c, int nv, int mode) {
-    constexpr int B = RunLines<R>::B, T = Team<N>::T;
-    dim3 grid(c.geo.ngrp, nv);
-    const int k = mode == 0 ? c.kstep : 0;
-    if (mode == 0)
-      hipLaunchKernelGGL((k_y_inv<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
-                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
-    else if (mode == 1)
-      hipLaunchKernelGGL((k_y_inv<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
-                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
-    else
-      hipLaunchKernelGGL((k_y_inv<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo,
-                         c.sx[k], c.sy[k], c.sz[k], (const CR*)c.tw);
+    constexpr int T = Team<N>::T, NT = YInv<R>::B * T;
+    const CR* sr = (const CR*)c.sr[mode == 0 ? c.kstep : 0];
+    if (mode == 0) {
+      const auto kern = k_y_inv<R, N, 0>;
+      static const unsigned grid = persistent_grid((const void*)kern, NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo, sr,
+                         (const CR*)c.tw, nv, c.geo.colx, c.geo.colz);
+    } else if (mode == 1) {
+      const auto kern = k_y_inv<R, N, 1>;
+      static const unsigned grid = persistent_grid((const void*)kern, NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo, sr,
+                         (const CR*)c.tw, nv, c.geo.colx, c.geo.colz);
+    } else {
+      const auto kern = k_y_inv<R, N, 2>;
+      static const unsigned grid = persistent_grid((const void*)kern, NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const CR*)c.u, (CR*)c.w1, c.geo, sr,
+                         (const CR*)c.tw, nv, c.geo.colx, c.geo.colz);
+    }
   }
   static void z_inv(const Ctx& c, int nv) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
     const auto kern = k_z_inv<R, N>;
     static const unsigned grid = persistent_grid((const void*)kern, B * T);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream,
-                       (const CR*)c.w1, (CR*)c.xs, c.geo, (const CR*)c.tw, nv);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xi, c.geo,
+                       (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
   }
   static void x_c2r(const Ctx& c, int nv) {
     using X = XPass<R, N>;
     dim3 grid(N * (N / X::B), nv);
-    hipLaunchKernelGGL((k_x_c2r<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.xs, (R*)c.ux, c.geo,
+    hipLaunchKernelGGL((k_x_c2r<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.xi, (R*)c.ux, c.geo,
                        (const CR*)c.tw);
   }
   static void x_r2c(const Ctx& c, int nv) {
@@ -1112,8 +1156,8 @@ struct Launch {
   static void x_fused(const Ctx& c, int want_velmax) {
     using X = XPass<R, N>;
     dim3 grid(N * (N / X::B));
-    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (CR*)c.xs, c.geo, (const CR*)c.tw,
-                       c.velmax_bits, want_velmax);
+    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
+                       (const CR*)c.tw, c.velmax_bits, want_velmax);
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
     constexpr int B = RunLines<R>::B, T = Team<N, PSDNS_ZFWD_E>::T;
