@@ -102,6 +102,14 @@ __device__ __forceinline__ size_t xiidx(int v, int y, int kx, int z, const Geo& 
   return (((size_t)v * g.n + y) * g.max_al_x + kx) * g.n + z;
 }
 
+// Scheduling fence: the compiler may not move instructions across it, so
+// loads issued before it are all in flight before anything after it runs
+// (and the waits before their first use count only what was issued later).
+__device__ __forceinline__ void issue_fence() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // Persistent work-groups: the grid holds as many work-groups as fit on the
 // device at once (a multiple of 8), and each walks a list of tiles.  XCD x
 // (work-groups with blockIdx.x % 8 == x) owns one contiguous range of tiles,
@@ -134,10 +142,12 @@ __device__ __forceinline__ void tile_loop(int ntiles, Buf& ba, Buf& bb, Load&& l
   for (;;) {
     const int t2 = t + w.step;
     load(bb, t2 < w.end ? t2 : t, t2 < w.end);
+    issue_fence();
     finish(ba, t);
     if (t2 >= w.end) break;
     t = t2 + w.step;
     load(ba, t < w.end ? t : t2, t < w.end);
+    issue_fence();
     finish(bb, t2);
     if (t >= w.end) break;
   }
@@ -371,22 +381,38 @@ struct XPass {
 
   // c2r of one xi variable (kx < max_al_x nonzero; kx Nyquist zero; Im X[0]
   // ignored): Z[k] = (X[k] + conj X[m-k]) + i e^{+2 pi i k/n} (X[k] - conj X[m-k]).
-  // Output: packed physical values (x[2i], x[2i+1]) of this thread's line
-  // in the natural mapping.
-  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xi, int var, int y, int z0,
-                                             const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
+  // c2r_load issues the gathered loads of X[k] and X[m-k] (transposed
+  // mapping, no branch: out-of-range lanes read zeros without a memory
+  // access); c2r_finish forms Z and transforms it, leaving the packed
+  // physical values (x[2i], x[2i+1]) of this thread's line in the natural
+  // mapping.  Splitting the two lets a variable's loads be in flight during
+  // the transform of the previous one.
+  struct Raw {
+    C<R> k[E], m[E];
+  };
+  static __device__ __forceinline__ void c2r_load(Raw& r, const C<R>* __restrict__ xi, int var, int y, int z0,
+                                                  const Geo& g) {
+    const int tid = opaque(threadIdx.x);
+    const int l = FI::transposed_line(tid), q = tid / B;
+    const BufView<R> src(xi + xiidx(var, y, 0, z0, g), kOob);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int k = q + T * j;
+      const int km = M - k;
+      r.k[j] = src.load(k < g.max_al_x ? ((unsigned)k * N + l) * (unsigned)sizeof(C<R>) : kOob);
+      r.m[j] = src.load(km < g.max_al_x ? ((unsigned)km * N + l) * (unsigned)sizeof(C<R>) : kOob);
+    }
+  }
+  static __device__ __forceinline__ void c2r_finish(C<R>* v, Raw& r, R* lds, const C<R>* stw, const C<R>* xtw) {
     const int tid = threadIdx.x;
     const Slot<R> in = FI::transposed(lds, tid);
     const Slot<R> out = FI::natural(lds, tid);
-    const BufView<R> src(xi + xiidx(var, y, 0, z0, g), kOob);
-    const int l = FI::transposed_line(tid);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const int k = in.q + T * j;
-      const int km = M - k;
-      C<R> xk = src.load(k < g.max_al_x ? ((unsigned)k * N + l) * (unsigned)sizeof(C<R>) : kOob);
+      C<R> xk = r.k[j];
       if (k == 0) xk.y = 0;
-      const C<R> xm = src.load(km < g.max_al_x ? ((unsigned)km * N + l) * (unsigned)sizeof(C<R>) : kOob);
+      const C<R> xm = r.m[j];
       const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
       C<R> w = xtw[k];
@@ -395,6 +421,12 @@ struct XPass {
       v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
     FI::template run<>(v, lds, in, out, stw);
+  }
+  static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xi, int var, int y, int z0,
+                                             const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
+    Raw r;
+    c2r_load(r, xi, var, y, z0, g);
+    c2r_finish(v, r, lds, stw, xtw);
   }
 
   // r2c of packed physical values v of this thread's line, scaled by
@@ -496,9 +528,15 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   const int bx = xcd_block(blockIdx.x, gridDim.x);
   const int y = bx / (N / X::B), z0 = (bx % (N / X::B)) * X::B;
   C<R> pu[E], pv[E], pw[E];
-  X::c2r(pu, xi, 0, y, z0, g, lds, stw, xtw);
-  X::c2r(pv, xi, 1, y, z0, g, lds, stw, xtw);
-  X::c2r(pw, xi, 2, y, z0, g, lds, stw, xtw);
+  typename X::Raw ra, rb;
+  X::c2r_load(ra, xi, 0, y, z0, g);
+  X::c2r_load(rb, xi, 1, y, z0, g);
+  issue_fence();
+  X::c2r_finish(pu, ra, lds, stw, xtw);
+  X::c2r_load(ra, xi, 2, y, z0, g);
+  issue_fence();
+  X::c2r_finish(pv, rb, lds, stw, xtw);
+  X::c2r_finish(pw, ra, lds, stw, xtw);
   if (want_velmax) {
     double vm = 0.0;
 #pragma unroll
@@ -1099,7 +1137,7 @@ static unsigned persistent_grid(const void* kernel, int threads) {
   // tuning: a larger grid than fits at once (work-groups take fewer tiles)
   constexpr int kScale = PSDNS_RESIDENT_SCALE;
 #else
-  constexpr int kScale = 1;
+  constexpr int kScale = 8;
 #endif
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
