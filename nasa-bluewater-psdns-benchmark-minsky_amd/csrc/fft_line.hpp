@@ -21,6 +21,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace psdns {
 
 template <typename R>
@@ -137,6 +139,19 @@ struct Slot {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
+// f(std::integral_constant<int, i>) for i = 0..N-1
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for_impl(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for_impl<I + 1, N>(f);
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl<0, N>(f);
+}
+
 // B lines of length L, E points per thread, LDS planes re [B][S], im [B][S].
 // Padding of the buffer written by the stage of span NSW (product of the
 // earlier radices), which writes chunks of NSW consecutive points with chunk
@@ -159,6 +174,23 @@ struct TeamFft {
       return i + (i >> ilog2(NSW * RAD)) * NSW;
     } else {
       return i;
+    }
+  }
+  // padding period of the buffer of span NSW: pos(i + c) = pos(i) + pos(c)
+  // for every multiple c of the period
+  template <int NSW>
+  static constexpr int period() {
+    constexpr int RAD = StagePlan<L, E, NSW>::kRad;
+    return NSW == 1 ? 32 : (NSW < 32 ? NSW * RAD : 1);
+  }
+  // pos(i + C) from pos(i), with the constant part folded at compile time
+  // where the period allows it
+  template <int NSW, int C>
+  static __device__ __forceinline__ int pos_plus(int i, int pi) {
+    if constexpr (C % period<NSW>() == 0) {
+      return pi + pos<NSW>(C);
+    } else {
+      return pos<NSW>(i + C);
     }
   }
   template <int NSW>
@@ -265,11 +297,14 @@ struct TeamFft {
     for (int i = 0; i < NB; i++) {
       const int j = s.q + T * i;
       const int jm = j & (NS - 1);
-#pragma unroll
-      for (int r = 0; r < RAD; r++) {
-        const int p = pos<NS / E>(j + r * (L / RAD));  // buffer written by the stage of span NS/E
+      // buffer written by the stage of span NS/E; L/RAD is a multiple of NS,
+      // hence of that buffer's padding period
+      const int pj = pos<NS / E>(j);
+      static_for<RAD>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const int p = pos_plus<NS / E, r * (L / RAD)>(j, pj);
         w[i * RAD + r] = {s.re[p], s.im[p]};
-      }
+      });
 #pragma unroll
       for (int r = 1; r < RAD; r++)
         w[i * RAD + r] = cmul(w[i * RAD + r], tw_at<NS, RAD>(stw, jm, r));
@@ -286,12 +321,13 @@ struct TeamFft {
       for (int i = 0; i < NB; i++) {
         const int j = s.q + T * i;
         const int jm = j & (NS - 1);
-        const int base = (j / NS) * NS * RAD + jm;
+        // base + r NS stays inside one padding period of this buffer, so
+        // its position is pos(base) + r NS
+        const int pb = pos<NS>((j / NS) * NS * RAD + jm);
 #pragma unroll
         for (int r = 0; r < RAD; r++) {
-          const int p = pos<NS>(base + r * NS);
-          s.re[p] = w[i * RAD + r].x;
-          s.im[p] = w[i * RAD + r].y;
+          s.re[pb + r * NS] = w[i * RAD + r].x;
+          s.im[pb + r * NS] = w[i * RAD + r].y;
         }
       }
     }
@@ -332,11 +368,12 @@ struct TeamFft {
     out = {lds + oo, lds + oo + B * W, qo};
     Dft<R, E, SIGN>::run(v);
     __syncthreads();
+    // q E + r stays inside one 32-entry padding period (E divides 32)
+    const int pq = pos<1>(in.q * E);
 #pragma unroll
     for (int r = 0; r < E; r++) {
-      const int p = pos<1>(in.q * E + r);
-      in.re[p] = v[r].x;
-      in.im[p] = v[r].y;
+      in.re[pq + r] = v[r].x;
+      in.im[pq + r] = v[r].y;
     }
     stages<E, TO_LDS>(v, natural(lds, tid), out, stw);
   }

@@ -546,7 +546,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
     }
     velmax_commit(vm * (double)N, velmax);
   }
-#pragma unroll 1
+#pragma unroll
   for (int s = 0; s < 5; s++) {
     C<R> q[E];
 #pragma unroll
