@@ -189,7 +189,10 @@ struct BufView {
   // kOob in range)
   __device__ __forceinline__ BufView(const void* base, unsigned bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000)) {}
+  // (the offset passes through an empty asm statement so that a select
+  // feeding it is not turned into a branch around the instruction)
   __device__ __forceinline__ C<R> load(unsigned off) const {
+    asm volatile("" : "+v"(off));
     if constexpr (sizeof(R) == 4) {
       const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
       return {__uint_as_float(d[0]), __uint_as_float(d[1])};
@@ -199,6 +202,7 @@ struct BufView {
     }
   }
   __device__ __forceinline__ void store(unsigned off, C<R> v) const {
+    asm volatile("" : "+v"(off));
     if constexpr (sizeof(R) == 4) {
       using V2 = unsigned __attribute__((ext_vector_type(2)));
       V2 d = {__float_as_uint(v.x), __float_as_uint(v.y)};
@@ -574,92 +578,147 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Forward z pass (transform.f:129-138) and the convective combination
 // (transform.f:169-176,229-234): C1 = i kx P1 + i kz P5, C3 = i kx P5 + i kz P3
-// (z Nyquist skipped, it is outside the cylinder).  X -> W1: a work-group
-// owns one y and B consecutive kx; the first stage gathers X in 64-byte runs
-// along kx (transposed mapping), the last stage leaves z lines in the
-// natural mapping for contiguous stores of the cylinder z.  MODE 0:
-// transform (5 slots in, C1 P2 C3 P4 out); MODE 1: xktran (nv slots,
-// x 1/(ny nz), xktran.f:121,150-157).
+// (z Nyquist skipped, it is outside the cylinder).  X -> W1: a tile is one y
+// and B consecutive kx; the first stage gathers X in 64-byte runs along kx
+// (transposed mapping), the last stage leaves z lines in the natural mapping
+// for contiguous stores of the cylinder z (the cut is skipped).
 // ---------------------------------------------------------------------------
 #ifndef PSDNS_ZFWD_E
-#define PSDNS_ZFWD_E 16
+#define PSDNS_ZFWD_E 8
 #endif
-template <typename R, int N, int MODE>
-__global__ void __launch_bounds__((RunLines<R>::B * Team<N, PSDNS_ZFWD_E>::T))
-    k_z_fwd(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw, int nv) {
-  constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
+template <typename R, int N>
+struct ZFwd {
+  static constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
+  static constexpr int NT = B * T;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
+  using Buf = Regs<R, E>;
+
+  // gathered loads of variable var of tile t = y * nkb + kx block
+  static __device__ __forceinline__ void load(Buf& b, const C<R>* __restrict__ xs, const Geo& g, int nkb, int t,
+                                              int var, bool valid, int lt0, int q0) {
+    const int y = t / nkb, kx0 = (t % nkb) * B;
+    const BufView<R> src(xs + xidx(var, y, 0, kx0, g), valid ? kOob : 0u);
+    const int l = opaque(lt0), q = opaque(q0);
+    const bool act = kx0 + l < g.max_al_x;
+#pragma unroll
+    for (int j = 0; j < E; j++)
+      b.v[j] = src.load(act ? ((unsigned)(q + T * j) * g.nxh + l) * (unsigned)sizeof(C<R>) : kOob);
+  }
+  // stores of the cylinder z of a transformed variable (natural mapping)
+  static __device__ __forceinline__ void store(const Buf& b, C<R>* __restrict__ w1, const Geo& g,
+                                               const int* __restrict__ cutz, int nkb, int t, int var, int ln0,
+                                               int q0) {
+    const int y = t / nkb, kx0 = (t % nkb) * B;
+    const int l = T >= 64 ? __builtin_amdgcn_readfirstlane(opaque(ln0)) : opaque(ln0), q = opaque(q0);
+    const int kx = kx0 + l;
+    const bool act = kx < g.max_al_x;
+    const int c = cutz[act ? kx : 0];
+    const BufView<R> dst(w1 + w1idx(var, kx0, y, 0, g), kOob);
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int z = q + T * m;
+      const bool keep = act && (z < N / 2 - c || z > N / 2 + c);
+      dst.store(keep ? ((unsigned)l * N * N + z) * (unsigned)sizeof(C<R>) : kOob, b.v[m]);
+    }
+  }
+};
+
+// transform (MODE 0 of the z pass): 5 slots in, C1 P2 C3 P4 out.  Each
+// variable's loads are issued before the transform of the previous one,
+// with three register buffers in fixed roles: A holds P5 (loaded during the
+// previous tile), B takes P1 then P2, C takes P3 then P4.
+template <typename R, int N>
+__global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_z_fwd(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw,
+            const int* __restrict__ cutz) {
+  using Z = ZFwd<R, N>;
+  using F = typename Z::F;
+  constexpr int T = Z::T, E = Z::E, B = Z::B;
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   const int tid = threadIdx.x;
-  const int nkb = (g.max_al_x + B - 1) / B;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const int y = bx / nkb, kx0 = (bx % nkb) * B;
   const Slot<R> in = F::transposed(lds, tid);
   const Slot<R> out = F::natural(lds, tid);
-  const int ki = kx0 + F::transposed_line(tid);
-  const int kx = kx0 + F::natural_line(tid);
-  const bool act = kx < g.max_al_x;
-  const int c = g.cutz[act ? kx : 0];
-  auto load = [&](C<R>* v, int var) {
-    if (ki < g.max_al_x) {
-      const C<R>* p = xs + xidx(var, y, 0, ki, g);
+  const int lt0 = F::transposed_line(tid), ln0 = F::natural_line(tid);
+  const int nkb = (g.max_al_x + B - 1) / B;
+  __syncthreads();
+  const TileWalk w = tile_walk(N * nkb);
+  int t = w.t;
+  if (t >= w.end) return;
+  auto fft = [&](typename Z::Buf& b) { F::template run<>(b.v, lds, in, out, stw); };
+  // o = i (kx a + kz b) in the natural mapping
+  auto conv = [&](typename Z::Buf& o, const typename Z::Buf& a, const typename Z::Buf& b) {
+    const R kxr = (R)((t % nkb) * B + F::natural_line(opaque(tid)));
+    const int q = opaque(out.q);
 #pragma unroll
-      for (int r = 0; r < E; r++) v[r] = p[(size_t)(in.q + T * r) * g.nxh];
-    } else {
-#pragma unroll
-      for (int r = 0; r < E; r++) v[r] = {0, 0};
+    for (int j = 0; j < E; j++) {
+      const R kz = (R)kwave(q + T * j, N);
+      const C<R> s = {kxr * a.v[j].x + kz * b.v[j].x, kxr * a.v[j].y + kz * b.v[j].y};
+      o.v[j] = {-s.y, s.x};
     }
   };
-  auto store = [&](const C<R>* v, int var) {
-    if (!act) return;
-    C<R>* p = w1 + w1idx(var, kx, y, 0, g);
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const int zz = out.q + T * j;
-      if (zz < N / 2 - c || zz > N / 2 + c) p[zz] = v[j];
-    }
-  };
-  if constexpr (MODE == 0) {
-    C<R> p5[E], w[E];
-    load(p5, 4);
-    F::template run<>(p5, lds, in, out, stw);
-    load(w, 0);
-    F::template run<>(w, lds, in, out, stw);
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
-      const C<R> a = {kxr * w[j].x + kz * p5[j].x, kxr * w[j].y + kz * p5[j].y};
-      w[j] = {-a.y, a.x};  // i (kx P1 + kz P5)
-    }
-    store(w, 0);
-    load(w, 2);
-    F::template run<>(w, lds, in, out, stw);
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const R kz = (R)kwave(out.q + T * j, N), kxr = (R)kx;
-      const C<R> a = {kxr * p5[j].x + kz * w[j].x, kxr * p5[j].y + kz * w[j].y};
-      w[j] = {-a.y, a.x};  // i (kx P5 + kz P3)
-    }
-    store(w, 2);
-    load(w, 1);
-    F::template run<>(w, lds, in, out, stw);
-    store(w, 1);
-    load(w, 3);
-    F::template run<>(w, lds, in, out, stw);
-    store(w, 3);
-  } else {
-    const R sc = (R)(1.0 / ((double)N * (double)N));
-    for (int var = 0; var < nv; var++) {
-      C<R> w[E];
-      load(w, var);
-      F::template run<>(w, lds, in, out, stw);
-#pragma unroll
-      for (int j = 0; j < E; j++) w[j] = cscale(w[j], sc);
-      store(w, var);
-    }
+  typename Z::Buf A, Bb, Cc;
+  Z::load(A, xs, g, nkb, t, 4, true, lt0, in.q);
+  for (;;) {
+    Z::load(Bb, xs, g, nkb, t, 0, true, lt0, in.q);
+    issue_fence();
+    fft(A);  // P5
+    Z::load(Cc, xs, g, nkb, t, 2, true, lt0, in.q);
+    issue_fence();
+    fft(Bb);  // P1
+    conv(Bb, Bb, A);
+    Z::store(Bb, w1, g, cutz, nkb, t, 0, ln0, out.q);
+    Z::load(Bb, xs, g, nkb, t, 1, true, lt0, in.q);
+    issue_fence();
+    fft(Cc);  // P3
+    conv(Cc, A, Cc);
+    Z::store(Cc, w1, g, cutz, nkb, t, 2, ln0, out.q);
+    Z::load(Cc, xs, g, nkb, t, 3, true, lt0, in.q);
+    issue_fence();
+    fft(Bb);  // P2
+    Z::store(Bb, w1, g, cutz, nkb, t, 1, ln0, out.q);
+    const int tn = t + w.step;
+    const bool more = tn < w.end;
+    Z::load(A, xs, g, nkb, more ? tn : t, 4, more, lt0, in.q);
+    issue_fence();
+    fft(Cc);  // P4
+    Z::store(Cc, w1, g, cutz, nkb, t, 3, ln0, out.q);
+    if (!more) break;
+    t = tn;
   }
+}
+
+// xktran (MODE 1 of the z pass): nv slots, x 1/(ny nz) (xktran.f:121,150-157);
+// tile t = var * N * nkb + (y * nkb + kx block)
+template <typename R, int N>
+__global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_z_fwd_norm(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw, int nv,
+                 const int* __restrict__ cutz) {
+  using Z = ZFwd<R, N>;
+  using F = typename Z::F;
+  constexpr int E = Z::E, B = Z::B;
+  __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  const int tid = threadIdx.x;
+  const Slot<R> in = F::transposed(lds, tid);
+  const Slot<R> out = F::natural(lds, tid);
+  const int lt0 = F::transposed_line(tid), ln0 = F::natural_line(tid);
+  const int nkb = (g.max_al_x + B - 1) / B, per = N * nkb;
+  const R sc = (R)(1.0 / ((double)N * (double)N));
+  __syncthreads();
+  auto load = [&](typename Z::Buf& b, int t, bool valid) {
+    Z::load(b, xs, g, nkb, t % per, t / per, valid, lt0, in.q);
+  };
+  auto finish = [&](typename Z::Buf& b, int t) {
+    F::template run<>(b.v, lds, in, out, stw);
+#pragma unroll
+    for (int j = 0; j < E; j++) b.v[j] = cscale(b.v[j], sc);
+    Z::store(b, w1, g, cutz, nkb, t % per, t / per, ln0, out.q);
+  };
+  typename Z::Buf ba, bb;
+  tile_loop(nv * per, ba, bb, load, finish);
 }
 
 // ---------------------------------------------------------------------------
@@ -1198,15 +1257,18 @@ struct Launch {
                        (const CR*)c.tw, c.velmax_bits, want_velmax);
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
-    constexpr int B = RunLines<R>::B, T = Team<N, PSDNS_ZFWD_E>::T;
-    const int nkb = (c.geo.max_al_x + B - 1) / B;
-    dim3 grid(N * nkb);
-    if (mode == 0)
-      hipLaunchKernelGGL((k_z_fwd<R, N, 0>), grid, dim3(B * T), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
-                         (const CR*)c.tw, nv);
-    else
-      hipLaunchKernelGGL((k_z_fwd<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
-                         (const CR*)c.tw, nv);
+    constexpr int NT = ZFwd<R, N>::NT;
+    if (mode == 0) {
+      const auto kern = k_z_fwd<R, N>;
+      static const unsigned grid = persistent_grid((const void*)kern, NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
+                         (const CR*)c.tw, c.geo.cutz);
+    } else {
+      const auto kern = k_z_fwd_norm<R, N>;
+      static const unsigned grid = persistent_grid((const void*)kern, NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const CR*)c.xs, (CR*)c.w1, c.geo,
+                         (const CR*)c.tw, nv, c.geo.cutz);
+    }
   }
   static void y_fwd(const Ctx& c, int nv) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
