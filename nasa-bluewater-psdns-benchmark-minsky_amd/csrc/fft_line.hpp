@@ -292,11 +292,19 @@ struct TeamFft {
   static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* stw) {
     constexpr int NB = E / RAD;  // butterflies per thread
     cpx<R> w[E];
+    // the twiddle table is constant: its reads are issued ahead of the
+    // barrier and overlap the wait
+    cpx<R> tws[NB][RAD - 1];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int jm = (s.q + T * i) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RAD; r++) tws[i][r - 1] = tw_at<NS, RAD>(stw, jm, r);
+    }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       const int j = s.q + T * i;
-      const int jm = j & (NS - 1);
       // buffer written by the stage of span NS/E; L/RAD is a multiple of NS,
       // hence of that buffer's padding period
       const int pj = pos<NS / E>(j);
@@ -306,8 +314,7 @@ struct TeamFft {
         w[i * RAD + r] = {s.re[p], s.im[p]};
       });
 #pragma unroll
-      for (int r = 1; r < RAD; r++)
-        w[i * RAD + r] = cmul(w[i * RAD + r], tw_at<NS, RAD>(stw, jm, r));
+      for (int r = 1; r < RAD; r++) w[i * RAD + r] = cmul(w[i * RAD + r], tws[i][r - 1]);
       Dft<R, RAD, SIGN>::run(w + i * RAD);
     }
     if constexpr (OUT == 1) {
@@ -355,17 +362,21 @@ struct TeamFft {
   // Middle stages use the natural mapping.  Every thread of the work-group
   // calls it (barriers inside; the first one protects earlier LDS reads).
   //
-  // The thread's slot coordinates pass through an empty asm statement, so
-  // that in a kernel looping over tiles the LDS addresses and twiddle loads
-  // of the stages are formed inside each call instead of being hoisted out
-  // of the loop and held in registers across it.
+  // With FRESH, the thread's slot coordinates pass through an empty asm
+  // statement, so that in a kernel looping over tiles the LDS addresses and
+  // twiddle reads of the stages are formed inside each call instead of
+  // being hoisted out of the loop and held in registers across it; without
+  // it, calls on the same slots share their address arithmetic.
   // stw is the LDS twiddle table (fill_twiddles).
-  template <bool TO_LDS = false>
+  template <bool TO_LDS = false, bool FRESH = true>
   static __device__ __forceinline__ void run(cpx<R>* v, R* lds, Slot<R> in, Slot<R> out, const cpx<R>* stw) {
-    int oi = (int)(in.re - lds), qi = in.q, oo = (int)(out.re - lds), qo = out.q, tid = (int)threadIdx.x;
-    asm volatile("" : "+v"(oi), "+v"(qi), "+v"(oo), "+v"(qo), "+v"(tid));
-    in = {lds + oi, lds + oi + B * W, qi};
-    out = {lds + oo, lds + oo + B * W, qo};
+    int tid = (int)threadIdx.x;
+    if constexpr (FRESH) {
+      int oi = (int)(in.re - lds), qi = in.q, oo = (int)(out.re - lds), qo = out.q;
+      asm volatile("" : "+v"(oi), "+v"(qi), "+v"(oo), "+v"(qo), "+v"(tid));
+      in = {lds + oi, lds + oi + B * W, qi};
+      out = {lds + oo, lds + oo + B * W, qo};
+    }
     Dft<R, E, SIGN>::run(v);
     __syncthreads();
     // q E + r stays inside one 32-entry padding period (E divides 32)

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 #pragma unroll
       for (int j = 0; j < E; j++) b.v[j] = cmul(b.v[j], cmul(ssy[q + T * j], sxz));
     }
-    F::template run<>(b.v, lds, io, io, stw);
+    F::template run<false, false>(b.v, lds, io, io, stw);
     const int li = line_of(), q = opaque(io.q);
     const BufView<R> dst(yw + ((size_t)var * g.na + a0) * N, kOob);
     const bool act = a0 + li < g.na;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
     }
   };
   auto finish = [&](Regs<R, E>& b, int t) {
-    F::template run<>(b.v, lds, in, out, stw);
+    F::template run<false, false>(b.v, lds, in, out, stw);
     const int vk = t / NYB, y0 = (t % NYB) * B;
     const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
     const int l = opaque(ln0), q = opaque(out.q);
@@ -372,6 +372,11 @@ struct XPass {
   using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN>;
   static constexpr int kLdsReals = FI::kLdsReals;
   static constexpr int kTw = FI::kTwEntries;  // shared by FI and FF
+#ifndef PSDNS_XFRESH
+#define PSDNS_XFRESH false
+#endif
+  // the transforms of a tile share their address arithmetic
+  static constexpr bool XFRESH = PSDNS_XFRESH;
 
   // LDS tables of a work-group: stage twiddles (from tw of length N, every
   // second entry) and xtw[k] = exp(-2 pi i k / N), k < M, for the real-data
@@ -396,7 +401,7 @@ struct XPass {
   };
   static __device__ __forceinline__ void c2r_load(Raw& r, const C<R>* __restrict__ xi, int var, int y, int z0,
                                                   const Geo& g) {
-    const int tid = opaque(threadIdx.x);
+    const int tid = threadIdx.x;
     const int l = FI::transposed_line(tid), q = tid / B;
     const BufView<R> src(xi + xiidx(var, y, 0, z0, g), kOob);
 #pragma unroll
@@ -424,7 +429,7 @@ struct XPass {
       const C<R> wd = cmul(w, d);
       v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
-    FI::template run<>(v, lds, in, out, stw);
+    FI::template run<false, true>(v, lds, in, out, stw);
   }
   static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xi, int var, int y, int z0,
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
@@ -441,22 +446,24 @@ struct XPass {
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw, R scale) {
     const int tid = threadIdx.x;
     const Slot<R> s = FF::natural(lds, tid);
-    FF::template run<true>(v, lds, s, s, stw);
+    FF::template run<true, XFRESH>(v, lds, s, s, stw);
     __syncthreads();
-    C<R>* line = xs + xidx(var, y, z0, 0, g) + FF::natural_line(tid) * g.nxh;
+    // no branch per point: kx >= max_al_x lanes store nothing (out of range)
+    const BufView<R> dst(xs + xidx(var, y, z0, 0, g), kOob);
+    const int q = s.q;
+    const unsigned lofs = (unsigned)(FF::natural_line(tid) * g.nxh);
     const R half = (R)0.5 * scale;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = s.q + T * j;
-      if (k < g.max_al_x) {
-        const int pk = FF::out_pos(k), pm = FF::out_pos(k == 0 ? 0 : M - k);
-        const C<R> zk = {s.re[pk], s.im[pk]};
-        const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
-        const C<R> sm = zk + zm;
-        const C<R> d = zk - zm;
-        const C<R> wd = cmul(xtw[k], d);
-        line[k] = {(sm.x + wd.y) * half, (sm.y - wd.x) * half};
-      }
+      const int k = q + T * j;
+      const int pk = FF::out_pos(k), pm = FF::out_pos(k == 0 ? 0 : M - k);
+      const C<R> zk = {s.re[pk], s.im[pk]};
+      const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
+      const C<R> sm = zk + zm;
+      const C<R> d = zk - zm;
+      const C<R> wd = cmul(xtw[k], d);
+      dst.store(k < g.max_al_x ? (lofs + k) * (unsigned)sizeof(C<R>) : kOob,
+                C<R>{(sm.x + wd.y) * half, (sm.y - wd.x) * half});
     }
   }
 };
@@ -521,7 +528,7 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
 // (+ velmax, FP64, realspace_vel.f:108-110) -> r2c x 1/n of the five
 // products into X slots 0..4.  Physical space stays in registers.
 template <typename R, int N>
-__global__ void __launch_bounds__((XPass<R, N>::NT))
+__global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_x_fused(const C<R>* __restrict__ xi, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
               unsigned long long* __restrict__ velmax, int want_velmax) {
   using X = XPass<R, N>;
@@ -646,7 +653,7 @@ __global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_
   const TileWalk w = tile_walk(N * nkb);
   int t = w.t;
   if (t >= w.end) return;
-  auto fft = [&](typename Z::Buf& b) { F::template run<>(b.v, lds, in, out, stw); };
+  auto fft = [&](typename Z::Buf& b) { F::template run<false, false>(b.v, lds, in, out, stw); };
   // o = i (kx a + kz b) in the natural mapping
   auto conv = [&](typename Z::Buf& o, const typename Z::Buf& a, const typename Z::Buf& b) {
     const R kxr = (R)((t % nkb) * B + F::natural_line(opaque(tid)));
@@ -712,7 +719,7 @@ __global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_
     Z::load(b, xs, g, nkb, t % per, t / per, valid, lt0, in.q);
   };
   auto finish = [&](typename Z::Buf& b, int t) {
-    F::template run<>(b.v, lds, in, out, stw);
+    F::template run<false, false>(b.v, lds, in, out, stw);
 #pragma unroll
     for (int j = 0; j < E; j++) b.v[j] = cscale(b.v[j], sc);
     Z::store(b, w1, g, cutz, nkb, t % per, t / per, ln0, out.q);
