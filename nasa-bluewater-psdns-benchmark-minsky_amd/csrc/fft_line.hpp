@@ -288,7 +288,26 @@ struct TeamFft {
   // Stockham stage of radix RAD after span NS.  Reads the stage input from
   // LDS line s.  OUT 0: results back to LDS (Stockham positions);
   // OUT 1: final results into registers (v[m] = X[q + T m]).
-  template <int RAD, int NS, int OUT>
+  // In the natural mapping a line of T = 64 threads is exactly one
+  // wavefront (wave64), so LDS traffic that stays inside a line needs only
+  // wavefront-scope ordering (a wavefront's LDS operations are executed in
+  // order); traffic across lines needs the work-group barrier.
+  static constexpr bool kWaveLines = (T == 64);
+  template <bool CROSS>
+  static __device__ __forceinline__ void sync() {
+    if constexpr (CROSS || !kWaveLines) {
+      __syncthreads();
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+
+  // CROSS_IN: the buffer read was written in another mapping than the
+  // natural one (the first middle stage after a transposed first stage, or
+  // a last stage reading in the transposed mapping)
+  template <int RAD, int NS, int OUT, bool CROSS_IN>
   static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* stw) {
     constexpr int NB = E / RAD;  // butterflies per thread
     cpx<R> w[E];
@@ -301,7 +320,7 @@ struct TeamFft {
 #pragma unroll
       for (int r = 1; r < RAD; r++) tws[i][r - 1] = tw_at<NS, RAD>(stw, jm, r);
     }
-    __syncthreads();
+    sync<CROSS_IN>();
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       const int j = s.q + T * i;
@@ -323,7 +342,7 @@ struct TeamFft {
 #pragma unroll
         for (int r = 0; r < RAD; r++) v[i + r * NB] = w[i * RAD + r];
     } else {
-      __syncthreads();
+      sync<false>();  // this stage's reads of the line are done
 #pragma unroll
       for (int i = 0; i < NB; i++) {
         const int j = s.q + T * i;
@@ -340,27 +359,32 @@ struct TeamFft {
     }
   }
 
-  template <int NS, bool TO_LDS>
+  template <int NS, bool TO_LDS, bool IN_T, bool OUT_T>
   static __device__ __forceinline__ void stages(cpx<R>* v, Slot<R> mid, Slot<R> out, const cpx<R>* stw) {
     if constexpr (NS < L) {
       constexpr int RAD = StagePlan<L, E, NS>::kRad;
       constexpr bool LAST = (NS * RAD == L);
+      constexpr bool FIRST = (NS == E);
       if constexpr (!LAST) {
-        stage<RAD, NS, 0>(v, mid, stw);
+        stage<RAD, NS, 0, FIRST && IN_T>(v, mid, stw);
       } else if constexpr (TO_LDS) {
-        stage<RAD, NS, 0>(v, out, stw);
+        static_assert(!OUT_T, "results left in LDS in the natural mapping only");
+        stage<RAD, NS, 0, FIRST && IN_T>(v, out, stw);
       } else {
-        stage<RAD, NS, 1>(v, out, stw);
+        stage<RAD, NS, 1, (FIRST && IN_T) || OUT_T>(v, out, stw);
       }
-      stages<NS * RAD, TO_LDS>(v, mid, out, stw);
+      stages<NS * RAD, TO_LDS, IN_T, OUT_T>(v, mid, out, stw);
     }
   }
 
   // v holds points in.q + T*j of line `in`; on return either v holds
   // X[out.q + T*m] of line `out` (TO_LDS = false) or the line in LDS holds
-  // X[k] at out_pos(k) (TO_LDS = true; read after a barrier).
+  // X[k] at out_pos(k) (TO_LDS = true; read after sync<false>()).
   // Middle stages use the natural mapping.  Every thread of the work-group
   // calls it (barriers inside; the first one protects earlier LDS reads).
+  // IN_T / OUT_T: `in` / `out` are in the transposed mapping.  With a
+  // natural `in`, the caller's earlier LDS accesses to the lines must have
+  // been made in the natural mapping too (each line by its own threads).
   //
   // With FRESH, the thread's slot coordinates pass through an empty asm
   // statement, so that in a kernel looping over tiles the LDS addresses and
@@ -368,7 +392,7 @@ struct TeamFft {
   // being hoisted out of the loop and held in registers across it; without
   // it, calls on the same slots share their address arithmetic.
   // stw is the LDS twiddle table (fill_twiddles).
-  template <bool TO_LDS = false, bool FRESH = true>
+  template <bool TO_LDS = false, bool FRESH = true, bool IN_T = true, bool OUT_T = true>
   static __device__ __forceinline__ void run(cpx<R>* v, R* lds, Slot<R> in, Slot<R> out, const cpx<R>* stw) {
     int tid = (int)threadIdx.x;
     if constexpr (FRESH) {
@@ -378,7 +402,7 @@ struct TeamFft {
       out = {lds + oo, lds + oo + B * W, qo};
     }
     Dft<R, E, SIGN>::run(v);
-    __syncthreads();
+    sync<IN_T>();
     // q E + r stays inside one 32-entry padding period (E divides 32)
     const int pq = pos<1>(in.q * E);
 #pragma unroll
@@ -386,7 +410,7 @@ struct TeamFft {
       in.re[pq + r] = v[r].x;
       in.im[pq + r] = v[r].y;
     }
-    stages<E, TO_LDS>(v, natural(lds, tid), out, stw);
+    stages<E, TO_LDS, IN_T, OUT_T>(v, natural(lds, tid), out, stw);
   }
 };
 

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 #pragma unroll
       for (int j = 0; j < E; j++) b.v[j] = cmul(b.v[j], cmul(ssy[q + T * j], sxz));
     }
-    F::template run<false, false>(b.v, lds, io, io, stw);
+    F::template run<false, false, false, false>(b.v, lds, io, io, stw);
     const int li = line_of(), q = opaque(io.q);
     const BufView<R> dst(yw + ((size_t)var * g.na + a0) * N, kOob);
     const bool act = a0 + li < g.na;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
     }
   };
   auto finish = [&](Regs<R, E>& b, int t) {
-    F::template run<false, false>(b.v, lds, in, out, stw);
+    F::template run<false, false, true, false>(b.v, lds, in, out, stw);
     const int vk = t / NYB, y0 = (t % NYB) * B;
     const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
     const int l = opaque(ln0), q = opaque(out.q);
@@ -429,7 +429,7 @@ struct XPass {
       const C<R> wd = cmul(w, d);
       v[j] = {sm.x - wd.y, sm.y + wd.x};
     }
-    FI::template run<false, true>(v, lds, in, out, stw);
+    FI::template run<false, true, true, false>(v, lds, in, out, stw);
   }
   static __device__ __forceinline__ void c2r(C<R>* v, const C<R>* __restrict__ xi, int var, int y, int z0,
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw) {
@@ -446,8 +446,8 @@ struct XPass {
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw, R scale) {
     const int tid = threadIdx.x;
     const Slot<R> s = FF::natural(lds, tid);
-    FF::template run<true, XFRESH>(v, lds, s, s, stw);
-    __syncthreads();
+    FF::template run<true, XFRESH, false, false>(v, lds, s, s, stw);
+    FF::template sync<false>();
     // no branch per point: kx >= max_al_x lanes store nothing (out of range)
     const BufView<R> dst(xs + xidx(var, y, z0, 0, g), kOob);
     const int q = s.q;
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_
   const TileWalk w = tile_walk(N * nkb);
   int t = w.t;
   if (t >= w.end) return;
-  auto fft = [&](typename Z::Buf& b) { F::template run<false, false>(b.v, lds, in, out, stw); };
+  auto fft = [&](typename Z::Buf& b) { F::template run<false, false, true, false>(b.v, lds, in, out, stw); };
   // o = i (kx a + kz b) in the natural mapping
   auto conv = [&](typename Z::Buf& o, const typename Z::Buf& a, const typename Z::Buf& b) {
     const R kxr = (R)((t % nkb) * B + F::natural_line(opaque(tid)));
@@ -719,7 +719,7 @@ __global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_
     Z::load(b, xs, g, nkb, t % per, t / per, valid, lt0, in.q);
   };
   auto finish = [&](typename Z::Buf& b, int t) {
-    F::template run<false, false>(b.v, lds, in, out, stw);
+    F::template run<false, false, true, false>(b.v, lds, in, out, stw);
 #pragma unroll
     for (int j = 0; j < E; j++) b.v[j] = cscale(b.v[j], sc);
     Z::store(b, w1, g, cutz, nkb, t % per, t / per, ln0, out.q);
@@ -764,7 +764,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
   C<R> v[E];
   y_gather<R, N, 16>(v, w1, var, g, grp, F::transposed_line(tid), in.q);
   const Slot<R> out = F::natural(lds, tid);
-  F::template run<>(v, lds, in, out, stw);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
   const int l = F::natural_line(tid);
   if (l < grp.y) {
     C<R>* col = u + ((size_t)var * g.na + grp.x + l) * N;
@@ -966,7 +966,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3
   C<R> a1[E], a2[E], a3[E], v[E];
   y_gather<R, N, 8>(v, w1, 1, g, grp, lin, in.q);  // P2
-  F::template run<>(v, lds, in, out, stw);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
@@ -974,11 +974,11 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     a2[m] = cscale(v[m], (R)x);
   }
   y_gather<R, N, 8>(v, w1, 0, g, grp, lin, in.q);  // C1
-  F::template run<>(v, lds, in, out, stw);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) a1[m] = a1[m] + v[m];
   y_gather<R, N, 8>(v, w1, 3, g, grp, lin, in.q);  // P4
-  F::template run<>(v, lds, in, out, stw);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
@@ -986,7 +986,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     a3[m] = {-ky * v[m].y, ky * v[m].x};
   }
   y_gather<R, N, 8>(v, w1, 2, g, grp, lin, in.q);  // C3
-  F::template run<>(v, lds, in, out, stw);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
 #pragma unroll
   for (int m = 0; m < E; m++) a3[m] = a3[m] + v[m];
   const C<R>* sxr = sr;
