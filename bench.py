@@ -43,7 +43,7 @@ def pass_bytes(n, s, num_al, max_al_x):
     outputs at storage precision s; complex = 2s):
       y-inverse   3 vars: read cylinder, write z-pencil columns     2 * 3 * num_al * n * 2s
       z-inverse   3 vars: read cylinder columns, write full lines   3 * n * max_al_x * (nkept + n) * 2s
-      x fused     3 spectral in, 5 spectral out (n/2 per x line)    8 * n^2 * (n/2) * 2s
+      x fused     3 spectral in, 5 spectral out (kx < max_al_x)     8 * n^2 * max_al_x * 2s
       z-forward   5 lines in, 4 cylinder columns out                (5 * n * max_al_x * n + 4 * num_al * n) * 2s
       y-fwd+upd   4 vars gathered + un read, u and un written       (4 + 3 + 3 + 3) * num_al * n * 2s
     (nkept * max_al_x = num_al)."""
@@ -51,7 +51,7 @@ def pass_bytes(n, s, num_al, max_al_x):
     return {
         "y_inverse": 2 * 3 * num_al * n * c,
         "z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
-        "x_fused": 8 * n * n * (n // 2) * c,
+        "x_fused": 8 * n * n * max_al_x * c,
         "z_forward": (5 * n * max_al_x * n + 4 * num_al * n) * c,
         "y_forward_update": 13 * num_al * n * c,
     }

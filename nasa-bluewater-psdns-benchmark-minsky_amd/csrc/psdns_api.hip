@@ -134,8 +134,12 @@ int upload_factors(psdns_ctx* c, const double* xfac, const double* yfac, const d
   return 0;
 }
 
+constexpr size_t kVelmaxBytes = sizeof(unsigned long long) * psdns::kVelmaxSlots * psdns::kVelmaxStride;
+
 int read_velmax(psdns_ctx* c, double* velmax) {
   unsigned long long bits = 0;
+  psdns::launch_velmax_reduce(c->c);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&bits, c->d_velmax, sizeof bits, hipMemcpyDeviceToHost, c->c.stream));
   HIPCHK(hipStreamSynchronize(c->c.stream));
   double v;
@@ -242,7 +246,7 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_shiftr, c->csize() * 2 * (c->nxh + 2 * n));
   ALLOC(c->d_fac, sizeof(double) * (c->nxh + 2 * n));
   ALLOC(c->d_red, sizeof(double) * (2 + 3 * (size_t)n + 64));
-  ALLOC(c->d_velmax, sizeof(unsigned long long));
+  ALLOC(c->d_velmax, kVelmaxBytes);
   CHK(hipMemcpy(c->d_colx, c->colx.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_colz, c->colz.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
   CHK(hipMemcpy(c->d_cutz, c->cutz.data(), sizeof(int) * c->max_al_x, hipMemcpyHostToDevice));
@@ -403,7 +407,7 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
   c->c.kstep = kstep - 1;
   int rc;
-  if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, sizeof(unsigned long long), c->c.stream));
+  if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
   if ((rc = pass(c, Pass::YInvPhase, 3))) return rc;
   if ((rc = mark(c, 1))) return rc;
@@ -610,7 +614,7 @@ int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
   if (int rc = check_ctx(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
-  HIPCHK(hipMemsetAsync(c->d_velmax, 0, sizeof(unsigned long long), c->c.stream));
+  HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   psdns::launch_realspace(c->c, rkstep == 1);
   HIPCHK(hipGetLastError());
   if (rkstep == 1) return read_velmax(c, velmax);

@@ -25,6 +25,10 @@ struct Geo {
   int ngrp;
 };
 
+// velmax partial maxima: slots 256 bytes apart (u64 bit patterns)
+constexpr int kVelmaxSlots = 512;
+constexpr int kVelmaxStride = 32;
+
 // columns per y-pass group (64-byte runs of complex values in W)
 inline int y_group_lines(int prec) { return prec == 8 ? 4 : 8; }
 
@@ -51,7 +55,7 @@ struct Ctx {
   const double* xfac;
   const double* yfac;
   const double* zfac;
-  unsigned long long* velmax_bits;
+  unsigned long long* velmax_bits;  // kVelmaxSlots slots, kVelmaxStride apart
   const int2* force_list;  // (a, y) entries inside the FEK forcing sphere
   int nforce;
 };
@@ -74,6 +78,8 @@ int launch_pass(const Ctx& c, Pass p, int a0, int a1);
 int launch_y_update(const Ctx& c, int stage, double norm);
 void launch_phshift(const Ctx& c, int nv);
 void launch_realspace(const Ctx& c, int want_velmax);
+// fold the velmax slots into slot 0
+void launch_velmax_reduce(const Ctx& c);
 void launch_wavespace(const Ctx& c, double norm);
 void launch_update(const Ctx& c, int stage);
 void launch_wavespace_update(const Ctx& c, double norm, int stage);

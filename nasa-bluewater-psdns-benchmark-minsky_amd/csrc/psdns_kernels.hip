@@ -506,10 +506,36 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   X::r2c(v, xs, var, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
 }
 
+// velmax (realspace_vel.f:108-110) is a maximum over the whole grid: each
+// work-group reduces its threads' values (wavefront shuffles, then LDS), and
+// one thread applies it with an atomic max to one of kVelmaxSlots slots,
+// 256 bytes apart, chosen by work-group, so that the atomics spread over
+// the L2 channels instead of queueing on one address; k_velmax_reduce folds
+// the slots into slot 0.  Non-negative doubles order as their bit patterns.
+// Every thread of the work-group calls it (one barrier inside).
 __device__ __forceinline__ void velmax_commit(double m, unsigned long long* velmax) {
-  // wave reduction, one atomic per wave (non-negative doubles order as integers)
+  __shared__ double wmax[16];
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(velmax, (unsigned long long)__double_as_longlong(m));
+  const int nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wmax[0];
+    for (int i = 1; i < nw; i++) b = fmax(b, wmax[i]);
+    atomicMax(velmax + (size_t)(blockIdx.x % kVelmaxSlots) * kVelmaxStride,
+              (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+__global__ void __launch_bounds__(kVelmaxSlots) k_velmax_reduce(unsigned long long* velmax) {
+  __shared__ unsigned long long s[kVelmaxSlots];
+  s[threadIdx.x] = velmax[(size_t)threadIdx.x * kVelmaxStride];
+  __syncthreads();
+  for (int h = kVelmaxSlots / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] > s[threadIdx.x + h] ? s[threadIdx.x] : s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) velmax[0] = s[0];
 }
 
 // realspace_vel.f:105-113 (b = 1): slot s of u^2-v^2, uv, w^2-v^2, vw, uw
@@ -1411,6 +1437,10 @@ int launch_y_update(const Ctx& c, int stage, double norm) {
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1) {
   return c.prec == 8 ? run_pass_n<double>(c, p, a0, a1) : run_pass_n<float>(c, p, a0, a1);
+}
+
+void launch_velmax_reduce(const Ctx& c) {
+  hipLaunchKernelGGL(k_velmax_reduce, dim3(1), dim3(kVelmaxSlots), 0, c.stream, c.velmax_bits);
 }
 
 void launch_realspace(const Ctx& c, int want_velmax) {
