@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/psdns_gpu.h"
+#include "psdns_gpu.h"
 #include "psdns_internal.h"
 
 using psdns::Ctx;

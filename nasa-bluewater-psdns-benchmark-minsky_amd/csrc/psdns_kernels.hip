@@ -972,81 +972,164 @@ template <typename R, int N, int STAGE>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
                    const C<R>* __restrict__ sr, const double* __restrict__ xfac, const double* __restrict__ yfac,
-                   const double* __restrict__ zfac, const C<R>* __restrict__ tw) {
+                   const double* __restrict__ zfac, const C<R>* __restrict__ tw, const int2* __restrict__ grp,
+                   const int* __restrict__ colx, const int* __restrict__ colz) {
   constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
+  using Buf = Regs<R, E>;
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
+  __shared__ C<R> ssy[N];
+  __shared__ double syf[STAGE == 1 ? N : 1];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  for (int k = threadIdx.x; k < N; k += F::NT) {
+    ssy[k] = sr[g.nxh + k];
+    if constexpr (STAGE == 1) syf[k] = yfac[k];
+  }
   const int tid = threadIdx.x;
   const Slot<R> in = F::transposed(lds, tid);
   const Slot<R> out = F::natural(lds, tid);
-  const int2 grp = g.grp[xcd_block(blockIdx.x, gridDim.x)];
-  const int lin = F::transposed_line(tid);
-  const int ln = F::natural_line(tid);
-  const bool act = ln < grp.y;
-  const int a = grp.x + (act ? ln : 0);
-  const int x = g.colx[a], z = g.colz[a], kz = kwave(z, N);
+  const int2 gr = grp[xcd_block(blockIdx.x, gridDim.x)];
+  const int x = colx[gr.x], z0 = colz[gr.x];
+  // gathered loads of one variable (64-byte runs along z); lanes past the
+  // group read nothing
+  const int lt = F::transposed_line(tid);
+  auto gather = [&](Buf& b, int var) {
+    const BufView<R> src(w1 + w1idx(var, x, 0, z0, g), kOob);
+    const bool act = lt < gr.y;
+#pragma unroll
+    for (int j = 0; j < E; j++)
+      b.v[j] = src.load(act ? ((unsigned)(in.q + T * j) * N + lt) * (unsigned)sizeof(C<R>) : kOob);
+  };
+  auto fft = [&](Buf& b) { F::template run<false, true, true, false>(b.v, lds, in, out, stw); };
   // shift-free sums (wavespace_vel.f:165-170 with sxyz factored out):
   // A1 = C1 + i ky P2, A2 = kx P2 + kz P4, A3 = C3 + i ky P4;
-  // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3
-  C<R> a1[E], a2[E], a3[E], v[E];
-  y_gather<R, N, 8>(v, w1, 1, g, grp, lin, in.q);  // P2
-  F::template run<false, true, true, false>(v, lds, in, out, stw);
+  // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3.  Each variable's loads are
+  // issued before the transform of the previous one.
+  const int ln = F::natural_line(tid);
+  const int z = z0 + ln, kz = kwave(z, N);
+  C<R> a1[E], a2[E], a3[E];
+  Buf X, Y;
+  gather(X, 1);  // P2
+  gather(Y, 0);  // C1
+  issue_fence();
+  fft(X);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
-    a1[m] = {-ky * v[m].y, ky * v[m].x};
-    a2[m] = cscale(v[m], (R)x);
+    a1[m] = {-ky * X.v[m].y, ky * X.v[m].x};
+    a2[m] = cscale(X.v[m], (R)x);
   }
-  y_gather<R, N, 8>(v, w1, 0, g, grp, lin, in.q);  // C1
-  F::template run<false, true, true, false>(v, lds, in, out, stw);
+  gather(X, 3);  // P4
+  issue_fence();
+  fft(Y);
 #pragma unroll
-  for (int m = 0; m < E; m++) a1[m] = a1[m] + v[m];
-  y_gather<R, N, 8>(v, w1, 3, g, grp, lin, in.q);  // P4
-  F::template run<false, true, true, false>(v, lds, in, out, stw);
+  for (int m = 0; m < E; m++) a1[m] = a1[m] + Y.v[m];
+  gather(Y, 2);  // C3
+  issue_fence();
+  fft(X);
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const R ky = (R)kwave(out.q + T * m, N);
-    a2[m] = a2[m] + cscale(v[m], (R)kz);
-    a3[m] = {-ky * v[m].y, ky * v[m].x};
+    a2[m] = a2[m] + cscale(X.v[m], (R)kz);
+    a3[m] = {-ky * X.v[m].y, ky * X.v[m].x};
   }
-  y_gather<R, N, 8>(v, w1, 2, g, grp, lin, in.q);  // C3
-  F::template run<false, true, true, false>(v, lds, in, out, stw);
+  fft(Y);
 #pragma unroll
-  for (int m = 0; m < E; m++) a3[m] = a3[m] + v[m];
-  const C<R>* sxr = sr;
-  const C<R>* syr = sr + g.nxh;
-  const C<R>* szr = sr + g.nxh + N;
-  const C<R> sxz = cmul(szr[z], sxr[x]);
-  const R rn = (R)norm;
-  if (!act) return;
+  for (int m = 0; m < E; m++) a3[m] = a3[m] + Y.v[m];
+
+  // wavespace_vel + proc4a / proc4b in the natural mapping.  nl replaces
+  // a1..a3 in place; un is read and used one component at a time (the
+  // three components at once do not fit the registers of 16 wavefronts per
+  // CU).  Off the mask u is zero and un is not written.
+  const bool act = ln < gr.y;
   const size_t tot = (size_t)g.na * N;
+  const size_t col = (size_t)gr.x * N;
+  const BufView<R> du0(u + col, kOob), du1(u + tot + col, kOob), du2(u + 2 * tot + col, kOob);
+  const BufView<R> dn0(un + col, kOob), dn1(un + tot + col, kOob), dn2(un + 2 * tot + col, kOob);
+  const unsigned off0 = (unsigned)(((size_t)ln * N + out.q) * sizeof(C<R>));
+  auto ofs = [&](int m) { return off0 + (unsigned)(T * m * sizeof(C<R>)); };
+  unsigned on = 0;  // mask bits of the thread's points
 #pragma unroll
   for (int m = 0; m < E; m++) {
-    Pt p;
-    p.a = a;
-    p.y = out.q + T * m;
-    p.x = x;
-    p.z = z;
-    p.ky = kwave(p.y, N);
-    p.kz = kz;
-    p.m = in_mask(x, p.ky, kz, p.y, z, N);
-    const size_t i = (size_t)a * N + p.y;
-    if (!p.m) {
+    const int y = out.q + T * m;
+    on |= in_mask(x, kwave(y, N), kz, y, z, N) ? 1u << m : 0u;
+  }
+  Buf U0;
+  {
+    const C<R> sxz = cmul(sr[g.nxh + N + z], sr[x]);
+    const R rn = (R)norm;
 #pragma unroll
-      for (int vv = 0; vv < 3; vv++) u[vv * tot + i] = {0, 0};
-      continue;
+    for (int m = 0; m < E; m++) {
+      const C<R> sp = cmul(sxz, ssy[out.q + T * m]);
+      const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
+      const bool o = (on >> m) & 1u;
+      a1[m] = o ? cmul(sxyz, a1[m]) : C<R>{0, 0};
+      a2[m] = o ? cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]) : C<R>{0, 0};
+      a3[m] = o ? cmul(sxyz, a3[m]) : C<R>{0, 0};
     }
-    const C<R> sp = cmul(sxz, syr[p.y]);
-    const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
-    C<R> nl[3] = {cmul(sxyz, a1[m]), cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]), cmul(sxyz, a3[m])};
-    C<R> o[3] = {un[i], un[tot + i], un[2 * tot + i]};
-    update_point<R, STAGE>(p, nl, o, xfac, yfac, zfac);
+  }
+  // one velocity component: STAGE 1 un' = f (un + N) (stored on the mask),
+  // nl <- f (un + 2 N) (proc4a.f:418-487); STAGE 2 nl <- un + N (proc4b.f:54-67)
+  double fxz = 0.0;
+  if constexpr (STAGE == 1) fxz = xfac[x] * zfac[z];
+  auto component = [&](C<R>* nl, const Buf& b, const BufView<R>& dn) {
 #pragma unroll
-    for (int vv = 0; vv < 3; vv++) {
-      u[vv * tot + i] = nl[vv];
-      un[vv * tot + i] = o[vv];
+    for (int m = 0; m < E; m++) {
+      if constexpr (STAGE == 1) {
+        const R f = (R)(fxz * syf[out.q + T * m]);
+        const C<R> ut = b.v[m];
+        const C<R> un2 = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
+        nl[m] = {f * (ut.x + nl[m].x * (R)2), f * (ut.y + nl[m].y * (R)2)};
+        dn.store(act && ((on >> m) & 1u) ? ofs(m) : kOob, un2);
+      } else {
+        nl[m] = b.v[m] + nl[m];
+      }
+    }
+  };
+#pragma unroll
+  for (int m = 0; m < E; m++) U0.v[m] = dn0.load(act ? ofs(m) : kOob);
+  issue_fence();
+  component(a1, U0, dn0);
+  issue_fence();
+#pragma unroll
+  for (int m = 0; m < E; m++) U0.v[m] = dn1.load(act ? ofs(m) : kOob);
+  issue_fence();
+  component(a2, U0, dn1);
+  issue_fence();
+#pragma unroll
+  for (int m = 0; m < E; m++) U0.v[m] = dn2.load(act ? ofs(m) : kOob);
+  issue_fence();
+  component(a3, U0, dn2);
+  issue_fence();
+  // projection (proc4a.f:540-571 / proc4b.f:85-106, k = 0 excluded; zeroed
+  // in STAGE 2, proc4b.f:176-205); u off the mask is zero
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int y = out.q + T * m, ky = kwave(y, N);
+    const bool o = (on >> m) & 1u;
+    const bool k0 = x == 0 && z == 0 && y == 0;
+    C<R> n0 = a1[m], n1 = a2[m], n2 = a3[m];
+    if (!k0) {
+      const R kxr = (R)x, kyr = (R)ky, kzr = (R)kz;
+      const R rden = (R)1 / (kxr * kxr + kyr * kyr + kzr * kzr);
+      const C<R> psi = {-(kxr * n0.x + kyr * n1.x + kzr * n2.x) * rden,
+                        -(kxr * n0.y + kyr * n1.y + kzr * n2.y) * rden};
+      n0 = {n0.x + kxr * psi.x, n0.y + kxr * psi.y};
+      n1 = {n1.x + kyr * psi.x, n1.y + kyr * psi.y};
+      n2 = {n2.x + kzr * psi.x, n2.y + kzr * psi.y};
+    } else if constexpr (STAGE == 2) {
+      n0 = n1 = n2 = {0, 0};
+    }
+    if (!o) n0 = n1 = n2 = {0, 0};
+    const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
+    du0.store(ou, n0);
+    du1.store(ou, n1);
+    du2.store(ou, n2);
+    if constexpr (STAGE == 2) {
+      dn0.store(on2, n0);
+      dn1.store(on2, n1);
+      dn2.store(on2, n2);
     }
   }
 }
@@ -1315,10 +1398,12 @@ struct Launch {
     const CR* sr = (const CR*)c.sr[c.kstep];
     if (stage == 1)
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
-                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw);
+                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp,
+                         c.geo.colx, c.geo.colz);
     else
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
-                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw);
+                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp,
+                         c.geo.colx, c.geo.colz);
   }
 };
 
