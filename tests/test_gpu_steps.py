@@ -68,6 +68,62 @@ def test_fp32_10_steps_vs_fp64_oracle(O, P, n):
     s.close()
 
 
+def skewness(g, uy, cx, cz):
+    """mean longitudinal velocity-derivative skewness <(du_i/dx_i)^3> /
+    <(du_i/dx_i)^2>^(3/2) over i = x, y, z, from the spectral field (oracle
+    inverse transform in FP64)"""
+    kx, ky, kz = kgrid(g.n, cx, cz)
+    out = []
+    for i, k in enumerate((kx, ky, kz)):
+        d = np.zeros((1,) + uy.shape[1:], np.complex128)
+        d[0] = 1j * k * uy[i]
+        f = g.kxtran(d, 1)[0]
+        out.append(np.mean(f ** 3) / np.mean(f ** 2) ** 1.5)
+    return float(np.mean(out))
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_forced_256_10_steps_vs_fp64_oracle(O, P, prec):
+    """Forced isotropic turbulence at 256^3, viscosity from kmax eta = 1.5,
+    FEK forcing of shells k <= 2, 10 RK2 steps with phase shifts: GPU (FP32
+    or FP64) vs the FP64 oracle from the same seeded field.  Reports the
+    spectral relative L2 error, E(k), dissipation and the velocity-derivative
+    skewness."""
+    n = 256
+    banner(f"forced 256^3 fp{8 * prec} 10 steps")
+    g = O.Grid(n)
+    u0 = g.field(seed=122929298)
+    _, _, _, eps1 = g.spectrum(u0, 1.0)
+    viscos = float(np.sqrt(eps1) * (1.5 / (np.sqrt(2.0) / 3.0 * n)) ** 2)
+    ro = O.Run(g, u0, viscos, cfl=np.pi, kshift=1, kf_shell=3, shift_seed=11)
+    s = P.Solver(n, prec, viscos=viscos)
+    s.put_state(u0)
+    rg = P.Run(s, viscos, np.pi, kf_shell=3, ekinf=ro.ekinf)
+    m = g.mask()[None]
+    errs = []
+    for _ in range(10):
+        gsh = ro.next_shifts()
+        ro.step(gsh)
+        rg.step(gsh)
+        errs.append(rel(s.get_state(), ro.uny, m))
+    ek, dk, tke, eps = s.shell_stats(n)
+    rek, rdk, rtke, reps = g.spectrum(ro.uny, viscos)
+    sel = rek > 1e-10 * rek.max()
+    ekerr = float(np.max(np.abs(ek[sel] / rek[sel] - 1)))
+    cx, cz, _ = s.columns()
+    sk, rsk = skewness(g, s.get_state().astype(np.complex128), cx, cz), skewness(g, ro.uny, cx, cz)
+    print(f"fp{8 * prec}: rel L2 per step {['%.2e' % e for e in errs]}; E(k) max rel {ekerr:.2e}; "
+          f"eps {eps:.6e} vs {reps:.6e}; skewness {sk:.6f} vs {rsk:.6f}", flush=True)
+    tol = 1e-5 if prec == 4 else 1e-11
+    assert errs[-1] < tol
+    assert np.allclose(rg.dts, ro.dts, rtol=1e-6 if prec == 4 else 1e-12)
+    assert ekerr < (1e-4 if prec == 4 else 1e-10)
+    assert eps == pytest.approx(reps, rel=tol)
+    assert tke == pytest.approx(rtke, rel=tol)
+    assert abs(sk - rsk) < (1e-4 if prec == 4 else 1e-10)
+    s.close()
+
+
 def test_fek_forcing_on_gpu(O, P):
     banner("fek forcing 64^3 fp64")
     n = 64
