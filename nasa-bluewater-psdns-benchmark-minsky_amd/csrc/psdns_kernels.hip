@@ -1312,7 +1312,7 @@ static unsigned persistent_grid(const void* kernel, int threads) {
   // tuning: a larger grid than fits at once (work-groups take fewer tiles)
   constexpr int kScale = PSDNS_RESIDENT_SCALE;
 #else
-  constexpr int kScale = 8;
+  constexpr int kScale = 16;
 #endif
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
