@@ -139,6 +139,12 @@ struct Slot {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
+// Work-group barrier for LDS traffic only: this wave's LDS operations are
+// complete, then all waves meet; no compiler motion of memory operations
+// across it.  (__syncthreads also waits for outstanding global memory
+// operations, including copies into LDS still in flight.)
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // f(std::integral_constant<int, i>) for i = 0..N-1
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for_impl(F&& f) {
@@ -293,10 +299,14 @@ struct TeamFft {
   // wavefront-scope ordering (a wavefront's LDS operations are executed in
   // order); traffic across lines needs the work-group barrier.
   static constexpr bool kWaveLines = (T == 64);
+  // The work-group case orders LDS only (the transforms exchange data
+  // through LDS alone): it waits for this wave's LDS operations, not for
+  // its outstanding global loads and stores, so copies issued before a
+  // transform stay in flight through its barriers.
   template <bool CROSS>
   static __device__ __forceinline__ void sync() {
     if constexpr (CROSS || !kWaveLines) {
-      __syncthreads();
+      lds_barrier();
     } else {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();

@@ -373,9 +373,10 @@ struct XPass {
   static constexpr int kLdsReals = FI::kLdsReals;
   static constexpr int kTw = FI::kTwEntries;  // shared by FI and FF
 #ifndef PSDNS_XFRESH
-#define PSDNS_XFRESH false
+#define PSDNS_XFRESH true
 #endif
-  // the transforms of a tile share their address arithmetic
+  // forward transforms form their addresses per call (the fused pass loops
+  // over tiles)
   static constexpr bool XFRESH = PSDNS_XFRESH;
 
   // LDS tables of a work-group: stage twiddles (from tw of length N, every
@@ -438,13 +439,68 @@ struct XPass {
     c2r_finish(v, r, lds, stw, xtw);
   }
 
+  // Staged variant for the fused pass: one variable of a tile, rows kx =
+  // 0..M-1 of B consecutive z (64 bytes per row; rows kx >= max_al_x and
+  // past the last tile read as zeros), is copied by the memory system
+  // straight into an LDS staging buffer S[M][B] (16 bytes per lane, no
+  // registers), so the copy of the next variable runs under the transform
+  // of the current one.  Each wave copies whole 16-row slices.
+  static constexpr int kStageBytes = M * B * (int)sizeof(C<R>);
+  static __device__ __forceinline__ void stage_load(C<R>* S, const C<R>* __restrict__ xi, int var, int y, int z0,
+                                                    const Geo& g, bool valid) {
+    constexpr int kRowBytes = B * (int)sizeof(C<R>);  // 64
+    static_assert(kRowBytes == 64, "64-byte rows");
+    // a work-group smaller than a wavefront (small n) is one partial wave
+    constexpr int kLanes = NT < 64 ? NT : 64;
+    constexpr int kWaves = NT / kLanes;
+    constexpr int kLaneRows = kLanes * 16 / kRowBytes;  // rows per wave instruction
+    static_assert(NT % kLanes == 0 && kLanes % 4 == 0 && kWaves >= 1 && kLaneRows >= 1, "staging shape");
+    const BufView<R> src(xi + xiidx(var, y, 0, z0, g), valid ? kOob : 0u);
+    const int lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
+    for (int r0 = wave * kLaneRows; r0 < M; r0 += kWaves * kLaneRows) {
+      const int row = r0 + lane / 4;
+      const unsigned off = row < g.max_al_x
+                               ? (unsigned)row * (unsigned)(N * sizeof(C<R>)) + (unsigned)(lane % 4) * 16u
+                               : kOob;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          src.r, (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(S) + r0 * kRowBytes), 16, off,
+          0, 0, 0);
+    }
+  }
+  // c2r of the staged variable: c2r_staged_pre forms Z[k] from S[k] and
+  // conj S[m-k] (transposed mapping: 8 lanes read one 64-byte row) and ends
+  // with a barrier, after which S may be refilled; c2r_staged_fft then
+  // transforms
+  static __device__ __forceinline__ void c2r_staged_pre(C<R>* v, const C<R>* S, const C<R>* xtw) {
+    const int tid = opaque(threadIdx.x);  // per tile, not hoisted out of the tile loop
+    const int l = FI::transposed_line(tid), q = tid / B;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int k = q + T * j;
+      C<R> xk = S[k * B + l];
+      if (k == 0) xk.y = 0;
+      const C<R> xm = k == 0 ? C<R>{0, 0} : S[(M - k) * B + l];
+      const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
+      const C<R> d = {xk.x - xm.x, xk.y + xm.y};
+      C<R> w = xtw[k];
+      w.y = -w.y;
+      const C<R> wd = cmul(w, d);
+      v[j] = {sm.x - wd.y, sm.y + wd.x};
+    }
+    lds_barrier();
+  }
+  static __device__ __forceinline__ void c2r_staged_fft(C<R>* v, R* lds, const C<R>* stw) {
+    const int tid = threadIdx.x;
+    FI::template run<false, true, true, false>(v, lds, FI::transposed(lds, tid), FI::natural(lds, tid), stw);
+  }
+
   // r2c of packed physical values v of this thread's line, scaled by
   // `scale`: the last stage leaves Z in LDS; X[k] = (Z[k] + conj Z[m-k]) / 2
   // - i e^{-2 pi i k/n} (Z[k] - conj Z[m-k]) / 2 for kx < max_al_x (kx
   // Nyquist dropped).
   static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ xs, int var, int y, int z0,
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw, R scale) {
-    const int tid = threadIdx.x;
+    const int tid = XFRESH ? opaque(threadIdx.x) : (int)threadIdx.x;
     const Slot<R> s = FF::natural(lds, tid);
     FF::template run<true, XFRESH, false, false>(v, lds, s, s, stw);
     FF::template sync<false>();
@@ -553,44 +609,64 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
 // Fused x pass: c2r of u, v, w (from xi) -> realspace_vel products
 // (+ velmax, FP64, realspace_vel.f:108-110) -> r2c x 1/n of the five
 // products into X slots 0..4.  Physical space stays in registers.
+// Persistent over tiles (one y, B consecutive z): the staging copy of the
+// next variable is issued before each inverse transform, and that of the
+// next tile's u before the five forward transforms.
 template <typename R, int N>
 __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_x_fused(const C<R>* __restrict__ xi, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
               unsigned long long* __restrict__ velmax, int want_velmax) {
   using X = XPass<R, N>;
-  constexpr int E = X::E;
+  constexpr int E = X::E, B = X::B;
   __shared__ R lds[X::kLdsReals];
   __shared__ C<R> stw[X::kTw], xtw[X::M];
+  __shared__ __attribute__((aligned(16))) C<R> S[X::M * B];
   X::fill(stw, xtw, tw);
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const int y = bx / (N / X::B), z0 = (bx % (N / X::B)) * X::B;
-  C<R> pu[E], pv[E], pw[E];
-  typename X::Raw ra, rb;
-  X::c2r_load(ra, xi, 0, y, z0, g);
-  X::c2r_load(rb, xi, 1, y, z0, g);
-  issue_fence();
-  X::c2r_finish(pu, ra, lds, stw, xtw);
-  X::c2r_load(ra, xi, 2, y, z0, g);
-  issue_fence();
-  X::c2r_finish(pv, rb, lds, stw, xtw);
-  X::c2r_finish(pw, ra, lds, stw, xtw);
-  if (want_velmax) {
-    double vm = 0.0;
+  constexpr int NZB = N / B;
+  const TileWalk w = tile_walk(N * NZB);
+  int t = w.t;
+  double vm = 0.0;
+  if (t < w.end) {
+    X::stage_load(S, xi, 0, t / NZB, (t % NZB) * B, g, true);
+    for (;;) {
+      const int y = t / NZB, z0 = (t % NZB) * B;
+      C<R> pu[E], pv[E], pw[E];
+      // the staged variable must be in LDS (its copy's wait is placed by the
+      // compiler, which tracks the LDS writes of the copy) for every wave
+      __syncthreads();
+      X::c2r_staged_pre(pu, S, xtw);
+      X::stage_load(S, xi, 1, y, z0, g, true);
+      X::c2r_staged_fft(pu, lds, stw);
+      __syncthreads();
+      X::c2r_staged_pre(pv, S, xtw);
+      X::stage_load(S, xi, 2, y, z0, g, true);
+      X::c2r_staged_fft(pv, lds, stw);
+      __syncthreads();
+      X::c2r_staged_pre(pw, S, xtw);
+      const int tn = t + w.step;
+      const bool more = tn < w.end;
+      if (more) X::stage_load(S, xi, 0, tn / NZB, (tn % NZB) * B, g, true);
+      X::c2r_staged_fft(pw, lds, stw);
+      if (want_velmax) {
 #pragma unroll
-    for (int j = 0; j < E; j++) {
-      vm = fmax(vm, fabs((double)pu[j].x) + fabs((double)pv[j].x) + fabs((double)pw[j].x));
-      vm = fmax(vm, fabs((double)pu[j].y) + fabs((double)pv[j].y) + fabs((double)pw[j].y));
+        for (int j = 0; j < E; j++) {
+          vm = fmax(vm, fabs((double)pu[j].x) + fabs((double)pv[j].x) + fabs((double)pw[j].x));
+          vm = fmax(vm, fabs((double)pu[j].y) + fabs((double)pv[j].y) + fabs((double)pw[j].y));
+        }
+      }
+#pragma unroll
+      for (int sl = 0; sl < 5; sl++) {
+        C<R> q[E];
+#pragma unroll
+        for (int j = 0; j < E; j++)
+          q[j] = {product(sl, pu[j].x, pv[j].x, pw[j].x), product(sl, pu[j].y, pv[j].y, pw[j].y)};
+        X::r2c(q, xs, sl, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
+      }
+      if (!more) break;
+      t = tn;
     }
-    velmax_commit(vm * (double)N, velmax);
   }
-#pragma unroll
-  for (int s = 0; s < 5; s++) {
-    C<R> q[E];
-#pragma unroll
-    for (int j = 0; j < E; j++)
-      q[j] = {product(s, pu[j].x, pv[j].x, pw[j].x), product(s, pu[j].y, pv[j].y, pw[j].y)};
-    X::r2c(q, xs, s, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
-  }
+  if (want_velmax) velmax_commit(vm * (double)N, velmax);
 }
 
 // realspace_vel alone on physical ux (slots 0..4)
@@ -1368,8 +1444,9 @@ struct Launch {
   }
   static void x_fused(const Ctx& c, int want_velmax) {
     using X = XPass<R, N>;
-    dim3 grid(N * (N / X::B));
-    hipLaunchKernelGGL((k_x_fused<R, N>), grid, dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
+    const auto kern = k_x_fused<R, N>;
+    static const unsigned grid = persistent_grid((const void*)kern, X::NT);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
                        (const CR*)c.tw, c.velmax_bits, want_velmax);
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
