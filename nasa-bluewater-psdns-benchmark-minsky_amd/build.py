@@ -61,8 +61,19 @@ def build_library(verbose=False):
             f.result()
     tmp = LIB / "libpsdns_gpu.so.tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _check_defined(tmp)
     os.replace(tmp, LIB / "libpsdns_gpu.so")
     return LIB / "libpsdns_gpu.so"
+
+
+def _check_defined(so):
+    """A kernel whose host-side stub was not emitted leaves an undefined
+    psdns:: symbol that only fails when the library is loaded: refuse it."""
+    nm = shutil.which("nm") or "/usr/bin/nm"
+    r = subprocess.run([nm, "-D", "--undefined-only", str(so)], capture_output=True, text=True)
+    bad = [l.split()[-1] for l in r.stdout.splitlines() if "5psdns" in l]
+    if bad:
+        raise RuntimeError(f"{so}: undefined library symbols (kernel stubs not emitted): {bad[:3]}")
 
 
 def build_fortran(verbose=False):
