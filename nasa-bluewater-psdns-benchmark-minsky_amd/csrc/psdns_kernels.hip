@@ -695,6 +695,10 @@ __global__ void __launch_bounds__(256)
 #ifndef PSDNS_ZFWD_E
 #define PSDNS_ZFWD_E 8
 #endif
+#ifndef PSDNS_ZFRESH
+#define PSDNS_ZFRESH false
+#endif
+constexpr bool ZFRESH = PSDNS_ZFRESH;
 template <typename R, int N>
 struct ZFwd {
   static constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
@@ -755,7 +759,7 @@ __global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_
   const TileWalk w = tile_walk(N * nkb);
   int t = w.t;
   if (t >= w.end) return;
-  auto fft = [&](typename Z::Buf& b) { F::template run<false, false, true, false>(b.v, lds, in, out, stw); };
+  auto fft = [&](typename Z::Buf& b) { F::template run<false, ZFRESH, true, false>(b.v, lds, in, out, stw); };
   // o = i (kx a + kz b) in the natural mapping
   auto conv = [&](typename Z::Buf& o, const typename Z::Buf& a, const typename Z::Buf& b) {
     const R kxr = (R)((t % nkb) * B + F::natural_line(opaque(tid)));
@@ -1056,11 +1060,11 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
   __shared__ C<R> ssy[N];
-  __shared__ double syf[STAGE == 1 ? N : 1];
+  __shared__ R syf[STAGE == 1 ? N : 1];  // yfac in the state precision
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   for (int k = threadIdx.x; k < N; k += F::NT) {
     ssy[k] = sr[g.nxh + k];
-    if constexpr (STAGE == 1) syf[k] = yfac[k];
+    if constexpr (STAGE == 1) syf[k] = (R)yfac[k];
   }
   const int tid = threadIdx.x;
   const Slot<R> in = F::transposed(lds, tid);
@@ -1147,13 +1151,15 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   }
   // one velocity component: STAGE 1 un' = f (un + N) (stored on the mask),
   // nl <- f (un + 2 N) (proc4a.f:418-487); STAGE 2 nl <- un + N (proc4b.f:54-67)
-  double fxz = 0.0;
-  if constexpr (STAGE == 1) fxz = xfac[x] * zfac[z];
+  // integrating factor xfac yfac zfac as (xfac zfac) yfac in the state
+  // precision
+  R fxz = 0;
+  if constexpr (STAGE == 1) fxz = (R)(xfac[x] * zfac[z]);
   auto component = [&](C<R>* nl, const Buf& b, const BufView<R>& dn) {
 #pragma unroll
     for (int m = 0; m < E; m++) {
       if constexpr (STAGE == 1) {
-        const R f = (R)(fxz * syf[out.q + T * m]);
+        const R f = fxz * syf[out.q + T * m];
         const C<R> ut = b.v[m];
         const C<R> un2 = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
         nl[m] = {f * (ut.x + nl[m].x * (R)2), f * (ut.y + nl[m].y * (R)2)};
