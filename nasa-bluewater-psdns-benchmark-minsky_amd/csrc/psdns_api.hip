@@ -62,6 +62,11 @@ struct psdns_ctx {
   int2* d_force = nullptr;
   int force_kfor = -1, nforce = 0;
   int last_nonlinear = 0;
+  // stage 1's update is deferred to the next call: stage_nonlinear(2) runs
+  // it in one kernel with its own phase shift and inverse y pass (yw left in
+  // the u slots); every other entry point runs it on its own first
+  bool pending_update = false;
+  double pending_norm = 0.0;
   // pass timers
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -157,6 +162,42 @@ int pass(psdns_ctx* c, Pass p, int a0, int a1 = 0) {
 int mark(psdns_ctx* c, int i) {
   if (c->timing) HIPCHK(hipEventRecord(c->ev[i], c->c.stream));
   return 0;
+}
+
+// adds the time between events a and b to pass timer t (after a sync)
+int add_time(psdns_ctx* c, int a, int b, int t) {
+  if (!c->timing) return 0;
+  float ms;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
+  c->times[t] += ms;
+  return 0;
+}
+
+// the fused forward y pass + update of a stage (timer 4, events 5 and 6);
+// inv2 also does the inverse y pass of stage 2
+int y_update(psdns_ctx* c, int kstep, double norm, bool inv2) {
+  int rc;
+  c->c.kstep = kstep - 1;
+  if ((rc = mark(c, 5))) return rc;
+  if (psdns::launch_y_update(c->c, kstep, norm, inv2) != 0) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  HIPCHK(hipGetLastError());
+  return mark(c, 6);
+}
+
+// runs a deferred stage-1 update on its own
+int flush(psdns_ctx* c) {
+  if (!c->pending_update) return 0;
+  c->pending_update = false;
+  int rc;
+  if ((rc = y_update(c, 1, c->pending_norm, false))) return rc;
+  if ((rc = sync(c))) return rc;
+  return add_time(c, 5, 6, 4);
+}
+
+// a valid context with no deferred work left
+int ready(psdns_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  return flush(c);
 }
 
 }  // namespace
@@ -342,7 +383,7 @@ int psdns_columns(const psdns_ctx* c, int* col_x, int* col_z, int* cut_z) {
 }
 
 int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!gsh) return fail(PSDNS_EINVAL, "null gsh");
   const int n = c->n, nxh = c->nxh;
   std::vector<double2> h(2 * (nxh + 2 * n));
@@ -384,7 +425,7 @@ int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
 }
 
 int psdns_put_state(psdns_ctx* c, const void* uny, int layout) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!uny) return fail(PSDNS_EINVAL, "null state");
   if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
   const size_t bytes = c->csize() * 3 * c->cyl_elems();
@@ -395,7 +436,7 @@ int psdns_put_state(psdns_ctx* c, const void* uny, int layout) {
 }
 
 int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!uny) return fail(PSDNS_EINVAL, "null state");
   if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
   HIPCHK(hipMemcpyAsync(uny, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToHost, c->c.stream));
@@ -405,13 +446,22 @@ int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
 int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if (int rc = check_ctx(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
-  c->c.kstep = kstep - 1;
   int rc;
+  // stage 2 right after stage_update(1): that update and this stage's phase
+  // shift and inverse y pass are one kernel, whose yw the z pass reads from u
+  const bool fused = kstep == 2 && c->pending_update;
+  if (fused) {
+    c->pending_update = false;
+    if ((rc = y_update(c, 1, c->pending_norm, true))) return rc;
+  } else if ((rc = flush(c))) {
+    return rc;
+  }
+  c->c.kstep = kstep - 1;
   if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
-  if ((rc = pass(c, Pass::YInvPhase, 3))) return rc;
+  if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) return rc;
   if ((rc = mark(c, 1))) return rc;
-  if ((rc = pass(c, Pass::ZInv, 3))) return rc;
+  if ((rc = pass(c, Pass::ZInv, 3, fused ? 1 : 0))) return rc;
   if ((rc = mark(c, 2))) return rc;
   if ((rc = pass(c, Pass::XFused, kstep == 1))) return rc;
   if ((rc = mark(c, 3))) return rc;
@@ -421,20 +471,16 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
     if ((rc = read_velmax(c, velmax))) return rc;
   }
   if ((rc = sync(c))) return rc;
-  if (c->timing) {
-    float ms;
-    for (int i = 0; i < 4; i++) {
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
-      c->times[i] += ms;
-    }
-  }
+  for (int i = 0; i < 4; i++)
+    if ((rc = add_time(c, i, i + 1, i))) return rc;
+  if (fused && (rc = add_time(c, 5, 6, 4))) return rc;
   c->last_nonlinear = kstep;
   return PSDNS_OK;
 }
 
 int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, const double* yfac,
                        const double* zfac, const void* forcing) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
   if (forcing) return fail(PSDNS_EINVAL, "EPFOR forcing increments are not supported (pass NULL)");
   if (c->last_nonlinear != kstep)
@@ -444,25 +490,22 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
   if (kstep == 1 || xfac) {
     if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
   }
-  c->c.kstep = kstep - 1;
-  if ((rc = mark(c, 5))) return rc;
   const double dt2 = dt / 2.0;
   const double norm = dt2 / c->n / c->n;  // wavespace_vel.f:80
-  if (psdns::launch_y_update(c->c, kstep, norm) != 0) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
-  HIPCHK(hipGetLastError());
-  if ((rc = mark(c, 6))) return rc;
-  if ((rc = sync(c))) return rc;
-  if (c->timing) {
-    float ms;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
-    c->times[4] += ms;
-  }
   c->last_nonlinear = 0;
-  return PSDNS_OK;
+  if (kstep == 1) {
+    // deferred (the factors are on the device already)
+    c->pending_update = true;
+    c->pending_norm = norm;
+    return sync(c);
+  }
+  if ((rc = y_update(c, kstep, norm, false))) return rc;
+  if ((rc = sync(c))) return rc;
+  return add_time(c, 5, 6, 4);
 }
 
 int psdns_stage_times(psdns_ctx* c, int enable, double* times) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (times) {
     double tot = 0;
     for (int i = 0; i < 5; i++) {
@@ -477,7 +520,7 @@ int psdns_stage_times(psdns_ctx* c, int enable, double* times) {
 }
 
 int psdns_ek_shell(psdns_ctx* c, int kfor, double* ek) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (kfor < 1 || kfor > 64 || !ek) return fail(PSDNS_EINVAL, "kfor = %d: need 1..64 and an output array", kfor);
   int rc;
   if ((rc = ensure_force_list(c, kfor))) return rc;
@@ -489,7 +532,7 @@ int psdns_ek_shell(psdns_ctx* c, int kfor, double* ek) {
 }
 
 int psdns_ek_force(psdns_ctx* c, int kf_shell, int kfor, const double* ekinf, double* ek_out) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (kf_shell < 1 || kf_shell > kfor || !ekinf)
     return fail(PSDNS_EINVAL, "need 1 <= kf_shell (%d) <= kfor (%d) and ekinf", kf_shell, kfor);
   std::vector<double> ek(kfor), ratio(kfor, 0.0);
@@ -510,7 +553,7 @@ int psdns_ek_force(psdns_ctx* c, int kf_shell, int kfor, const double* ekinf, do
 }
 
 int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* tke, double* eps) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (nshell < 0 || nshell > c->n) return fail(PSDNS_EINVAL, "nshell = %d out of range", nshell);
   if (nshell > 0 && (!ek || !dk)) return fail(PSDNS_EINVAL, "null spectrum arrays");
   double* d_tot = c->d_red;
@@ -533,7 +576,7 @@ int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* 
 }
 
 int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double tke) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!(kpeak > 0) || !(tke > 0)) return fail(PSDNS_EINVAL, "kpeak and tke must be positive");
   psdns::launch_field(c->c, seed, kpeak, c->d_colz0);
   HIPCHK(hipGetLastError());
@@ -549,7 +592,7 @@ int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double
 }
 
 int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!host) return fail(PSDNS_EINVAL, "null host array");
   size_t bytes;
   void* dst;
@@ -571,7 +614,7 @@ int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
 }
 
 int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (!host) return fail(PSDNS_EINVAL, "null host array");
   size_t bytes;
   const void* src;
@@ -593,7 +636,7 @@ int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
 }
 
 int psdns_phase1(psdns_ctx* c, int kstep) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_phshift(c->c, 3);
@@ -601,7 +644,7 @@ int psdns_phase1(psdns_ctx* c, int kstep) {
 }
 
 int psdns_itransform_vel(psdns_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::YInvPlain, 3))) return rc;
@@ -611,7 +654,7 @@ int psdns_itransform_vel(psdns_ctx* c) {
 }
 
 int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
@@ -622,7 +665,7 @@ int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
 }
 
 int psdns_transform(psdns_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::XR2C, 5))) return rc;
@@ -632,7 +675,7 @@ int psdns_transform(psdns_ctx* c) {
 }
 
 int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_wavespace(c->c, dt2 / c->n / c->n);
@@ -640,7 +683,7 @@ int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
 }
 
 int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   int rc;
   if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
   psdns::launch_update(c->c, 1);
@@ -648,13 +691,13 @@ int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const dou
 }
 
 int psdns_proc4b(psdns_ctx* c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   psdns::launch_update(c->c, 2);
   return sync(c);
 }
 
 int psdns_kxtran(psdns_ctx* c, int nv) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;
@@ -665,7 +708,7 @@ int psdns_kxtran(psdns_ctx* c, int nv) {
 }
 
 int psdns_xktran(psdns_ctx* c, int nv) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = ready(c)) return rc;
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;

@@ -64,7 +64,7 @@ enum class Pass {
   YInvPhase,  // phase1 + y c2c +i, u -> yw (arg: number of variables)
   YInvKx,     // kxtran: zero y Nyquist + y c2c +i, u -> yw
   YInvPlain,  // y c2c +i, u -> yw
-  ZInv,       // gather yw + z c2c +i -> xi (arg: number of variables)
+  ZInv,       // gather yw + z c2c +i -> xi (args: number of variables, yw in u)
   XC2R,       // gather xi + x c2r -> physical (arg: number of variables)
   XR2C,       // physical -> x r2c / n (arg: number of variables)
   XFused,     // c2r u,v,w (xi) -> products (+velmax, arg) -> r2c / n of 5 (X)
@@ -74,8 +74,9 @@ enum class Pass {
 };
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1);
-// fused y-forward + wavespace_vel + proc4a (stage 1) / proc4b (stage 2)
-int launch_y_update(const Ctx& c, int stage, double norm);
+// fused y-forward + wavespace_vel + proc4a (stage 1) / proc4b (stage 2);
+// inv2 (stage 1): also stage 2's phase shift and inverse y pass, yw -> u
+int launch_y_update(const Ctx& c, int stage, double norm, bool inv2);
 void launch_phshift(const Ctx& c, int nv);
 void launch_realspace(const Ctx& c, int want_velmax);
 // fold the velmax slots into slot 0

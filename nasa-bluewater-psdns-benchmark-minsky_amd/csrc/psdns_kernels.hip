@@ -1046,14 +1046,21 @@ __global__ void __launch_bounds__(256)
 // registers (natural mapping), then the integrating-factor update and
 // projection with un (proc4a.f:418-571, proc4b.f:54-205).  Writes u slots
 // 0..2 (zero off the mask, as wavespace_vel.f:171-173) and un on the mask.
-// Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n].
+// With INV2 (STAGE 1 only) the kernel goes on with the next stage's first
+// pass instead of writing u: the stage-2 phase shift (phase1,
+// timestep_module.f:152-160, with the mask) and y c2c +i of each component
+// (kxcomm1_clean.f:118), stored as the y-transformed cylinder yw into the u
+// slots 0..2, where the inverse z pass of stage 2 reads it.
+// Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n] of this
+// stage, sr2 those of stage 2 (INV2).
 // ---------------------------------------------------------------------------
-template <typename R, int N, int STAGE>
+template <typename R, int N, int STAGE, bool INV2>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
-                   const C<R>* __restrict__ sr, const double* __restrict__ xfac, const double* __restrict__ yfac,
-                   const double* __restrict__ zfac, const C<R>* __restrict__ tw, const int2* __restrict__ grp,
-                   const int* __restrict__ colx, const int* __restrict__ colz) {
+                   const C<R>* __restrict__ sr, const C<R>* __restrict__ sr2, const double* __restrict__ xfac,
+                   const double* __restrict__ yfac, const double* __restrict__ zfac, const C<R>* __restrict__ tw,
+                   const int2* __restrict__ grp, const int* __restrict__ colx, const int* __restrict__ colz) {
+  static_assert(!INV2 || STAGE == 1, "the next stage's y pass follows stage 1 only");
   constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   using Buf = Regs<R, E>;
@@ -1061,10 +1068,12 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   __shared__ C<R> stw[F::kTwEntries];
   __shared__ C<R> ssy[N];
   __shared__ R syf[STAGE == 1 ? N : 1];  // yfac in the state precision
+  __shared__ C<R> ssy2[INV2 ? N : 1];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   for (int k = threadIdx.x; k < N; k += F::NT) {
     ssy[k] = sr[g.nxh + k];
     if constexpr (STAGE == 1) syf[k] = (R)yfac[k];
+    if constexpr (INV2) ssy2[k] = sr2[g.nxh + k];
   }
   const int tid = threadIdx.x;
   const Slot<R> in = F::transposed(lds, tid);
@@ -1086,8 +1095,10 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   // A1 = C1 + i ky P2, A2 = kx P2 + kz P4, A3 = C3 + i ky P4;
   // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3.  Each variable's loads are
   // issued before the transform of the previous one.
+  // lanes past the group take the group's first column (nothing of theirs
+  // is stored), so that their table reads stay inside sz and zfac
   const int ln = F::natural_line(tid);
-  const int z = z0 + ln, kz = kwave(z, N);
+  const int z = z0 + (ln < gr.y ? ln : 0), kz = kwave(z, N);
   C<R> a1[E], a2[E], a3[E];
   Buf X, Y;
   gather(X, 1);  // P2
@@ -1204,15 +1215,38 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
       n0 = n1 = n2 = {0, 0};
     }
     if (!o) n0 = n1 = n2 = {0, 0};
-    const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
-    du0.store(ou, n0);
-    du1.store(ou, n1);
-    du2.store(ou, n2);
-    if constexpr (STAGE == 2) {
-      dn0.store(on2, n0);
-      dn1.store(on2, n1);
-      dn2.store(on2, n2);
+    if constexpr (INV2) {
+      a1[m] = n0;
+      a2[m] = n1;
+      a3[m] = n2;
+    } else {
+      const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
+      du0.store(ou, n0);
+      du1.store(ou, n1);
+      du2.store(ou, n2);
+      if constexpr (STAGE == 2) {
+        dn0.store(on2, n0);
+        dn1.store(on2, n1);
+        dn2.store(on2, n2);
+      }
     }
+  }
+  if constexpr (INV2) {
+    // u* is zero off the mask, so the shift needs no mask of its own; the
+    // transforms run in the natural mapping in and out, on the lines the
+    // forward transforms left (each line read and written by its own threads)
+    using FI = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
+    const C<R> sxz2 = cmul(sr2[x], sr2[g.nxh + N + z]);
+    auto inverse = [&](C<R>* a, const BufView<R>& d) {
+#pragma unroll
+      for (int m = 0; m < E; m++) a[m] = cmul(a[m], cmul(ssy2[out.q + T * m], sxz2));
+      FI::template run<false, true, false, false>(a, lds, out, out, stw);
+#pragma unroll
+      for (int m = 0; m < E; m++) d.store(act ? ofs(m) : kOob, a[m]);
+    };
+    inverse(a1, du0);
+    inverse(a2, du1);
+    inverse(a3, du2);
   }
 }
 
@@ -1429,12 +1463,13 @@ struct Launch {
                          (const CR*)c.tw, nv, c.geo.colx, c.geo.colz);
     }
   }
-  static void z_inv(const Ctx& c, int nv) {
+  // yw from the W1 allocation (from_u = 0) or from the u slots 0..2
+  static void z_inv(const Ctx& c, int nv, int from_u) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
     const auto kern = k_z_inv<R, N>;
     static const unsigned grid = persistent_grid((const void*)kern, B * T);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xi, c.geo,
-                       (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)(from_u ? c.u : c.w1), (CR*)c.xi,
+                       c.geo, (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
   }
   static void x_c2r(const Ctx& c, int nv) {
     using X = XPass<R, N>;
@@ -1475,18 +1510,25 @@ struct Launch {
     hipLaunchKernelGGL((k_y_fwd<R, N>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u, c.geo,
                        (const CR*)c.tw);
   }
-  static void y_update(const Ctx& c, int stage, double norm) {
+  // stage 1 uses sr[0] (and sr[1] for the fused y pass of stage 2), stage 2
+  // sr[1]
+  static void y_update(const Ctx& c, int stage, double norm, bool inv2) {
     constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
     dim3 grid(c.geo.ngrp);
-    const CR* sr = (const CR*)c.sr[c.kstep];
-    if (stage == 1)
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
-                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp,
-                         c.geo.colx, c.geo.colz);
+    const CR* sr = (const CR*)c.sr[stage - 1];
+    const CR* sr2 = (const CR*)c.sr[1];
+    if (stage == 1 && inv2)
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, true>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
+                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
+                         c.geo.grp, c.geo.colx, c.geo.colz);
+    else if (stage == 1)
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
+                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
+                         c.geo.grp, c.geo.colx, c.geo.colz);
     else
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u,
-                         (CR*)c.un, c.geo, norm, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp,
-                         c.geo.colx, c.geo.colz);
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
+                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
+                         c.geo.grp, c.geo.colx, c.geo.colz);
   }
 };
 
@@ -1562,7 +1604,7 @@ static void run_pass(const Ctx& c, Pass p, int a0, int a1) {
     case Pass::YInvPhase: Launch<R, N>::y_inv(c, a0, 0); break;
     case Pass::YInvKx: Launch<R, N>::y_inv(c, a0, 1); break;
     case Pass::YInvPlain: Launch<R, N>::y_inv(c, a0, 2); break;
-    case Pass::ZInv: Launch<R, N>::z_inv(c, a0); break;
+    case Pass::ZInv: Launch<R, N>::z_inv(c, a0, a1); break;
     case Pass::XC2R: Launch<R, N>::x_c2r(c, a0); break;
     case Pass::XR2C: Launch<R, N>::x_r2c(c, a0); break;
     case Pass::XFused: Launch<R, N>::x_fused(c, a0); break;
@@ -1570,7 +1612,6 @@ static void run_pass(const Ctx& c, Pass p, int a0, int a1) {
     case Pass::ZFwdNorm: Launch<R, N>::z_fwd(c, 1, a0); break;
     case Pass::YFwd: Launch<R, N>::y_fwd(c, a0); break;
   }
-  (void)a1;
 }
 
 template <typename R>
@@ -1587,20 +1628,21 @@ static int run_pass_n(const Ctx& c, Pass p, int a0, int a1) {
 }
 
 template <typename R>
-static int y_update_n(const Ctx& c, int stage, double norm) {
+static int y_update_n(const Ctx& c, int stage, double norm, bool inv2) {
   switch (c.geo.n) {
-    case 32: Launch<R, 32>::y_update(c, stage, norm); return 0;
-    case 64: Launch<R, 64>::y_update(c, stage, norm); return 0;
-    case 128: Launch<R, 128>::y_update(c, stage, norm); return 0;
-    case 256: Launch<R, 256>::y_update(c, stage, norm); return 0;
-    case 512: Launch<R, 512>::y_update(c, stage, norm); return 0;
-    case 1024: Launch<R, 1024>::y_update(c, stage, norm); return 0;
+    case 32: Launch<R, 32>::y_update(c, stage, norm, inv2); return 0;
+    case 64: Launch<R, 64>::y_update(c, stage, norm, inv2); return 0;
+    case 128: Launch<R, 128>::y_update(c, stage, norm, inv2); return 0;
+    case 256: Launch<R, 256>::y_update(c, stage, norm, inv2); return 0;
+    case 512: Launch<R, 512>::y_update(c, stage, norm, inv2); return 0;
+    case 1024: Launch<R, 1024>::y_update(c, stage, norm, inv2); return 0;
   }
   return -1;
 }
 
-int launch_y_update(const Ctx& c, int stage, double norm) {
-  return c.prec == 8 ? y_update_n<double>(c, stage, norm) : y_update_n<float>(c, stage, norm);
+int launch_y_update(const Ctx& c, int stage, double norm, bool inv2) {
+  if (inv2 && stage != 1) return -1;
+  return c.prec == 8 ? y_update_n<double>(c, stage, norm, inv2) : y_update_n<float>(c, stage, norm, inv2);
 }
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1) {
