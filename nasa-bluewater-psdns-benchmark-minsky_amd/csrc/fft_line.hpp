@@ -223,16 +223,17 @@ struct TeamFft {
   // LDS position of output point k after run<true>
   static __device__ __forceinline__ int out_pos(int k) { return pos<last_span<1>()>(k); }
 
+  // (thread indices are non-negative: unsigned division is a shift)
   static __device__ __forceinline__ Slot<R> natural(R* lds, int tid) {
-    const int line = tid / T;
-    return {lds + line * W, lds + B * W + line * W, tid % T};
+    const int line = (int)((unsigned)tid / T);
+    return {lds + line * W, lds + B * W + line * W, (int)((unsigned)tid % T)};
   }
   static __device__ __forceinline__ Slot<R> transposed(R* lds, int tid) {
-    const int line = tid % B;
-    return {lds + line * W, lds + B * W + line * W, tid / B};
+    const int line = (int)((unsigned)tid % B);
+    return {lds + line * W, lds + B * W + line * W, (int)((unsigned)tid / B)};
   }
-  static __device__ __forceinline__ int natural_line(int tid) { return tid / T; }
-  static __device__ __forceinline__ int transposed_line(int tid) { return tid % B; }
+  static __device__ __forceinline__ int natural_line(int tid) { return (int)((unsigned)tid / T); }
+  static __device__ __forceinline__ int transposed_line(int tid) { return (int)((unsigned)tid % B); }
 
   // Twiddles come from a table in LDS that depends on (L, E) only, so that
   // the transforms of both signs share it: entries hold exp(-2 pi i m / L)
@@ -359,7 +360,7 @@ struct TeamFft {
         const int jm = j & (NS - 1);
         // base + r NS stays inside one padding period of this buffer, so
         // its position is pos(base) + r NS
-        const int pb = pos<NS>((j / NS) * NS * RAD + jm);
+        const int pb = pos<NS>((int)((unsigned)j / NS) * NS * RAD + jm);
 #pragma unroll
         for (int r = 0; r < RAD; r++) {
           s.re[pb + r * NS] = w[i * RAD + r].x;

@@ -7,8 +7,10 @@
 //     xi      xi[v][y][kx][z]   z-transformed (z pass output, kx < max_al_x):
 //             the z lines are contiguous
 //   forward transform:
-//     X       xs[v][y][z][kx]   x pass output (kx < max_al_x used, stride
-//             n/2): the x lines are contiguous
+//     X       xs[v][y][kx/K][z][kx%K]  x pass output (kx < max_al_x used, n/2
+//             allocated), K = one 64-byte run of kx: a forward z tile of K kx
+//             reads one contiguous block, an x tile of K z writes K x 64
+//             contiguous bytes per block
 //     W1      w1[v][kx][y][z]   z pass output (cylinder z only): the z lines
 //             are contiguous, a cylinder column's y-run stays in one kx slab
 //   physical  ux[v][y][z][x]    reference ux(nx, zisz, yjsz, nv) (only for the
@@ -18,7 +20,8 @@
 // replaced by these layouts so that every pass writes whole lines and the
 // transposes happen on the read side, in 64-byte runs inside a few pages
 // (scattered 64-byte writes reach about two thirds of the bandwidth of
-// scattered 64-byte reads on this memory system).  The cylinder cut is never
+// scattered 64-byte reads on this memory system); X is blocked instead, as
+// the x tile's lines fill whole blocks together.  The cylinder cut is never
 // stored: the z passes read it as zeros and skip it on output.
 #include <hip/hip_runtime.h>
 
@@ -90,12 +93,17 @@ __device__ __forceinline__ int2 xcd_block2() {
   return make_int2(l % gridDim.x, l / gridDim.x);
 }
 
-// W1 index of (v, kx, y, z) and X index of (v, y, z, kx)
+// W1 index of (v, kx, y, z)
 __device__ __forceinline__ size_t w1idx(int v, int kx, int y, int z, const Geo& g) {
   return (((size_t)v * g.max_al_x + kx) * g.n + y) * g.n + z;
 }
+// X index of (v, y, z, kx) in blocks of K kx (one 64-byte run):
+// X[v][y][kx / K][z][kx % K].  The forward x pass writes, per tile of K
+// consecutive z, K x 64 contiguous bytes per block; the forward z pass reads
+// a tile of K consecutive kx as one contiguous N x 64-byte block.
+template <int K>
 __device__ __forceinline__ size_t xidx(int v, int y, int z, int kx, const Geo& g) {
-  return (((size_t)v * g.n + y) * g.n + z) * g.nxh + kx;
+  return ((((size_t)v * g.n + y) * (g.nxh / K) + kx / K) * g.n + z) * K + kx % K;
 }
 // xi index of (v, y, kx, z): inverse z pass output, x pass input
 __device__ __forceinline__ size_t xiidx(int v, int y, int kx, int z, const Geo& g) {
@@ -355,8 +363,9 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
 // x pass building blocks (kxcomm2.f:278-333 inverse, xkcomm1.f:97,135
 // forward): a tile is one y and B consecutive z.  The inverse transform
 // gathers its x lines from xi[v][y][kx][z] in 64-byte runs along z
-// (transposed mapping); the forward transform writes contiguous x lines of
-// X[v][y][z][kx] (natural mapping).  The real transform of length n runs as
+// (transposed mapping); the forward transform writes its x lines into the
+// kx blocks of X (natural mapping: each wave writes 64-byte runs, and the
+// tile's B lines together fill B x 64 contiguous bytes per block).  The real transform of length n runs as
 // a complex transform of length m = n/2 on packed pairs (x[2i], x[2i+1]).
 // ---------------------------------------------------------------------------
 #ifndef PSDNS_XPASS_E
@@ -403,7 +412,7 @@ struct XPass {
   static __device__ __forceinline__ void c2r_load(Raw& r, const C<R>* __restrict__ xi, int var, int y, int z0,
                                                   const Geo& g) {
     const int tid = threadIdx.x;
-    const int l = FI::transposed_line(tid), q = tid / B;
+    const int l = FI::transposed_line(tid), q = (int)((unsigned)tid / B);
     const BufView<R> src(xi + xiidx(var, y, 0, z0, g), kOob);
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -419,9 +428,9 @@ struct XPass {
     const Slot<R> out = FI::natural(lds, tid);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = in.q + T * j;
+      const int k = in.q + T * j;  // k > 0 for j > 0
       C<R> xk = r.k[j];
-      if (k == 0) xk.y = 0;
+      if (j == 0 && k == 0) xk.y = 0;
       const C<R> xm = r.m[j];
       const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
@@ -457,7 +466,12 @@ struct XPass {
     static_assert(NT % kLanes == 0 && kLanes % 4 == 0 && kWaves >= 1 && kLaneRows >= 1, "staging shape");
     const BufView<R> src(xi + xiidx(var, y, 0, z0, g), valid ? kOob : 0u);
     const int lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
-    for (int r0 = wave * kLaneRows; r0 < M; r0 += kWaves * kLaneRows) {
+    __builtin_assume(wave < kWaves);
+    constexpr int kStep = kWaves * kLaneRows, kIters = (M + kStep - 1) / kStep;
+#pragma unroll
+    for (int i = 0; i < kIters; i++) {
+      const int r0 = wave * kLaneRows + i * kStep;
+      if (M % kStep != 0 && r0 >= M) break;
       const int row = r0 + lane / 4;
       const unsigned off = row < g.max_al_x
                                ? (unsigned)row * (unsigned)(N * sizeof(C<R>)) + (unsigned)(lane % 4) * 16u
@@ -473,13 +487,16 @@ struct XPass {
   // transforms
   static __device__ __forceinline__ void c2r_staged_pre(C<R>* v, const C<R>* S, const C<R>* xtw) {
     const int tid = opaque(threadIdx.x);  // per tile, not hoisted out of the tile loop
-    const int l = FI::transposed_line(tid), q = tid / B;
+    const int l = FI::transposed_line(tid), q = (int)((unsigned)tid / B);
 #pragma unroll
     for (int j = 0; j < E; j++) {
+      // k > 0 for j > 0; for k = 0 the row read (k itself) is discarded: X[m]
+      // (kx Nyquist) is zero
       const int k = q + T * j;
       C<R> xk = S[k * B + l];
-      if (k == 0) xk.y = 0;
-      const C<R> xm = k == 0 ? C<R>{0, 0} : S[(M - k) * B + l];
+      const C<R> xr = S[(j == 0 ? (M - k) & (M - 1) : M - k) * B + l];
+      if (j == 0 && k == 0) xk.y = 0;
+      const C<R> xm = j == 0 && k == 0 ? C<R>{0, 0} : xr;
       const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
       const C<R> d = {xk.x - xm.x, xk.y + xm.y};
       C<R> w = xtw[k];
@@ -505,20 +522,21 @@ struct XPass {
     FF::template run<true, XFRESH, false, false>(v, lds, s, s, stw);
     FF::template sync<false>();
     // no branch per point: kx >= max_al_x lanes store nothing (out of range)
-    const BufView<R> dst(xs + xidx(var, y, z0, 0, g), kOob);
+    const BufView<R> dst(xs + xidx<B>(var, y, z0, 0, g), kOob);
     const int q = s.q;
-    const unsigned lofs = (unsigned)(FF::natural_line(tid) * g.nxh);
+    const unsigned lofs = (unsigned)FF::natural_line(tid) * B;
     const R half = (R)0.5 * scale;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-      const int k = q + T * j;
-      const int pk = FF::out_pos(k), pm = FF::out_pos(k == 0 ? 0 : M - k);
+      const int k = q + T * j;  // k > 0 for j > 0
+      const int pk = FF::out_pos(k), pm = FF::out_pos(j == 0 ? (M - k) & (M - 1) : M - k);
       const C<R> zk = {s.re[pk], s.im[pk]};
       const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
       const C<R> sm = zk + zm;
       const C<R> d = zk - zm;
       const C<R> wd = cmul(xtw[k], d);
-      dst.store(k < g.max_al_x ? (lofs + k) * (unsigned)sizeof(C<R>) : kOob,
+      const unsigned ku = (unsigned)k;
+      dst.store(k < g.max_al_x ? (lofs + ku / B * (unsigned)(N * B) + ku % B) * (unsigned)sizeof(C<R>) : kOob,
                 C<R>{(sm.x + wd.y) * half, (sm.y - wd.x) * half});
     }
   }
@@ -688,9 +706,9 @@ __global__ void __launch_bounds__(256)
 // Forward z pass (transform.f:129-138) and the convective combination
 // (transform.f:169-176,229-234): C1 = i kx P1 + i kz P5, C3 = i kx P5 + i kz P3
 // (z Nyquist skipped, it is outside the cylinder).  X -> W1: a tile is one y
-// and B consecutive kx; the first stage gathers X in 64-byte runs along kx
-// (transposed mapping), the last stage leaves z lines in the natural mapping
-// for contiguous stores of the cylinder z (the cut is skipped).
+// and B consecutive kx, one contiguous kx block of X, read in the transposed
+// mapping; the last stage leaves z lines in the natural mapping for
+// contiguous stores of the cylinder z (the cut is skipped).
 // ---------------------------------------------------------------------------
 #ifndef PSDNS_ZFWD_E
 #define PSDNS_ZFWD_E 8
@@ -710,12 +728,12 @@ struct ZFwd {
   static __device__ __forceinline__ void load(Buf& b, const C<R>* __restrict__ xs, const Geo& g, int nkb, int t,
                                               int var, bool valid, int lt0, int q0) {
     const int y = t / nkb, kx0 = (t % nkb) * B;
-    const BufView<R> src(xs + xidx(var, y, 0, kx0, g), valid ? kOob : 0u);
+    const BufView<R> src(xs + xidx<B>(var, y, 0, kx0, g), valid ? kOob : 0u);
     const int l = opaque(lt0), q = opaque(q0);
     const bool act = kx0 + l < g.max_al_x;
 #pragma unroll
     for (int j = 0; j < E; j++)
-      b.v[j] = src.load(act ? ((unsigned)(q + T * j) * g.nxh + l) * (unsigned)sizeof(C<R>) : kOob);
+      b.v[j] = src.load(act ? ((unsigned)(q + T * j) * B + l) * (unsigned)sizeof(C<R>) : kOob);
   }
   // stores of the cylinder z of a transformed variable (natural mapping)
   static __device__ __forceinline__ void store(const Buf& b, C<R>* __restrict__ w1, const Geo& g,
