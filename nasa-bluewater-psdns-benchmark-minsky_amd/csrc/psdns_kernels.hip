@@ -4,6 +4,10 @@
 //   cylinder  u[v][a][y]        reference uy(ny, num_al, nv)  (state, y fastest)
 //   inverse transform:
 //     yw      w1 as [v][a][y]   y-transformed cylinder (y pass output)
+//             or, from the stage-1 update, u as [v][y/K][a][y%K] (K = one
+//             64-byte run of y: an inverse z tile of K y reads one
+//             contiguous run per kx; the update's tile of K columns writes
+//             K x 64 contiguous bytes per block)
 //     xi      xi[v][y][kx][z]   z-transformed (z pass output, kx < max_al_x):
 //             the z lines are contiguous
 //   forward transform:
@@ -93,6 +97,14 @@ __device__ __forceinline__ int2 xcd_block2() {
   return make_int2(l % gridDim.x, l / gridDim.x);
 }
 
+// yw index of (v, a, y) in blocks of K y (one 64-byte run):
+// yw[v][y / K][a][y % K].  The inverse z pass reads a tile of K consecutive
+// y as one contiguous run over the kx's columns; the y passes write, per
+// tile of consecutive columns, one 64-byte run per column and block.
+template <int K>
+__device__ __forceinline__ size_t ywidx(int v, int a, int y, const Geo& g) {
+  return (((size_t)v * (g.n / K) + y / K) * g.na + a) * K + y % K;
+}
 // W1 index of (v, kx, y, z)
 __device__ __forceinline__ size_t w1idx(int v, int kx, int y, int z, const Geo& g) {
   return (((size_t)v * g.max_al_x + kx) * g.n + y) * g.n + z;
@@ -310,12 +322,13 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 // Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): y-transformed
 // cylinder yw -> xi[v][y][kx][z].  A tile is one variable, one kx
 // (< max_al_x) and B consecutive y.  Its z lines are gathered from the kx's
-// cylinder columns in 64-byte runs along y (transposed mapping), the cut
+// cylinder columns in 64-byte runs along y, or read as one contiguous run
+// from the y-blocked yw (transposed mapping), the cut
 // (which always holds the z Nyquist) read as zero (kxcomm1_clean.f:238-240);
 // z c2c +i; the last stage leaves the lines in the natural mapping for
 // contiguous stores along z.
 // ---------------------------------------------------------------------------
-template <typename R, int N>
+template <typename R, int N, bool YB>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((amdgpu_waves_per_eu(4)))
     k_z_inv(const C<R>* __restrict__ yw, C<R>* __restrict__ xi, Geo g, const C<R>* __restrict__ tw, int nv,
             const int* __restrict__ cutz, const int* __restrict__ col0) {
@@ -335,14 +348,17 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
     const int vk = t / NYB, y0 = (t % NYB) * B;
     const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
     const int c = cutz[kx];
-    const BufView<R> src(yw + ((size_t)var * g.na + col0[kx]) * N + y0, valid ? kOob : 0u);
+    // YB: the kx's columns in y block y0 / B, one contiguous run; else
+    // 64-byte runs along y of the kx's columns
+    const BufView<R> src(yw + (YB ? ywidx<B>(var, col0[kx], y0, g) : ((size_t)var * g.na + col0[kx]) * N + y0),
+                         valid ? kOob : 0u);
     const int l = opaque(lt0), q = opaque(in.q);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const int zz = q + T * j;
       const bool lo = zz < N / 2 - c, hi = zz > N / 2 + c;
       const int ai = lo ? zz : zz - 2 * c - 1;  // column of z within kx
-      b.v[j] = src.load(lo || hi ? ((unsigned)ai * N + l) * (unsigned)sizeof(C<R>) : kOob);
+      b.v[j] = src.load(lo || hi ? ((unsigned)ai * (YB ? B : N) + l) * (unsigned)sizeof(C<R>) : kOob);
     }
   };
   auto finish = [&](Regs<R, E>& b, int t) {
@@ -1254,17 +1270,23 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     // transforms run in the natural mapping in and out, on the lines the
     // forward transforms left (each line read and written by its own threads)
     using FI = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
+    static_assert(T % B == 0, "y blocks of a store");
     const C<R> sxz2 = cmul(sr2[x], sr2[g.nxh + N + z]);
-    auto inverse = [&](C<R>* a, const BufView<R>& d) {
+    // y-blocked yw in the u slots (blocks of B y, as the inverse z pass reads)
+    const unsigned lo = (((unsigned)out.q / B * g.na + ln) * B + (unsigned)out.q % B) * (unsigned)sizeof(C<R>);
+    auto inverse = [&](C<R>* a, int v) {
 #pragma unroll
       for (int m = 0; m < E; m++) a[m] = cmul(a[m], cmul(ssy2[out.q + T * m], sxz2));
       FI::template run<false, true, false, false>(a, lds, out, out, stw);
 #pragma unroll
-      for (int m = 0; m < E; m++) d.store(act ? ofs(m) : kOob, a[m]);
+      for (int m = 0; m < E; m++) {
+        const BufView<R> d(u + ywidx<B>(v, gr.x, T * m, g), kOob);
+        d.store(act ? lo : kOob, a[m]);
+      }
     };
-    inverse(a1, du0);
-    inverse(a2, du1);
-    inverse(a3, du2);
+    inverse(a1, 0);
+    inverse(a2, 1);
+    inverse(a3, 2);
   }
 }
 
@@ -1481,13 +1503,21 @@ struct Launch {
                          (const CR*)c.tw, nv, c.geo.colx, c.geo.colz);
     }
   }
-  // yw from the W1 allocation (from_u = 0) or from the u slots 0..2
+  // yw from the W1 allocation (from_u = 0, column-major cylinder) or
+  // y-blocked from the u slots 0..2 (the fused stage-1 update's output)
   static void z_inv(const Ctx& c, int nv, int from_u) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
-    const auto kern = k_z_inv<R, N>;
-    static const unsigned grid = persistent_grid((const void*)kern, B * T);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)(from_u ? c.u : c.w1), (CR*)c.xi,
-                       c.geo, (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
+    if (from_u) {
+      const auto kern = k_z_inv<R, N, true>;
+      static const unsigned grid = persistent_grid((const void*)kern, B * T);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.xi, c.geo,
+                         (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
+    } else {
+      const auto kern = k_z_inv<R, N, false>;
+      static const unsigned grid = persistent_grid((const void*)kern, B * T);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xi, c.geo,
+                         (const CR*)c.tw, nv, c.geo.cutz, c.geo.col0);
+    }
   }
   static void x_c2r(const Ctx& c, int nv) {
     using X = XPass<R, N>;
