@@ -40,6 +40,12 @@ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
 }
 template <typename R>
 __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return {a.x * s, a.y * s}; }
+// a * b + c, one rounding
+template <typename R>
+__device__ __forceinline__ R fmar(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
 template <typename R>
 __device__ __forceinline__ cpx<R> conjg(cpx<R> a) { return {a.x, -a.y}; }
 // multiply by +i (s = +1) or -i (s = -1)

@@ -422,6 +422,14 @@ struct XPass {
   // physical values (x[2i], x[2i+1]) of this thread's line in the natural
   // mapping.  Splitting the two lets a variable's loads be in flight during
   // the transform of the previous one.
+  // Z[k] = (X[k] + conj X[m-k]) + i conj(t) (X[k] - conj X[m-k]) with
+  // t = e^{-2 pi i k/n}, the products fused into the sums
+  static __device__ __forceinline__ C<R> c2r_point(C<R> xk, C<R> xm, C<R> t) {
+    const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
+    const C<R> d = {xk.x - xm.x, xk.y + xm.y};
+    return {fmar(t.y, d.x, fmar(-t.x, d.y, sm.x)),
+            fmar(t.y, d.y, fmar(t.x, d.x, sm.y))};
+  }
   struct Raw {
     C<R> k[E], m[E];
   };
@@ -448,12 +456,7 @@ struct XPass {
       C<R> xk = r.k[j];
       if (j == 0 && k == 0) xk.y = 0;
       const C<R> xm = r.m[j];
-      const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
-      const C<R> d = {xk.x - xm.x, xk.y + xm.y};
-      C<R> w = xtw[k];
-      w.y = -w.y;
-      const C<R> wd = cmul(w, d);
-      v[j] = {sm.x - wd.y, sm.y + wd.x};
+      v[j] = c2r_point(xk, xm, xtw[k]);
     }
     FI::template run<false, true, true, false>(v, lds, in, out, stw);
   }
@@ -513,12 +516,7 @@ struct XPass {
       const C<R> xr = S[(j == 0 ? (M - k) & (M - 1) : M - k) * B + l];
       if (j == 0 && k == 0) xk.y = 0;
       const C<R> xm = j == 0 && k == 0 ? C<R>{0, 0} : xr;
-      const C<R> sm = {xk.x + xm.x, xk.y - xm.y};
-      const C<R> d = {xk.x - xm.x, xk.y + xm.y};
-      C<R> w = xtw[k];
-      w.y = -w.y;
-      const C<R> wd = cmul(w, d);
-      v[j] = {sm.x - wd.y, sm.y + wd.x};
+      v[j] = c2r_point(xk, xm, xtw[k]);
     }
     lds_barrier();
   }
@@ -528,9 +526,11 @@ struct XPass {
   }
 
   // r2c of packed physical values v of this thread's line, scaled by
-  // `scale`: the last stage leaves Z in LDS; X[k] = (Z[k] + conj Z[m-k]) / 2
-  // - i e^{-2 pi i k/n} (Z[k] - conj Z[m-k]) / 2 for kx < max_al_x (kx
-  // Nyquist dropped).
+  // `scale` / 2: the last stage leaves Z in LDS; 2 X[k] = (Z[k] + conj Z[m-k])
+  // - i e^{-2 pi i k/n} (Z[k] - conj Z[m-k]) for kx < max_al_x (kx Nyquist
+  // dropped).  Without SCALE the values stored are 2 X[k] unnormalized, n
+  // times the transform / n (x 2n, exact in binary).
+  template <bool SCALE>
   static __device__ __forceinline__ void r2c(C<R>* v, C<R>* __restrict__ xs, int var, int y, int z0,
                                              const Geo& g, R* lds, const C<R>* stw, const C<R>* xtw, R scale) {
     const int tid = XFRESH ? opaque(threadIdx.x) : (int)threadIdx.x;
@@ -550,10 +550,13 @@ struct XPass {
       const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
       const C<R> sm = zk + zm;
       const C<R> d = zk - zm;
-      const C<R> wd = cmul(xtw[k], d);
+      const C<R> t = xtw[k];
+      // sm - i t d, the products fused into the sums
+      C<R> o = {fmar(t.y, d.x, fmar(t.x, d.y, sm.x)),
+                fmar(t.y, d.y, fmar(-t.x, d.x, sm.y))};
+      if constexpr (SCALE) o = cscale(o, half);
       const unsigned ku = (unsigned)k;
-      dst.store(k < g.max_al_x ? (lofs + ku / B * (unsigned)(N * B) + ku % B) * (unsigned)sizeof(C<R>) : kOob,
-                C<R>{(sm.x + wd.y) * half, (sm.y - wd.x) * half});
+      dst.store(k < g.max_al_x ? (lofs + ku / B * (unsigned)(N * B) + ku % B) * (unsigned)sizeof(C<R>) : kOob, o);
     }
   }
 };
@@ -593,7 +596,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT))
   C<R> v[X::E];
 #pragma unroll
   for (int j = 0; j < X::E; j++) v[j] = src[q + X::T * j];
-  X::r2c(v, xs, var, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
+  X::template r2c<true>(v, xs, var, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
 }
 
 // velmax (realspace_vel.f:108-110) is a maximum over the whole grid: each
@@ -694,7 +697,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
 #pragma unroll
         for (int j = 0; j < E; j++)
           q[j] = {product(sl, pu[j].x, pv[j].x, pw[j].x), product(sl, pu[j].y, pv[j].y, pw[j].y)};
-        X::r2c(q, xs, sl, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
+        X::template r2c<false>(q, xs, sl, y, z0, g, lds, stw, xtw, (R)1);
       }
       if (!more) break;
       t = tn;
@@ -1563,6 +1566,8 @@ struct Launch {
   static void y_update(const Ctx& c, int stage, double norm, bool inv2) {
     constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
     dim3 grid(c.geo.ngrp);
+    // W1 from the fused x pass carries its unnormalized r2c: x 2n
+    norm /= 2.0 * N;
     const CR* sr = (const CR*)c.sr[stage - 1];
     const CR* sr2 = (const CR*)c.sr[1];
     if (stage == 1 && inv2)
