@@ -387,11 +387,15 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
 #ifndef PSDNS_XPASS_E
 #define PSDNS_XPASS_E 8
 #endif
+#ifndef PSDNS_XPASS_RUNS
+#define PSDNS_XPASS_RUNS 1
+#endif
 template <typename R, int N>
 struct XPass {
   static constexpr int M = N / 2;
   static constexpr int T = Team<M, PSDNS_XPASS_E>::T, E = Team<M, PSDNS_XPASS_E>::E;
-  static constexpr int B = RunLines<R>::B;  // z lines per tile
+  static constexpr int K = RunLines<R>::B;          // one 64-byte run of z (or of kx in X)
+  static constexpr int B = K * PSDNS_XPASS_RUNS;    // z lines per tile
   static constexpr int NT = B * T;
   using FI = TeamFft<R, M, E, B, +1, LineShift<E, B>::IN>;
   using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN>;
@@ -476,13 +480,14 @@ struct XPass {
   static constexpr int kStageBytes = M * B * (int)sizeof(C<R>);
   static __device__ __forceinline__ void stage_load(C<R>* S, const C<R>* __restrict__ xi, int var, int y, int z0,
                                                     const Geo& g, bool valid) {
-    constexpr int kRowBytes = B * (int)sizeof(C<R>);  // 64
-    static_assert(kRowBytes == 64, "64-byte rows");
+    constexpr int kRowBytes = B * (int)sizeof(C<R>);  // whole 64-byte runs
+    constexpr int kRowLanes = kRowBytes / 16;
+    static_assert(kRowBytes % 64 == 0, "64-byte runs");
     // a work-group smaller than a wavefront (small n) is one partial wave
     constexpr int kLanes = NT < 64 ? NT : 64;
     constexpr int kWaves = NT / kLanes;
-    constexpr int kLaneRows = kLanes * 16 / kRowBytes;  // rows per wave instruction
-    static_assert(NT % kLanes == 0 && kLanes % 4 == 0 && kWaves >= 1 && kLaneRows >= 1, "staging shape");
+    constexpr int kLaneRows = kLanes / kRowLanes;  // rows per wave instruction
+    static_assert(NT % kLanes == 0 && kLanes % kRowLanes == 0 && kWaves >= 1 && kLaneRows >= 1, "staging shape");
     const BufView<R> src(xi + xiidx(var, y, 0, z0, g), valid ? kOob : 0u);
     const int lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
     __builtin_assume(wave < kWaves);
@@ -491,9 +496,9 @@ struct XPass {
     for (int i = 0; i < kIters; i++) {
       const int r0 = wave * kLaneRows + i * kStep;
       if (M % kStep != 0 && r0 >= M) break;
-      const int row = r0 + lane / 4;
+      const int row = r0 + lane / kRowLanes;
       const unsigned off = row < g.max_al_x
-                               ? (unsigned)row * (unsigned)(N * sizeof(C<R>)) + (unsigned)(lane % 4) * 16u
+                               ? (unsigned)row * (unsigned)(N * sizeof(C<R>)) + (unsigned)(lane % kRowLanes) * 16u
                                : kOob;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           src.r, (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(S) + r0 * kRowBytes), 16, off,
@@ -538,9 +543,9 @@ struct XPass {
     FF::template run<true, XFRESH, false, false>(v, lds, s, s, stw);
     FF::template sync<false>();
     // no branch per point: kx >= max_al_x lanes store nothing (out of range)
-    const BufView<R> dst(xs + xidx<B>(var, y, z0, 0, g), kOob);
+    const BufView<R> dst(xs + xidx<K>(var, y, z0, 0, g), kOob);
     const int q = s.q;
-    const unsigned lofs = (unsigned)FF::natural_line(tid) * B;
+    const unsigned lofs = (unsigned)FF::natural_line(tid) * K;
     const R half = (R)0.5 * scale;
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -556,7 +561,7 @@ struct XPass {
                 fmar(t.y, d.y, fmar(-t.x, d.x, sm.y))};
       if constexpr (SCALE) o = cscale(o, half);
       const unsigned ku = (unsigned)k;
-      dst.store(k < g.max_al_x ? (lofs + ku / B * (unsigned)(N * B) + ku % B) * (unsigned)sizeof(C<R>) : kOob, o);
+      dst.store(k < g.max_al_x ? (lofs + ku / K * (unsigned)(N * K) + ku % K) * (unsigned)sizeof(C<R>) : kOob, o);
     }
   }
 };
