@@ -81,7 +81,12 @@ int psdns_set_shifts(psdns_ctx *c, const double gsh[6]);
  * (kstep 1) or proc4b (kstep 2) (timestep_module.f:244-317).  xfac(n/2),
  * yfac(n), zfac(n) are the integrating factors of step.f:366-385; forcing
  * must be NULL (EPFOR forcing increments are not supported; FEK forcing is
- * psdns_ek_force). */
+ * psdns_ek_force).
+ * stage_update(1) only records its arguments: the update itself runs in
+ * the next call.  When that call is stage_nonlinear(2), the update and stage
+ * 2's phase shift and inverse y transform are one kernel; any other call
+ * runs the update on its own first, so every result the caller can observe
+ * is the same either way. */
 int psdns_stage_nonlinear(psdns_ctx *c, int kstep, double *velmax);
 int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
                        const double *yfac, const double *zfac, const void *forcing);
@@ -126,7 +131,9 @@ int psdns_xktran(psdns_ctx *c, int nv);                    /* xktran.f:1-88: ux(
 
 /* Device time of the last stage calls, per pass (ms), the reference's
  * t_itrans/t_trans split (timestep_module.f:98-437); times[0..5] =
- * y-inverse, z-inverse, x-fused, z-forward, y-forward+update, total. */
+ * y-inverse, z-inverse, x-fused, z-forward, y-forward+update, total.  A
+ * stage 2 that follows stage_update(1) directly has no y-inverse of its own:
+ * that transform is timed with the y-forward+update of stage 1. */
 int psdns_stage_times(psdns_ctx *c, int enable, double *times);
 
 const char *psdns_last_error(void);
