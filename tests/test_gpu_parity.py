@@ -212,6 +212,33 @@ def test_fused_stages(O, P, ctxs, n, prec, peek):
     assert rel(s.get_field(P.FIELD_U, 3), uy2[:3], m) < TOL[prec]
 
 
+@pytest.mark.parametrize("n,prec", [(64, 8), (64, 4)])
+def test_deferred_update_keeps_its_shifts(O, P, ctxs, n, prec):
+    """stage_update(1) runs in the next call; new shifts set in between apply
+    to stage 2 only (the deferred update still uses the stage-1 factors it
+    was issued with)."""
+    banner("deferred update + set_shifts", n, prec)
+    g, s = O.Grid(n), ctxs(n, prec)
+    gsh1 = np.array([0.35, 0.15, 0.65, 0.85, 0.65, 0.15])
+    gsh2 = np.array([0.55, 0.25, 0.05, 0.45, 0.95, 0.75])
+    g.set_shifts(gsh1)
+    s.set_shifts(gsh1)
+    u0 = g.field(seed=78)
+    s.put_state(u0)
+    s.stage_nonlinear(1)
+    uy, uny, _, dt = oracle_stage(g, u0.copy(), u0.copy(), 1)
+    xf, yf, zf = g.step_factors(0.01, dt)
+    s.stage_update(1, dt, xf, yf, zf)
+    g.set_shifts(gsh2)
+    s.set_shifts(gsh2)
+    s.stage_nonlinear(2)
+    s.stage_update(2, dt, xf, yf, zf)
+    uy2, uny2, _, _ = oracle_stage(g, uy, uny, 2, dt)
+    m = g.mask()[None]
+    assert rel(s.get_state(), uny2, m) < TOL[prec]
+    assert rel(s.get_field(P.FIELD_U, 3), uy2[:3], m) < TOL[prec]
+
+
 def test_stage_order_is_enforced(P, ctxs):
     s = ctxs(32, 8)
     with pytest.raises(P.PsdnsError, match="without a preceding"):
