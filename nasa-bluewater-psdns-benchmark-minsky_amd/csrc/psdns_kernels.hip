@@ -221,18 +221,20 @@ struct BufView {
       return {__hiloint2double((int)d[1], (int)d[0]), __hiloint2double((int)d[3], (int)d[2])};
     }
   }
-  __device__ __forceinline__ void store(unsigned off, C<R> v) const {
+  // soff: a wave-uniform byte offset added to off (a scalar register); a
+  // lane with off = kOob stays out of range
+  __device__ __forceinline__ void store(unsigned off, C<R> v, unsigned soff = 0) const {
     asm volatile("" : "+v"(off));
     if constexpr (sizeof(R) == 4) {
       using V2 = unsigned __attribute__((ext_vector_type(2)));
       V2 d = {__float_as_uint(v.x), __float_as_uint(v.y)};
-      __builtin_amdgcn_raw_buffer_store_b64(d, r, off, 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(d, r, off, soff, 0);
     } else {
       using V4 = unsigned __attribute__((ext_vector_type(4)));
       const unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
       const unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
       V4 d = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
-      __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(d, r, off, soff, 0);
     }
   }
 };
@@ -547,10 +549,17 @@ struct XPass {
     const int q = s.q;
     const unsigned lofs = (unsigned)FF::natural_line(tid) * K;
     const R half = (R)0.5 * scale;
+    // with T a multiple of K, point k = q + T j lies in kx block q / K + j T / K
+    // at q % K: one lane offset, and j's part in the scalar offset
+    const unsigned qu = (unsigned)q;
+    const unsigned vq = (lofs + qu / K * (unsigned)(N * K) + qu % K) * (unsigned)sizeof(C<R>);
+    // m - k = (T - q) + (m - T (j + 1)) for j > 0: a lane part (kept opaque so
+    // that it is not folded back into m - k) and a constant
+    const int tq = opaque(T - q);
 #pragma unroll
     for (int j = 0; j < E; j++) {
       const int k = q + T * j;  // k > 0 for j > 0
-      const int pk = FF::out_pos(k), pm = FF::out_pos(j == 0 ? (M - k) & (M - 1) : M - k);
+      const int pk = FF::out_pos(k), pm = FF::out_pos(j == 0 ? (M - k) & (M - 1) : tq + (M - T * (j + 1)));
       const C<R> zk = {s.re[pk], s.im[pk]};
       const C<R> zm = {s.re[pm], -s.im[pm]};  // conj Z[m-k]
       const C<R> sm = zk + zm;
@@ -560,8 +569,14 @@ struct XPass {
       C<R> o = {fmar(t.y, d.x, fmar(t.x, d.y, sm.x)),
                 fmar(t.y, d.y, fmar(-t.x, d.x, sm.y))};
       if constexpr (SCALE) o = cscale(o, half);
-      const unsigned ku = (unsigned)k;
-      dst.store(k < g.max_al_x ? (lofs + ku / K * (unsigned)(N * K) + ku % K) * (unsigned)sizeof(C<R>) : kOob, o);
+      if constexpr (T % K == 0) {
+        // k < max_al_x as q < max_al_x - T j (a scalar bound)
+        const int lim = __builtin_amdgcn_readfirstlane(g.max_al_x - T * j);
+        dst.store(q < lim ? vq : kOob, o, (unsigned)(j * (T / K) * N * K * (int)sizeof(C<R>)));
+      } else {
+        const unsigned ku = (unsigned)k;
+        dst.store(k < g.max_al_x ? (lofs + ku / K * (unsigned)(N * K) + ku % K) * (unsigned)sizeof(C<R>) : kOob, o);
+      }
     }
   }
 };
@@ -649,8 +664,10 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
 }
 
 // Fused x pass: c2r of u, v, w (from xi) -> realspace_vel products
-// (+ velmax, FP64, realspace_vel.f:108-110) -> r2c x 1/n of the five
-// products into X slots 0..4.  Physical space stays in registers.
+// (+ velmax, realspace_vel.f:108-110: the sums and maximum in the state
+// precision, the work-group and device maxima in FP64) -> r2c of the five
+// products into X slots 0..4, unnormalized (x 2n; the update's norm
+// carries the factor).  Physical space stays in registers.
 // Persistent over tiles (one y, B consecutive z): the staging copy of the
 // next variable is issued before each inverse transform, and that of the
 // next tile's u before the five forward transforms.
@@ -667,7 +684,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
   constexpr int NZB = N / B;
   const TileWalk w = tile_walk(N * NZB);
   int t = w.t;
-  double vm = 0.0;
+  R vm = 0;  // in the state precision, as realspace_vel's working precision
   if (t < w.end) {
     X::stage_load(S, xi, 0, t / NZB, (t % NZB) * B, g, true);
     for (;;) {
@@ -692,8 +709,8 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
       if (want_velmax) {
 #pragma unroll
         for (int j = 0; j < E; j++) {
-          vm = fmax(vm, fabs((double)pu[j].x) + fabs((double)pv[j].x) + fabs((double)pw[j].x));
-          vm = fmax(vm, fabs((double)pu[j].y) + fabs((double)pv[j].y) + fabs((double)pw[j].y));
+          vm = fmax(vm, fabs(pu[j].x) + fabs(pv[j].x) + fabs(pw[j].x));
+          vm = fmax(vm, fabs(pu[j].y) + fabs(pv[j].y) + fabs(pw[j].y));
         }
       }
 #pragma unroll
@@ -708,7 +725,7 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
       t = tn;
     }
   }
-  if (want_velmax) velmax_commit(vm * (double)N, velmax);
+  if (want_velmax) velmax_commit((double)vm * (double)N, velmax);
 }
 
 // realspace_vel alone on physical ux (slots 0..4)
