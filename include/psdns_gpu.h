@@ -103,6 +103,14 @@ int psdns_ek_force(psdns_ctx *c, int kf_shell, int kfor, const double *ekinf, do
  * nshell shells; tke and eps over all modes. */
 int psdns_shell_stats(psdns_ctx *c, double *ek, double *dk, int nshell, double *tke, double *eps);
 
+/* Velocity-gradient moments (vgmomt.f:68-143): du/dx and dw/dz of the state
+ * un (on the mask, unshifted grid) in physical space, dudx[m-1] and
+ * dwdz[m-1] = sum of the m-th power over the grid / (nx ny nz), m = 1..8,
+ * FP64 sums.  Skewness = [2] / [1]^1.5, flatness = [3] / [1]^2 (the
+ * normalised moments of vgmomt.f:152-156).  Uses the u slots 4 and 5
+ * (1-based) and the physical array ux(1:2) as work space. */
+int psdns_vg_moments(psdns_ctx *c, double *dudx, double *dwdz);
+
 /* Random-phase solenoidal initial field on the device (homogeneous/field.f:
  * 172-257 construction, consym.f symmetry), E(k) ~ k^4 exp(-2 (k/kpeak)^2),
  * rescaled to the given TKE; sets un and u. */

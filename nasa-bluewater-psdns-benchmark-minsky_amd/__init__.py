@@ -30,7 +30,7 @@ EXPORTS = (
     "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_init_field", "psdns_put_field",
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
-    "psdns_stage_times", "psdns_last_error",
+    "psdns_stage_times", "psdns_vg_moments", "psdns_last_error",
 )
 
 
@@ -83,6 +83,7 @@ def load_library():
         "psdns_ek_shell": [_vp, i, _dp],
         "psdns_ek_force": [_vp, i, i, _dp, _dp],
         "psdns_shell_stats": [_vp, _dp, _dp, i, _dp, _dp],
+        "psdns_vg_moments": [_vp, _dp, _dp],
         "psdns_init_field": [_vp, ctypes.c_ulonglong, d, d],
         "psdns_put_field": [_vp, i, i, _vp],
         "psdns_get_field": [_vp, i, i, _vp],
@@ -218,6 +219,12 @@ class Solver:
         t, e = np.zeros(1), np.zeros(1)
         self._chk(self.L.psdns_shell_stats(self.h, _dptr(ek), _dptr(dk), nshell, _dptr(t), _dptr(e)))
         return ek[:nshell], dk[:nshell], float(t[0]), float(e[0])
+
+    def vg_moments(self):
+        """Moments 1..8 of du/dx and dw/dz over the grid (vgmomt.f:68-143)."""
+        du, dw = np.zeros(8), np.zeros(8)
+        self._chk(self.L.psdns_vg_moments(self.h, _dptr(du), _dptr(dw)))
+        return du, dw
 
     def stage_times(self, enable=True):
         t = np.zeros(6)

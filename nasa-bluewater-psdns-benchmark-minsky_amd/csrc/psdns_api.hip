@@ -575,6 +575,35 @@ int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* 
   return PSDNS_OK;
 }
 
+int psdns_vg_moments(psdns_ctx* c, double* dudx, double* dwdz) {
+  if (int rc = ready(c)) return rc;
+  if (!dudx || !dwdz) return fail(PSDNS_EINVAL, "null moment arrays");
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  // i kx u and i kz w into the u slots 3 and 4, inverse transform into ux
+  // slots 0 and 1 (vgmomt's utx), power sums in FP64
+  void* d = (char*)c->d_u + 3 * c->csize() * c->cyl_elems();
+  psdns::launch_vg_deriv(c->c, d);
+  Ctx k = c->c;
+  k.u = d;
+  if (psdns::launch_pass(k, Pass::YInvKx, 2, 0) || psdns::launch_pass(k, Pass::ZInv, 2, 0) ||
+      psdns::launch_pass(k, Pass::XC2R, 2, 0))
+    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(double) * 16, c->c.stream));
+  psdns::launch_moments(c->c, c->d_red);
+  HIPCHK(hipGetLastError());
+  double sums[16];
+  HIPCHK(hipMemcpyAsync(sums, c->d_red, sizeof sums, hipMemcpyDeviceToHost, c->c.stream));
+  if ((rc = sync(c))) return rc;
+  // vgmomt.f:134-138: divided by the number of grid points
+  const double n3 = (double)c->n * c->n * c->n;
+  for (int m = 0; m < 8; m++) {
+    dudx[m] = sums[m] / n3;
+    dwdz[m] = sums[8 + m] / n3;
+  }
+  return PSDNS_OK;
+}
+
 int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double tke) {
   if (int rc = ready(c)) return rc;
   if (!(kpeak > 0) || !(tke > 0)) return fail(PSDNS_EINVAL, "kpeak and tke must be positive");

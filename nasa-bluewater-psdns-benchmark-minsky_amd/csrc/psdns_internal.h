@@ -89,5 +89,9 @@ void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
 void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot);
 void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0);
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
+// velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
+// power sums 1..8 of ux slots 0 and 1 added into d_out[16]
+void launch_vg_deriv(const Ctx& c, void* d);
+void launch_moments(const Ctx& c, double* d_out);
 
 }  // namespace psdns

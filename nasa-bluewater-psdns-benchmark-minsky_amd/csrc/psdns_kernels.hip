@@ -1395,6 +1395,49 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// Velocity-gradient moments (vgmomt.f:68-156): du/dx and dw/dz in physical
+// space, from i kx u and i kz w of the state on the mask (k_vg_deriv, into
+// two cylinder slots for the inverse transform), then the sums of their
+// powers 1..8 over the grid in FP64 (k_moments: out[8 v + m - 1] += sum
+// f_v^m for the two fields v = 0, 1).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_vg_deriv(const C<R>* __restrict__ un, C<R>* __restrict__ d, Geo g) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    const C<R> a = un[i], w = un[2 * tot + i];
+    d[i] = p.m ? C<R>{-(R)p.x * a.y, (R)p.x * a.x} : C<R>{0, 0};
+    d[tot + i] = p.m ? C<R>{-(R)p.kz * w.y, (R)p.kz * w.x} : C<R>{0, 0};
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_moments(const R* __restrict__ f, size_t n3, double* __restrict__ out) {
+  double s[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) s[m] = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const double a = (double)f[v * n3 + i];
+      double pw = a;
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        s[8 * v + m] += pw;
+        pw *= a;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    double t = s[m];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out + m, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Initial field (homogeneous/field.f:172-257 construction, counter-based
 // generator keyed by (seed, x, y, z)); consym on kx = 0 in a second kernel.
 // ---------------------------------------------------------------------------
@@ -1665,6 +1708,18 @@ struct LaunchP {
     hipLaunchKernelGGL((k_consym<R>), dim3(grid1d((size_t)c.geo.n * c.geo.n, 256)), dim3(256), 0, c.stream,
                        (CR*)c.un, c.geo, d_colz0);
   }
+  // i kx u, i kz w of un into d[0..1] (cylinder slots)
+  static void vg_deriv(const Ctx& c, void* d) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    hipLaunchKernelGGL((k_vg_deriv<R>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (const CR*)c.un,
+                       (CR*)d, c.geo);
+  }
+  // power sums of physical slots 0 and 1 of ux (a fixed grid: few atomics)
+  static void moments(const Ctx& c, double* d_out) {
+    const size_t n3 = (size_t)c.geo.n * c.geo.n * c.geo.n;
+    hipLaunchKernelGGL((k_moments<R>), dim3(grid1d(n3, 256 * 8) < 2048u ? grid1d(n3, 256 * 8) : 2048u), dim3(256), 0, c.stream,
+                       (const R*)c.ux, n3, d_out);
+  }
   static void scale(const Ctx& c, void* p, size_t cnt, double s) {
     hipLaunchKernelGGL((k_scale<R>), dim3(grid1d(cnt, 256 * 4)), dim3(256), 0, c.stream, (CR*)p, cnt, s);
   }
@@ -1767,6 +1822,14 @@ void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s) {
   if (c.prec == 8) LaunchP<double>::scale(c, p, cnt, s);
   else LaunchP<float>::scale(c, p, cnt, s);
+}
+void launch_vg_deriv(const Ctx& c, void* d) {
+  if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);
+  else LaunchP<float>::vg_deriv(c, d);
+}
+void launch_moments(const Ctx& c, double* d_out) {
+  if (c.prec == 8) LaunchP<double>::moments(c, d_out);
+  else LaunchP<float>::moments(c, d_out);
 }
 
 }  // namespace psdns

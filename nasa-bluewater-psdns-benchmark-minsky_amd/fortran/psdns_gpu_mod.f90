@@ -96,6 +96,12 @@ module psdns_gpu
       real(c_double), intent(out) :: tke, eps
     end function psdns_shell_stats
 
+    integer(c_int) function psdns_vg_moments(ctx, dudx, dwdz) bind(C, name="psdns_vg_moments")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dudx(8), dwdz(8)
+    end function psdns_vg_moments
+
     integer(c_int) function psdns_init_field(ctx, seed, kpeak, tke) bind(C, name="psdns_init_field")
       import :: c_int, c_ptr, c_double, c_long_long
       type(c_ptr), value :: ctx

@@ -177,6 +177,24 @@ class Grid:
         e = np.ascontiguousarray(ekinf, np.float64)
         self.L.orc_ek_force(self.h, _p(uy), _p(uny), kf_shell, kfor, _p(e))
 
+    def vg_moments(self, uny):
+        """Moments m = 1..8 of du/dx and dw/dz over the grid / n^3
+        (PSDNS/vgmomt.f:68-143): i kx u and i kz w of the state on the mask,
+        inverse transformed by kxtran, powers summed in FP64."""
+        n = self.n
+        kx = self.col_x.astype(np.float64)[:, None]
+        kz = np.where(self.col_z > n // 2, self.col_z - n, self.col_z).astype(np.float64)[:, None]
+        m = self.mask()
+        d = self.zeros_cyl(2)
+        d[0] = np.where(m, 1j * kx * uny[0], 0)
+        d[1] = np.where(m, 1j * kz * uny[2], 0)
+        f = self.kxtran(d, 2)
+        out = []
+        for v in range(2):
+            a = f[v].ravel()
+            out.append(np.array([np.sum(a ** p) for p in range(1, 9)]) / float(n) ** 3)
+        return out[0], out[1]
+
     def spectrum(self, uny, viscos, nshell=None):
         nshell = nshell or self.n
         ek, dk = np.zeros(nshell), np.zeros(nshell)

@@ -251,3 +251,22 @@ def test_stage_order_is_enforced(P, ctxs):
         rc = s.L.psdns_stage_update(s.h, 1, 0.01, xf.ctypes.data_as(dp), yf.ctypes.data_as(dp),
                                     zf.ctypes.data_as(dp), ctypes.c_void_p(1))
         s._chk(rc)
+
+
+@pytest.mark.parametrize("n,prec", [(32, 8), (64, 8), (64, 4), (128, 4)])
+def test_vg_moments(O, P, ctxs, n, prec):
+    """Moments 1..8 of du/dx and dw/dz (vgmomt.f:68-143) vs the oracle on a
+    seeded field; FP64 sums on both sides."""
+    banner("vg_moments", n, prec)
+    g, s = O.Grid(n), ctxs(n, prec)
+    u0 = g.field(seed=91 + n)
+    s.put_state(u0)
+    du, dw = s.vg_moments()
+    rdu, rdw = g.vg_moments(u0)
+    tol = 1e-11 if prec == 8 else 2e-5
+    for got, ref in ((du, rdu), (dw, rdw)):
+        # relative, with the m-th power of the rms as the floor for the
+        # near-zero odd moments
+        scale = np.maximum(np.abs(ref), np.abs(ref[1]) ** (np.arange(1, 9) / 2.0))
+        assert np.all(np.abs(got - ref) <= tol * scale), (got, ref)
+    assert du[1] > 0 and dw[1] > 0

@@ -281,3 +281,33 @@ def test_initial_field_spectrum_is_deterministic(O):
     u = g.field(seed=3)
     back = g.xktran(g.kxtran(np.concatenate([u, g.zeros_cyl(2)]), 3), 3)
     assert rel(back, u, g.mask()[None]) < 1e-14
+
+
+def test_vg_moments_closed_form(O):
+    """vgmomt.f:68-143 moments of du/dx and dw/dz against the analytic
+    derivatives of a few Fourier modes (kx >= 1) sampled on the grid."""
+    n = 32
+    g = O.Grid(n)
+    # kx = 1, 1, -2 close a triad: du/dx has a nonzero third moment
+    modes_u = [((1, 0, 0), 0.5), ((2, 0, 0), 0.3j), ((2, 1, -1), -0.25j)]
+    modes_w = [((1, 0, 3), 0.5), ((2, -2, 1), 0.1 + 0.2j)]
+    u = g.zeros_cyl(3)
+    for comp, modes in ((0, modes_u), (2, modes_w)):
+        for (kx, ky, kz), c in modes:
+            a = next(i for i in range(g.num_al) if g.col_x[i] == kx and g.col_z[i] == kz % n)
+            u[comp, a, ky % n] = c
+    x = np.arange(n) * 2 * np.pi / n
+    Y, Z, X = np.meshgrid(x, x, x, indexing="ij")
+
+    def deriv(modes, axis):
+        f = np.zeros((n, n, n))
+        for k, c in modes:
+            f += 2 * np.real(1j * k[axis] * c * np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z)))
+        return f
+
+    du, dw = g.vg_moments(u)
+    for got, f in ((du, deriv(modes_u, 0)), (dw, deriv(modes_w, 2))):
+        ref = np.array([np.mean(f ** p) for p in range(1, 9)])
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+    # the third moment is not zero, so the sign of i k is pinned
+    assert abs(du[2]) > 0.1
