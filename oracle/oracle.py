@@ -64,6 +64,7 @@ def lib():
         L.orc_ek_shell.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.orc_ek_force.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_spectrum.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_vg_moments.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
@@ -179,21 +180,10 @@ class Grid:
 
     def vg_moments(self, uny):
         """Moments m = 1..8 of du/dx and dw/dz over the grid / n^3
-        (PSDNS/vgmomt.f:68-143): i kx u and i kz w of the state on the mask,
-        inverse transformed by kxtran, powers summed in FP64."""
-        n = self.n
-        kx = self.col_x.astype(np.float64)[:, None]
-        kz = np.where(self.col_z > n // 2, self.col_z - n, self.col_z).astype(np.float64)[:, None]
-        m = self.mask()
-        d = self.zeros_cyl(2)
-        d[0] = np.where(m, 1j * kx * uny[0], 0)
-        d[1] = np.where(m, 1j * kz * uny[2], 0)
-        f = self.kxtran(d, 2)
-        out = []
-        for v in range(2):
-            a = f[v].ravel()
-            out.append(np.array([np.sum(a ** p) for p in range(1, 9)]) / float(n) ** 3)
-        return out[0], out[1]
+        (PSDNS/vgmomt.f:68-143), from the state on the mask."""
+        du, dw = np.zeros(8), np.zeros(8)
+        self.L.orc_vg_moments(self.h, _p(np.ascontiguousarray(uny[:3])), _p(du), _p(dw))
+        return du, dw
 
     def spectrum(self, uny, viscos, nshell=None):
         nshell = nshell or self.n

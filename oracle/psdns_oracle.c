@@ -616,6 +616,45 @@ void orc_spectrum(const orc_grid *g, const double *uny_, double viscos, int nshe
   *eps = e;
 }
 
+/* velocity-gradient moments (vgmomt.f:68-143): du/dx = i kx u and dw/dz =
+ * i kz w of the state on the mask, inverse transformed (kxtran), then the
+ * sums of the powers 1..8 over the grid divided by n^3 (lines 70-87 tally,
+ * 140-143 normalise). */
+void orc_vg_moments(const orc_grid *g, const double *uny_, double *dudx, double *dwdz) {
+  const int n = g->n, na = g->num_al;
+  const size_t vs = (size_t)na * n, n3 = (size_t)n * n * n;
+  const cplx *uny = (const cplx *)uny_;
+  cplx *d = calloc(2 * vs, sizeof(cplx));
+  double *f = malloc(2 * n3 * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int a = 0; a < na; a++) {
+    const double kx = g->col_x[a], kz = kwave(g->col_z[a], n);
+    for (int y = 0; y < n; y++) {
+      const size_t i = (size_t)a * n + y;
+      if (!orc_mask(g, y, a)) continue;
+      d[i] = I * kx * uny[i];
+      d[vs + i] = I * kz * uny[2 * vs + i];
+    }
+  }
+  orc_kxtran(g, (const double *)d, f, 2);
+  for (int v = 0; v < 2; v++) {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double *fv = f + v * n3;
+#pragma omp parallel for schedule(static) reduction(+ : s[:8])
+    for (size_t i = 0; i < n3; i++) {
+      double p = fv[i];
+      for (int m = 0; m < 8; m++) {
+        s[m] += p;
+        p *= fv[i];
+      }
+    }
+    double *out = v == 0 ? dudx : dwdz;
+    for (int m = 0; m < 8; m++) out[m] = s[m] / (double)n3;
+  }
+  free(d);
+  free(f);
+}
+
 /* ---------------------------------------------------------------------- */
 /* initial field                                                            */
 /* ---------------------------------------------------------------------- */

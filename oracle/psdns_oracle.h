@@ -60,6 +60,8 @@ void orc_ek_force(const orc_grid *g, double *uy, double *uny, int kf_shell,
                   int kfor, const double *ekinf);
 void orc_spectrum(const orc_grid *g, const double *uny, double viscos,
                   int nshell, double *ek, double *dk, double *tke, double *eps);
+/* moments 1..8 of du/dx and dw/dz over the grid / n^3 (vgmomt.f:68-143) */
+void orc_vg_moments(const orc_grid *g, const double *uny, double *dudx, double *dwdz);
 
 /* initial field: random-phase solenoidal construction of field.f with a
  * counter-based generator, conjugate symmetry on kx=0 (consym.f) */
