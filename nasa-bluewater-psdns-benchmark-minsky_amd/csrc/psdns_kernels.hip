@@ -386,18 +386,14 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
 // tile's B lines together fill B x 64 contiguous bytes per block).  The real transform of length n runs as
 // a complex transform of length m = n/2 on packed pairs (x[2i], x[2i+1]).
 // ---------------------------------------------------------------------------
-#ifndef PSDNS_XPASS_E
-#define PSDNS_XPASS_E 8
-#endif
-#ifndef PSDNS_XPASS_RUNS
-#define PSDNS_XPASS_RUNS 1
-#endif
 template <typename R, int N>
 struct XPass {
   static constexpr int M = N / 2;
-  static constexpr int T = Team<M, PSDNS_XPASS_E>::T, E = Team<M, PSDNS_XPASS_E>::E;
+  static constexpr int T = Team<M, 8>::T, E = Team<M, 8>::E;
   static constexpr int K = RunLines<R>::B;          // one 64-byte run of z (or of kx in X)
-  static constexpr int B = K * PSDNS_XPASS_RUNS;    // z lines per tile
+  // z lines per tile (one run; two runs per tile measured slower: the
+  // LDS of 16 lines leaves one work-group per CU)
+  static constexpr int B = K;
   static constexpr int NT = B * T;
   using FI = TeamFft<R, M, E, B, +1, LineShift<E, B>::IN>;
   using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN>;
