@@ -385,42 +385,9 @@ int psdns_columns(const psdns_ctx* c, int* col_x, int* col_z, int* cut_z) {
 int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
   if (int rc = ready(c)) return rc;
   if (!gsh) return fail(PSDNS_EINVAL, "null gsh");
-  const int n = c->n, nxh = c->nxh;
-  std::vector<double2> h(2 * (nxh + 2 * n));
-  // shifts.f:146-178: exp(i 2 pi k gsh(dir, stage) / n)
-  for (int s = 0; s < 2; s++) {
-    for (int x = 0; x < nxh; x++) {
-      const double arg = 2.0 * M_PI / n * x * gsh[3 * s + 0];
-      h[s * nxh + x] = make_double2(std::cos(arg), std::sin(arg));
-    }
-    for (int y = 0; y < n; y++) {
-      const double arg = 2.0 * M_PI / n * kwave(y, n) * gsh[3 * s + 1];
-      h[2 * nxh + s * n + y] = make_double2(std::cos(arg), std::sin(arg));
-    }
-    for (int z = 0; z < n; z++) {
-      const double arg = 2.0 * M_PI / n * kwave(z, n) * gsh[3 * s + 2];
-      h[2 * nxh + 2 * n + s * n + z] = make_double2(std::cos(arg), std::sin(arg));
-    }
-  }
-  HIPCHK(hipMemcpyAsync(c->d_shift, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->c.stream));
-  // the same factors rounded to the state precision, stage-major sx | sy | sz
-  std::vector<double> hr(2 * h.size());
-  size_t o = 0;
-  for (int s = 0; s < 2; s++) {
-    const double2* parts[3] = {h.data() + s * nxh, h.data() + 2 * nxh + s * n, h.data() + 2 * nxh + 2 * n + s * n};
-    const int len[3] = {nxh, n, n};
-    for (int d = 0; d < 3; d++)
-      for (int i = 0; i < len[d]; i++) {
-        hr[o++] = parts[d][i].x;
-        hr[o++] = parts[d][i].y;
-      }
-  }
-  if (c->prec == 8) {
-    HIPCHK(hipMemcpyAsync(c->d_shiftr, hr.data(), sizeof(double) * hr.size(), hipMemcpyHostToDevice, c->c.stream));
-    return sync(c);
-  }
-  std::vector<float> hf(hr.begin(), hr.end());
-  HIPCHK(hipMemcpyAsync(c->d_shiftr, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, c->c.stream));
+  // shifts.f:146-178: exp(i 2 pi k gsh(dir, stage) / n), formed on the
+  // device in FP64 (and rounded to the state precision for the passes)
+  psdns::launch_shifts(c->c, c->d_shift, c->d_shiftr, gsh);
   return sync(c);
 }
 

@@ -89,6 +89,9 @@ void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
 void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot);
 void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0);
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
+// phase-shift tables of both stages from gsh(3,2): FP64 s64 and state
+// precision sr (layouts of Ctx::sx/sy/sz and Ctx::sr)
+void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]
 void launch_vg_deriv(const Ctx& c, void* d);

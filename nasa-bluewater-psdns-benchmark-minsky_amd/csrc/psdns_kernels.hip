@@ -1391,6 +1391,33 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// Phase-shift factors (shifts.f:146-178): exp(i 2 pi k gsh(dir, stage) / n)
+// in FP64, entry e of the per-stage tables sx[nxh] | sy[n] | sz[n] (state
+// precision, stage-major) and of the FP64 tables sx[2][nxh], sy[2][n],
+// sz[2][n].
+// ---------------------------------------------------------------------------
+struct Gsh {
+  double v[6];
+};
+template <typename R>
+__global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>* __restrict__ sr, int n, int nxh,
+                                                Gsh gsh) {
+  const int per = nxh + 2 * n;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * per) return;
+  const int st = e / per, r = e % per;
+  const int dir = r < nxh ? 0 : (r < nxh + n ? 1 : 2);
+  const int i = dir == 0 ? r : (dir == 1 ? r - nxh : r - nxh - n);
+  const int k = dir == 0 ? i : kwave(i, n);
+  const double arg = 2.0 * M_PI / n * k * gsh.v[3 * st + dir];
+  double sn, cs;
+  sincos(arg, &sn, &cs);
+  sr[e] = C<R>{(R)cs, (R)sn};
+  const int o = dir == 0 ? st * nxh + i : (dir == 1 ? 2 * nxh + st * n + i : 2 * nxh + 2 * n + st * n + i);
+  s64[o] = make_double2(cs, sn);
+}
+
+// ---------------------------------------------------------------------------
 // Velocity-gradient moments (vgmomt.f:68-156): du/dx and dw/dz in physical
 // space, from i kx u and i kz w of the state on the mask (k_vg_deriv, into
 // two cylinder slots for the inverse transform), then the sums of their
@@ -1818,6 +1845,17 @@ void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s) {
   if (c.prec == 8) LaunchP<double>::scale(c, p, cnt, s);
   else LaunchP<float>::scale(c, p, cnt, s);
+}
+void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]) {
+  Gsh g;
+  for (int i = 0; i < 6; i++) g.v[i] = gsh[i];
+  const int tot = 2 * (c.geo.nxh + 2 * c.geo.n);
+  if (c.prec == 8)
+    hipLaunchKernelGGL((k_shifts<double>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, s64, (C<double>*)sr,
+                       c.geo.n, c.geo.nxh, g);
+  else
+    hipLaunchKernelGGL((k_shifts<float>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, s64, (C<float>*)sr,
+                       c.geo.n, c.geo.nxh, g);
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);
