@@ -79,7 +79,8 @@ int psdns_set_shifts(psdns_ctx *c, const double gsh[6]);
  * returns velmax (FP64) for the caller's dt = cfl/velmax (step.f:64-78).
  * stage_update = the y-forward part of transform + wavespace_vel + proc4a
  * (kstep 1) or proc4b (kstep 2) (timestep_module.f:244-317).  xfac(n/2),
- * yfac(n), zfac(n) are the integrating factors of step.f:366-385; forcing
+ * yfac(n), zfac(n) are the integrating factors of step.f:366-385, used by
+ * kstep 1 only (proc4b has none; they may be NULL on kstep 2); forcing
  * must be NULL (EPFOR forcing increments are not supported; FEK forcing is
  * psdns_ek_force).
  * stage_update(1) only records its arguments: the update itself runs in

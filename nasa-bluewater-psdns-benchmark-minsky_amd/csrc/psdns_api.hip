@@ -57,6 +57,7 @@ struct psdns_ctx {
   double2* d_shift = nullptr;  // sx[2][nxh], sy[2][n], sz[2][n]
   void* d_shiftr = nullptr;    // per stage: sx[nxh] | sy[n] | sz[n], precision prec
   double* d_fac = nullptr;     // xfac[nxh], yfac[n], zfac[n]
+  double* h_fac = nullptr;     // pinned host staging of the same, one copy per upload
   double* d_red = nullptr;     // reductions scratch: [2 + 2*n + n]
   unsigned long long* d_velmax = nullptr;
   int2* d_force = nullptr;
@@ -130,11 +131,15 @@ int ensure_force_list(psdns_ctx* c, int kfor) {
   return 0;
 }
 
+// the three arrays go through the pinned staging buffer as one copy; the
+// staging buffer is free again once the stream has passed the copy (every
+// caller synchronises before returning)
 int upload_factors(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
   if (!xfac || !yfac || !zfac) return fail(PSDNS_EINVAL, "null integrating-factor array");
-  HIPCHK(hipMemcpyAsync(c->d_fac, xfac, sizeof(double) * c->nxh, hipMemcpyHostToDevice, c->c.stream));
-  HIPCHK(hipMemcpyAsync(c->d_fac + c->nxh, yfac, sizeof(double) * c->n, hipMemcpyHostToDevice, c->c.stream));
-  HIPCHK(hipMemcpyAsync(c->d_fac + c->nxh + c->n, zfac, sizeof(double) * c->n, hipMemcpyHostToDevice,
+  std::memcpy(c->h_fac, xfac, sizeof(double) * c->nxh);
+  std::memcpy(c->h_fac + c->nxh, yfac, sizeof(double) * c->n);
+  std::memcpy(c->h_fac + c->nxh + c->n, zfac, sizeof(double) * c->n);
+  HIPCHK(hipMemcpyAsync(c->d_fac, c->h_fac, sizeof(double) * (c->nxh + 2 * (size_t)c->n), hipMemcpyHostToDevice,
                         c->c.stream));
   return 0;
 }
@@ -286,6 +291,8 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
   ALLOC(c->d_shiftr, c->csize() * 2 * (c->nxh + 2 * n));
   ALLOC(c->d_fac, sizeof(double) * (c->nxh + 2 * n));
+  if (hipHostMalloc((void**)&c->h_fac, sizeof(double) * (c->nxh + 2 * n), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(PSDNS_ENOMEM, "hipHostMalloc of the factor staging buffer failed"));
   ALLOC(c->d_red, sizeof(double) * (2 + 3 * (size_t)n + 64));
   ALLOC(c->d_velmax, kVelmaxBytes);
   CHK(hipMemcpy(c->d_colx, c->colx.data(), sizeof(int) * c->na, hipMemcpyHostToDevice));
@@ -360,6 +367,7 @@ int psdns_destroy(psdns_ctx* c) {
                   c->d_red,  c->d_velmax, c->d_force};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->h_fac) (void)hipHostFree(c->h_fac);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->c.stream) (void)hipStreamDestroy(c->c.stream);
@@ -454,7 +462,8 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
     return fail(PSDNS_ESTATE, "stage_update(%d) without a preceding stage_nonlinear(%d)", kstep, kstep);
   if (!(dt > 0.0)) return fail(PSDNS_EINVAL, "dt = %g must be positive", dt);
   int rc;
-  if (kstep == 1 || xfac) {
+  // proc4b (stage 2) has no integrating factors: they are used by stage 1 only
+  if (kstep == 1) {
     if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
   }
   const double dt2 = dt / 2.0;
