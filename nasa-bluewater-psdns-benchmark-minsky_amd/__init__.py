@@ -115,10 +115,16 @@ def _dptr(a):
 class Solver:
     """One device context (psdns_ctx) for an n^3 box at precision prec (4|8)."""
 
-    def __init__(self, n, prec=8, viscos=0.01):
+    def __init__(self, n, prec=8, viscos=0.01, nc=0, pr=None, grad=None):
+        """nc passive scalars with Prandtl/Schmidt numbers pr[nc] and mean
+        gradients grad[nc][3] (state slots 3 .. 2 + nc)."""
         self.L = load_library()
-        self.n, self.prec, self.viscos = n, prec, viscos
-        p = Params(nx=n, ny=n, nz=n, nc=0, prec=prec, kshift=1, mask_kind=0, viscos=viscos)
+        self.n, self.prec, self.viscos, self.nc = n, prec, viscos, nc
+        p = Params(nx=n, ny=n, nz=n, nc=nc, prec=prec, kshift=1, mask_kind=0, viscos=viscos)
+        for i in range(nc):
+            p.pr[i] = (pr or [1.0] * nc)[i]
+            for d in range(3):
+                p.grad[i][d] = (grad or [(0.0, 0.0, 0.0)] * nc)[i][d]
         h = _vp()
         self._chk(self.L.psdns_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -152,11 +158,14 @@ class Solver:
 
     # state and fields
     def put_state(self, uny):
-        a = np.ascontiguousarray(uny[:3], self.cdtype)
+        """velocity (3) and, with scalars, the nc scalar fields"""
+        a = np.ascontiguousarray(uny[:3 + self.nc], self.cdtype)
+        if a.shape[0] != 3 + self.nc:
+            raise PsdnsError(f"state needs {3 + self.nc} variables, got {a.shape[0]}")
         self._chk(self.L.psdns_put_state(self.h, a.ctypes.data, 0))
 
     def get_state(self):
-        a = np.zeros((3, self.num_al, self.n), self.cdtype)
+        a = np.zeros((3 + self.nc, self.num_al, self.n), self.cdtype)
         self._chk(self.L.psdns_get_state(self.h, a.ctypes.data, 0))
         return a
 

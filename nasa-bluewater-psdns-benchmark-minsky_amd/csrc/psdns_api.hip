@@ -47,6 +47,11 @@ struct psdns_ctx {
   Ctx c;
   int n = 0, nxh = 0, na = 0, max_al_x = 0, prec = 8;
   double viscos = 0.0;
+  // passive scalars: state in un slots 3.., stage values in d_usc, and
+  // d_sc = 3 derivative slots + one transformed advection term per scalar
+  int nc = 0;
+  double pr[8] = {}, grad[8][3] = {};
+  void *d_usc = nullptr, *d_sc = nullptr;
   std::vector<int> colx, colz, cutz;
   // device allocations
   int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr, *d_col0 = nullptr;
@@ -87,7 +92,8 @@ int check_ctx(const psdns_ctx* c) {
 
 int ensure_ux(psdns_ctx* c) {
   if (c->d_ux) return 0;
-  size_t bytes = (size_t)5 * c->n * c->n * c->n * c->rsize();
+  // 5 slots (products); with scalars 6 (u, v, w and one scalar's gradient)
+  size_t bytes = (size_t)(c->nc > 0 ? 6 : 5) * c->n * c->n * c->n * c->rsize();
   if (hipMalloc(&c->d_ux, bytes) != hipSuccess)
     return fail(PSDNS_ENOMEM, "hipMalloc of the physical array (%zu bytes) failed", bytes);
   HIPCHK(hipMemsetAsync(c->d_ux, 0, bytes, c->c.stream));
@@ -199,6 +205,50 @@ int flush(psdns_ctx* c) {
   return add_time(c, 5, 6, 4);
 }
 
+// The scalars' part of a stage's nonlinear terms (run before the velocity
+// passes, whose work arrays it borrows): the phase-shifted physical velocity
+// (y inverse with phase1, z inverse, x c2r into ux 0..2), then per scalar the
+// shifted gradient (ux 3..5), the advection term (ux 3) and its normalised
+// forward transform into the scalar's term slot of d_sc.
+int scalar_nonlinear(psdns_ctx* c) {
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  const Ctx& k = c->c;
+  const size_t vb = c->csize() * c->cyl_elems();
+  const size_t pb = c->rsize() * (size_t)c->n * c->n * c->n;
+  if (psdns::launch_pass(k, Pass::YInvPhase, 3, 0) || psdns::launch_pass(k, Pass::ZInv, 3, 0) ||
+      psdns::launch_pass(k, Pass::XC2R, 3, 0))
+    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  for (int i = 0; i < c->nc; i++) {
+    psdns::launch_sc_deriv(k, (char*)c->d_usc + i * vb, c->d_sc);
+    Ctx kd = k;
+    kd.u = c->d_sc;
+    kd.ux = (char*)c->d_ux + 3 * pb;
+    if (psdns::launch_pass(kd, Pass::YInvPlain, 3, 0) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
+        psdns::launch_pass(kd, Pass::XC2R, 3, 0))
+      return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+    psdns::launch_sc_term(k, c->grad[i]);
+    Ctx kf = k;
+    kf.ux = (char*)c->d_ux + 3 * pb;
+    kf.u = (char*)c->d_sc + (3 + i) * vb;
+    if (psdns::launch_pass(kf, Pass::XR2C, 1, 0) || psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0) ||
+        psdns::launch_pass(kf, Pass::YFwd, 1, 0))
+      return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the scalars' update of stage kstep (after the velocity's)
+int scalar_update(psdns_ctx* c, int kstep, double dt2) {
+  const size_t vb = c->csize() * c->cyl_elems();
+  for (int i = 0; i < c->nc; i++)
+    psdns::launch_sc_update(c->c, kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
+                            (const char*)c->d_sc + (3 + i) * vb, dt2, 1.0 / c->pr[i]);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // a valid context with no deferred work left
 int ready(psdns_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
@@ -217,7 +267,9 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   const int n = p->nx;
   if (p->ny != n || p->nz != n) return fail(PSDNS_EINVAL, "nx, ny, nz must be equal (got %d %d %d)", p->nx, p->ny, p->nz);
   if (n < 32 || n > 1024 || (n & (n - 1))) return fail(PSDNS_EINVAL, "n = %d: need a power of two in [32, 1024]", n);
-  if (p->nc != 0) return fail(PSDNS_EINVAL, "nc = %d: passive scalars are not supported", p->nc);
+  if (p->nc < 0 || p->nc > 8) return fail(PSDNS_EINVAL, "nc = %d: need 0 to 8 passive scalars", p->nc);
+  for (int i = 0; i < p->nc; i++)
+    if (!(p->pr[i] > 0)) return fail(PSDNS_EINVAL, "pr[%d] = %g: Prandtl/Schmidt numbers must be positive", i, p->pr[i]);
   if (p->prec != 4 && p->prec != 8) return fail(PSDNS_EINVAL, "prec = %d: need 4 or 8", p->prec);
   if (p->mask_kind != 0) return fail(PSDNS_EINVAL, "mask_kind = %d: only the reference sphere (0)", p->mask_kind);
 
@@ -226,6 +278,11 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   c->nxh = n / 2;
   c->prec = p->prec;
   c->viscos = p->viscos;
+  c->nc = p->nc;
+  for (int i = 0; i < c->nc; i++) {
+    c->pr[i] = p->pr[i];
+    for (int d = 0; d < 3; d++) c->grad[i][d] = p->grad[i][d];
+  }
   // mpisetup.f:311-340
   const int ikmax = (int)(n * std::sqrt(2.0) / 3.0);
   c->max_al_x = std::min(ikmax + 1, c->nxh);
@@ -282,7 +339,13 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_colz0, sizeof(int) * n);
   ALLOC(c->d_grp, sizeof(int2) * c->ngrp);
   ALLOC(c->d_u, c->csize() * 5 * cyl);
-  ALLOC(c->d_un, c->csize() * 3 * cyl);
+  ALLOC(c->d_un, c->csize() * (3 + c->nc) * cyl);
+  if (c->nc > 0) {
+    ALLOC(c->d_usc, c->csize() * c->nc * cyl);
+    ALLOC(c->d_sc, c->csize() * (3 + c->nc) * cyl);
+    CHK(hipMemsetAsync(c->d_usc, 0, c->csize() * c->nc * cyl, c->c.stream));
+    CHK(hipMemsetAsync((char*)c->d_un + c->csize() * 3 * cyl, 0, c->csize() * c->nc * cyl, c->c.stream));
+  }
   const size_t w1_elems = (size_t)5 * c->max_al_x * n * n, xs_elems = (size_t)5 * n * n * c->nxh;
   ALLOC(c->d_w1, c->csize() * w1_elems);
   ALLOC(c->d_xi, c->csize() * w1_elems);
@@ -364,7 +427,7 @@ int psdns_destroy(psdns_ctx* c) {
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
-                  c->d_red,  c->d_velmax, c->d_force};
+                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_fac) (void)hipHostFree(c->h_fac);
@@ -403,9 +466,11 @@ int psdns_put_state(psdns_ctx* c, const void* uny, int layout) {
   if (int rc = ready(c)) return rc;
   if (!uny) return fail(PSDNS_EINVAL, "null state");
   if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
-  const size_t bytes = c->csize() * 3 * c->cyl_elems();
-  HIPCHK(hipMemcpyAsync(c->d_un, uny, bytes, hipMemcpyHostToDevice, c->c.stream));
-  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, bytes, hipMemcpyDeviceToDevice, c->c.stream));
+  const size_t vb = c->csize() * c->cyl_elems();
+  HIPCHK(hipMemcpyAsync(c->d_un, uny, vb * (3 + c->nc), hipMemcpyHostToDevice, c->c.stream));
+  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, vb * 3, hipMemcpyDeviceToDevice, c->c.stream));
+  if (c->nc > 0)
+    HIPCHK(hipMemcpyAsync(c->d_usc, (char*)c->d_un + 3 * vb, vb * c->nc, hipMemcpyDeviceToDevice, c->c.stream));
   c->last_nonlinear = 0;
   return sync(c);
 }
@@ -414,7 +479,8 @@ int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
   if (int rc = ready(c)) return rc;
   if (!uny) return fail(PSDNS_EINVAL, "null state");
   if (layout != 0) return fail(PSDNS_EINVAL, "layout %d: only 0 (reference cylinder) is supported", layout);
-  HIPCHK(hipMemcpyAsync(uny, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToHost, c->c.stream));
+  HIPCHK(hipMemcpyAsync(uny, c->d_un, c->csize() * (3 + c->nc) * c->cyl_elems(), hipMemcpyDeviceToHost,
+                        c->c.stream));
   return sync(c);
 }
 
@@ -432,6 +498,7 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
     return rc;
   }
   c->c.kstep = kstep - 1;
+  if (c->nc > 0 && (rc = scalar_nonlinear(c))) return rc;
   if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
   if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) return rc;
@@ -469,13 +536,14 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
   const double dt2 = dt / 2.0;
   const double norm = dt2 / c->n / c->n;  // wavespace_vel.f:80
   c->last_nonlinear = 0;
-  if (kstep == 1) {
+  if (kstep == 1 && c->nc == 0) {
     // deferred (the factors are on the device already)
     c->pending_update = true;
     c->pending_norm = norm;
     return sync(c);
   }
   if ((rc = y_update(c, kstep, norm, false))) return rc;
+  if (c->nc > 0 && (rc = scalar_update(c, kstep, dt2))) return rc;
   if ((rc = sync(c))) return rc;
   return add_time(c, 5, 6, 4);
 }
@@ -604,7 +672,7 @@ int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
   if (which == PSDNS_FIELD_U && nv >= 1 && nv <= 5) {
     dst = c->d_u;
     bytes = c->csize() * nv * c->cyl_elems();
-  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3) {
+  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3 + c->nc) {
     dst = c->d_un;
     bytes = c->csize() * nv * c->cyl_elems();
   } else if (which == PSDNS_FIELD_UX && nv >= 1 && nv <= 5) {
@@ -626,7 +694,7 @@ int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
   if (which == PSDNS_FIELD_U && nv >= 1 && nv <= 5) {
     src = c->d_u;
     bytes = c->csize() * nv * c->cyl_elems();
-  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3) {
+  } else if (which == PSDNS_FIELD_UN && nv >= 1 && nv <= 3 + c->nc) {
     src = c->d_un;
     bytes = c->csize() * nv * c->cyl_elems();
   } else if (which == PSDNS_FIELD_UX && nv >= 1 && nv <= 5) {

@@ -92,6 +92,13 @@ void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
 // phase-shift tables of both stages from gsh(3,2): FP64 s64 and state
 // precision sr (layouts of Ctx::sx/sy/sz and Ctx::sr)
 void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
+// passive scalars, stage c.kstep: derivatives i k (shift phi) of one scalar
+// into three cylinder slots d; the advection term of physical slots 0..5 of
+// c.ux into slot 3; the scalar update (stage 1 or 2) from the normalised
+// transform tr of the term
+void launch_sc_deriv(const Ctx& c, const void* phi, void* d);
+void launch_sc_term(const Ctx& c, const double grad[3]);
+void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr);
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]
 void launch_vg_deriv(const Ctx& c, void* d);

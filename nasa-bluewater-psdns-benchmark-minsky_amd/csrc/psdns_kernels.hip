@@ -1418,6 +1418,74 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 }
 
 // ---------------------------------------------------------------------------
+// Passive scalars (per scalar, beside the velocity stage):
+//  k_sc_deriv   phase1.f:44-64 / itransform.f:190-203: phi_s = sx sy sz phi
+//               on the mask (0 elsewhere); d[0..2] = i kx, i ky, i kz phi_s
+//  k_sc_term    itransform.f:522-531: term = u (G1 + phi_x) + v (G2 + phi_y)
+//               + w (G3 + phi_z) in physical space, into f[3 n3]
+//               (f[0..2] = u, v, w, f[3..5] = the derivatives)
+//  k_sc_update  wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz) T with
+//               T the normalised forward transform of term; proc4a.f:506-520
+//               (STAGE 1: F = xdif ydif zdif, xdif = xfac^(1/pr), step.f:
+//               387-407; un <- F (un + N), u <- F (un + 2 N)) or proc4b.f:
+//               69-74,183-190,201-205 (STAGE 2: u <- un + N, un <- u, zero
+//               mean); on the mask only.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_sc_deriv(const C<R>* __restrict__ phi, C<R>* __restrict__ d, Geo g,
+                                                  const double2* __restrict__ sx, const double2* __restrict__ sy,
+                                                  const double2* __restrict__ sz) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    C<R> ps = {0, 0};
+    if (p.m) {
+      const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
+      ps = cmul(phi[i], to_c<R>(s));
+    }
+    d[i] = {-(R)p.x * ps.y, (R)p.x * ps.x};
+    d[tot + i] = {-(R)p.ky * ps.y, (R)p.ky * ps.x};
+    d[2 * tot + i] = {-(R)p.kz * ps.y, (R)p.kz * ps.x};
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_sc_term(R* __restrict__ f, size_t n3, double g0, double g1, double g2) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+    const R u = f[i], v = f[n3 + i], w = f[2 * n3 + i];
+    const R px = f[3 * n3 + i], py = f[4 * n3 + i], pz = f[5 * n3 + i];
+    f[3 * n3 + i] = u * ((R)g0 + px) + v * ((R)g1 + py) + w * ((R)g2 + pz);
+  }
+}
+
+template <typename R, int STAGE>
+__global__ void __launch_bounds__(256)
+    k_sc_update(C<R>* __restrict__ u, C<R>* __restrict__ un, const C<R>* __restrict__ tr, Geo g, double dt2,
+                const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
+                const double* __restrict__ xfac, const double* __restrict__ yfac, const double* __restrict__ zfac,
+                double rpr) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (!p.m) continue;
+    const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
+    const C<R> sn = {(R)(-dt2 * s.x), (R)(dt2 * s.y)};  // -(dt/2) conj(s)
+    const C<R> nl = cmul(sn, tr[i]);
+    if constexpr (STAGE == 1) {
+      const R f = (R)(pow(xfac[p.x], rpr) * pow(yfac[p.y], rpr) * pow(zfac[p.z], rpr));
+      const C<R> ut = un[i];
+      un[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
+      u[i] = {f * (ut.x + nl.x * (R)2), f * (ut.y + nl.y * (R)2)};
+    } else {
+      const bool k0 = p.a == 0 && p.y == 0 && p.x == 0 && p.z == 0;
+      const C<R> v = k0 ? C<R>{0, 0} : un[i] + nl;
+      u[i] = v;
+      un[i] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Velocity-gradient moments (vgmomt.f:68-156): du/dx and dw/dz in physical
 // space, from i kx u and i kz w of the state on the mask (k_vg_deriv, into
 // two cylinder slots for the inverse transform), then the sums of their
@@ -1856,6 +1924,37 @@ void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]) {
   else
     hipLaunchKernelGGL((k_shifts<float>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, s64, (C<float>*)sr,
                        c.geo.n, c.geo.nxh, g);
+}
+void launch_sc_deriv(const Ctx& c, const void* phi, void* d) {
+  const size_t tot = (size_t)c.geo.na * c.geo.n;
+  const int k = c.kstep;
+  if (c.prec == 8)
+    hipLaunchKernelGGL((k_sc_deriv<double>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                       (const C<double>*)phi, (C<double>*)d, c.geo, c.sx[k], c.sy[k], c.sz[k]);
+  else
+    hipLaunchKernelGGL((k_sc_deriv<float>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                       (const C<float>*)phi, (C<float>*)d, c.geo, c.sx[k], c.sy[k], c.sz[k]);
+}
+void launch_sc_term(const Ctx& c, const double grad[3]) {
+  const size_t n3 = (size_t)c.geo.n * c.geo.n * c.geo.n;
+  if (c.prec == 8)
+    hipLaunchKernelGGL((k_sc_term<double>), dim3(grid1d(n3, 256 * 8)), dim3(256), 0, c.stream, (double*)c.ux, n3,
+                       grad[0], grad[1], grad[2]);
+  else
+    hipLaunchKernelGGL((k_sc_term<float>), dim3(grid1d(n3, 256 * 8)), dim3(256), 0, c.stream, (float*)c.ux, n3,
+                       grad[0], grad[1], grad[2]);
+}
+template <typename R>
+static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
+  const size_t tot = (size_t)c.geo.na * c.geo.n;
+  const int k = c.kstep;
+  const auto kern = stage == 1 ? k_sc_update<R, 1> : k_sc_update<R, 2>;
+  hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (C<R>*)u, (C<R>*)un,
+                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], c.xfac, c.yfac, c.zfac, rpr);
+}
+void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
+  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, rpr);
+  else sc_update_r<float>(c, stage, u, un, tr, dt2, rpr);
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);

@@ -193,6 +193,83 @@ class Grid:
         return ek, dk, float(t[0]), float(e[0])
 
 
+# ---------------------------------------------------------------------------
+# passive scalars (numpy over the C transforms)
+# ---------------------------------------------------------------------------
+def shift_factors(n, gsh):
+    """shifts.f:146-178: sx (2, n/2), sy (2, n), sz (2, n) = exp(i 2 pi k
+    gsh(dir, stage) / n), k the FFT-order wavenumber."""
+    gg = np.asarray(gsh, np.float64).reshape(2, 3)
+    kx = np.arange(n // 2, dtype=np.float64)
+    k = np.arange(n)
+    kw = np.where(k > n // 2, k - n, k).astype(np.float64)
+    ph = lambda kk, d: np.exp(1j * (2.0 * np.pi / n * kk[None, :] * gg[:, d:d + 1]))
+    return ph(kx, 0), ph(kw, 1), ph(kw, 2)
+
+
+def scalar_shift(g, gsh, kstep):
+    """sx(kx) sz(kz) sy(ky) of stage kstep on the cylinder (num_al, n)"""
+    sx, sy, sz = shift_factors(g.n, gsh)
+    s = kstep - 1
+    return (sx[s][g.col_x] * sz[s][g.col_z])[:, None] * sy[s][None, :]
+
+
+def scalar_terms(g, ux, phi, kstep, grad, gsh):
+    """Advection terms of the passive scalars phi (nc, num_al, n), stage
+    kstep, with ux[0:3] the phase-shifted physical velocity:
+      phase1.f:44-64          phi_s = sx sy sz phi on the mask (0 elsewhere)
+      phase1.f:58, itransform.f:190-203
+                              d phi/dy, d phi/dz, d phi/dx = i k phi_s
+      itransform.f:522-531    term = u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
+      transform.f             forward transform of term; here fully
+                              normalised (xktran: 1/(nx ny nz)), which is the
+                              reference's x r2c / nx with wavespace's
+                              1/(ny nz) (wavespace_module.f:347-352)
+    Returns xktran(term) (nc, num_al, n)."""
+    n = g.n
+    m = g.mask()
+    kx = g.col_x.astype(np.float64)[:, None]
+    kz = np.where(g.col_z > n // 2, g.col_z - n, g.col_z).astype(np.float64)[:, None]
+    y = np.arange(n)
+    ky = np.where(y > n // 2, y - n, y).astype(np.float64)[None, :]
+    sh = scalar_shift(g, gsh, kstep)
+    out = g.zeros_cyl(len(phi))
+    for i in range(len(phi)):
+        ps = np.where(m, sh * phi[i], 0)
+        d = g.zeros_cyl(3)
+        d[0], d[1], d[2] = 1j * kx * ps, 1j * ky * ps, 1j * kz * ps
+        f = g.kxtran(d, 3)
+        term = ux[0] * (grad[i][0] + f[0]) + ux[1] * (grad[i][1] + f[1]) + ux[2] * (grad[i][2] + f[2])
+        out[i] = g.xktran(np.ascontiguousarray(term[None]), 1)[0]
+    return out
+
+
+def scalar_update(g, phi_u, phi_n, terms, kstep, dt, gsh, factors=None, pr=None):
+    """Scalar part of the update, in place on phi_u / phi_n (nc, num_al, n):
+      wavespace_module.f:347-352   N = -(dt/2) conj(sx sy sz) terms on the mask
+      proc4a.f:506-520 (stage 1)   F = xdif ydif zdif, xdif = xfac^(1/pr)
+                                   (step.f:387-407);
+                                   phi_n <- F (phi_n + N), phi_u <- F (phi_n + 2 N)
+      proc4b.f:69-74, 183-190, 201-205 (stage 2)
+                                   phi_u <- phi_n + N, phi_n <- phi_u, zero mean"""
+    m = g.mask()
+    nsh = -(dt / 2.0) * np.conj(scalar_shift(g, gsh, kstep))
+    for i in range(len(phi_u)):
+        nl = nsh * terms[i]
+        if kstep == 1:
+            xf, yf, zf = factors
+            r = 1.0 / pr[i]
+            f = (xf[g.col_x] ** r * zf[g.col_z] ** r)[:, None] * (yf ** r)[None, :]
+            un = phi_n[i].copy()
+            phi_n[i] = np.where(m, f * (un + nl), phi_n[i])
+            phi_u[i] = np.where(m, f * (un + 2.0 * nl), phi_u[i])
+        else:
+            phi_u[i] = np.where(m, phi_n[i] + nl, phi_u[i])
+            phi_n[i] = np.where(m, phi_u[i], phi_n[i])
+            phi_u[i, 0, 0] = 0
+            phi_n[i, 0, 0] = 0
+
+
 def fft_c2c(x, sign):
     a = np.ascontiguousarray(x, np.complex128).copy()
     lib().orc_fft_c2c(_p(a), a.shape[-1], a.size // a.shape[-1], sign)
@@ -220,11 +297,16 @@ class Run:
     (stage 1: step -> dt, factors), wavespace_vel, proc4a | proc4b] x 2,
     then FEK forcing (ek_force) after the corrector."""
 
-    def __init__(self, grid, uny, viscos, cfl, kshift=4, kf_shell=0, shift_seed=7, ekinf=None, dt_fixed=None):
+    def __init__(self, grid, uny, viscos, cfl, kshift=4, kf_shell=0, shift_seed=7, ekinf=None, dt_fixed=None,
+                 phi=None, pr=None, grad=None):
         self.g = grid
         self.uny = np.ascontiguousarray(uny, np.complex128).copy()
         self.uy = grid.zeros_cyl(5)
         self.uy[:3] = self.uny
+        # passive scalars: state phi_n and the stage value phi_u
+        self.phi_n = None if phi is None else np.ascontiguousarray(phi, np.complex128).copy()
+        self.phi_u = None if phi is None else self.phi_n.copy()
+        self.pr, self.grad = pr, grad
         self.viscos, self.cfl, self.kshift = viscos, cfl, kshift
         self.dt_fixed = dt_fixed
         self.kf_shell = kf_shell
@@ -251,9 +333,12 @@ class Run:
     def step(self, gsh=None):
         g = self.g
         g.set_shifts(self.next_shifts() if gsh is None else gsh)
+        gsh_now = self.gsh if gsh is None else np.asarray(gsh)
         for k in (1, 2):
             g.phshift_inplace(self.uy, 3, k)
             ux = g.itransform_vel(self.uy)
+            if self.phi_n is not None:
+                terms = scalar_terms(g, ux, self.phi_u, k, self.grad, gsh_now)
             vm = g.realspace_vel(ux, k)
             nl = g.transform(ux)
             if k == 1:
@@ -267,6 +352,8 @@ class Run:
                 g.proc4a(self.uy, self.uny, *self.factors)
             else:
                 g.proc4b(self.uy, self.uny)
+            if self.phi_n is not None:
+                scalar_update(g, self.phi_u, self.phi_n, terms, k, self.dt, gsh_now, self.factors, self.pr)
         if self.kfor:
             g.ek_force(self.uy, self.uny, self.kf_shell, self.kfor, self.ekinf)
         self.istep += 1

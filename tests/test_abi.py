@@ -47,7 +47,8 @@ def test_library_exports_every_declared_symbol(P, lib):
 @pytest.mark.parametrize("bad,msg", [
     (dict(nx=48, ny=48, nz=48), "power of two"),
     (dict(nx=64, ny=32, nz=64), "must be equal"),
-    (dict(nc=2), "scalars"),
+    (dict(nc=9), "passive scalars"),
+    (dict(nc=2), "Prandtl/Schmidt numbers must be positive"),
     (dict(prec=2), "prec"),
     (dict(mask_kind=1), "mask_kind"),
 ])

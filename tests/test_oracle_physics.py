@@ -311,3 +311,60 @@ def test_vg_moments_closed_form(O):
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
     # the third moment is not zero, so the sign of i k is pinned
     assert abs(du[2]) > 0.1
+
+
+def test_scalar_shift_matches_the_velocity_shift(O):
+    """the scalar phase factors equal the oracle's velocity shift tables
+    (phshift_inplace of a field of ones on the mask)"""
+    import oracle as orc
+    n = 32
+    g = O.Grid(n)
+    gsh = np.array([0.3, 0.7, 0.1, 0.45, 0.05, 0.9])
+    g.set_shifts(gsh)
+    for k in (1, 2):
+        w = g.zeros_cyl(5)
+        w[:3] = 1.0
+        g.phshift_inplace(w, 3, k)
+        m = g.mask()
+        assert np.abs((w[0] - orc.scalar_shift(g, gsh, k))[m]).max() < 1e-14
+
+
+def test_scalar_pure_diffusion_is_exact(O):
+    """u = 0, no mean gradient: the integrating factor makes every scalar
+    mode decay as exp(-nu/pr k^2 t) exactly (proc4a/proc4b with zero terms)"""
+    n, nu, pr, dt = 32, 0.05, 0.7, 0.01
+    g = O.Grid(n)
+    phi0 = random_cylinder(g, 1, 5)
+    m = g.mask()
+    phi0[0, 0, 0] = 0  # the mean is zeroed by proc4b
+    phi0[0][~m] = 0
+    ro = O.Run(g, g.zeros_cyl(3), nu, cfl=np.pi, kshift=4, dt_fixed=dt, phi=phi0, pr=[pr], grad=[(0, 0, 0)])
+    for _ in range(3):
+        ro.step()
+    kx, ky, kz = kgrid(n, g.col_x, g.col_z)
+    ref = phi0[0] * np.exp(-nu / pr * (kx * kx + ky * ky + kz * kz) * 3 * dt)
+    assert rel(ro.phi_n[0], ref, m) < 1e-13
+
+
+def test_scalar_terms_vs_numpy(O):
+    """u.(G + grad phi) for band-limited fields (no aliasing) against a numpy
+    evaluation on the grid, no phase shift"""
+    import oracle as orc
+    n = 32
+    g = O.Grid(n)
+    x = np.arange(n) * 2 * np.pi / n
+    Y, Z, X = np.meshgrid(x, x, x, indexing="ij")
+    vel = [np.cos(X + 2 * Y) + 0.3 * np.sin(Z), 0.5 * np.sin(2 * X - Z), np.cos(Y) - 0.2 * np.cos(X + Z)]
+    phi = 0.7 * np.cos(X + Y + Z) + 0.4 * np.sin(3 * X - Z)
+    dphi = [-0.7 * np.sin(X + Y + Z) + 1.2 * np.cos(3 * X - Z), -0.7 * np.sin(X + Y + Z),
+            -0.7 * np.sin(X + Y + Z) - 0.4 * np.cos(3 * X - Z)]
+    grad = (1.0, -0.5, 0.25)
+    term = sum(vel[j] * (grad[j] + dphi[j]) for j in range(3))
+    ux = g.zeros_phys(5)
+    ux[:3] = vel
+    phic = g.xktran(phi[None], 1)
+    gsh = np.zeros(6)
+    got = orc.scalar_terms(g, ux, phic, 1, [grad], gsh)[0]
+    ref = g.xktran(term[None], 1)[0]
+    # xktran itself is pinned against numpy rfftn (test_xktran_vs_numpy)
+    assert rel(got, ref) < 1e-13
