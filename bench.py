@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--prec", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nc", type=int, default=0,
+                    help="passive scalars (Sc = 1, unit mean gradient along x, zero initial fluctuation)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -129,7 +131,8 @@ def main():
     s_bytes = 4 if prec == 4 else 8
     seed = 122929298
     print(f"[bench] n={n} prec=fp{8 * s_bytes} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    solver = P.Solver(n, prec, viscos=1.0)
+    nc = args.nc
+    solver = P.Solver(n, prec, viscos=1.0, nc=nc, pr=[1.0] * nc, grad=[(1.0, 0.0, 0.0)] * nc)
     solver.init_field(seed=seed, kpeak=3.0, tke=0.5)
     # viscosity from kmax eta = 1.5: eps = 2 nu D, eta = (nu^3/eps)^(1/4)
     _, _, tke0, eps_nu1 = solver.shell_stats(0)
@@ -167,9 +170,10 @@ def main():
     nb = pass_bytes(n, s_bytes, solver.num_al, solver.max_al_x)
     # launches per RK2 step: two of each pass, except the y inverse of
     # stage 2, which the stage-1 y update kernel performs (same bytes there:
-    # it writes yw in place of u)
+    # it writes yw in place of u) when there are no scalars; the scalars'
+    # own transforms are not in these pass timers
     per_step = {nm: 2 for nm in names}
-    per_step["y_inverse"] = 1
+    per_step["y_inverse"] = 1 if nc == 0 else 2
     passes = {}
     for i, nm in enumerate(names):
         ms = times[i] / (per_step[nm] * args.steps)
@@ -202,9 +206,11 @@ def main():
         "dtype": "f32" if prec == 4 else "f64",
         "data": "synthetic: seeded random-phase solenoidal field, E(k) ~ k^4 exp(-2(k/3)^2), TKE 0.5",
         "config": {
-            "workload": f"forced isotropic DNS {n}^3 {'FP32' if prec == 4 else 'FP64'}, velocity only (nc=0), "
-                        "RK2 + phase shifts (kshift 1), FEK forcing kf_shell 3, CFL pi",
-            "n": n, "nc": 0, "viscos": viscos, "cfl": math.pi, "kmax_eta": 1.5,
+            "workload": f"forced isotropic DNS {n}^3 {'FP32' if prec == 4 else 'FP64'}, "
+                        + ("velocity only (nc=0), " if nc == 0 else
+                           f"{nc} passive scalars (Sc=1, mean gradient along x, zero initial fluctuation), ")
+                        + "RK2 + phase shifts (kshift 1), FEK forcing kf_shell 3, CFL pi",
+            "n": n, "nc": nc, "viscos": viscos, "cfl": math.pi, "kmax_eta": 1.5,
             "grid_points_per_s": n ** 3 / sps,
             "tke": tke, "epsilon": eps, "mean_dt": float(np.mean(run.dts[-args.steps:])),
         },
