@@ -44,13 +44,13 @@ typedef struct psdns_ctx psdns_ctx;
 
 typedef struct {
   int nx, ny, nz;    /* grid, must be equal */
-  int nc;            /* passive scalars: only 0 is supported */
+  int nc;            /* passive scalars, 0..8 (state variables 4 .. 3 + nc) */
   int prec;          /* 4 = FP32, 8 = FP64 */
   int kshift;        /* informational (shifts come in through psdns_set_shifts) */
   int mask_kind;     /* 0 = reference sphere |k|/n <= sqrt(2)/3 (masks.f:67-77) */
   double viscos;     /* kinematic viscosity (input, PSDNS/input) */
-  double pr[8];      /* Prandtl/Schmidt numbers (scalars; unused while nc = 0) */
-  double grad[8][3]; /* mean scalar gradients (unused while nc = 0) */
+  double pr[8];      /* Prandtl/Schmidt numbers of the scalars (> 0), step.f:387-407 */
+  double grad[8][3]; /* mean scalar gradients G (cb), itransform.f:522-531 */
 } psdns_params;
 
 /* Set-up: mpisetup2 cylinder (mpisetup.f:292-358), masks (masks.f:46-83),
@@ -64,7 +64,8 @@ int psdns_geometry(const psdns_ctx *c, int *num_al, int *max_al_x);
 int psdns_columns(const psdns_ctx *c, int *col_x, int *col_z, int *cut_z);
 
 /* State transfer (host <-> device), layout 0 = reference cylinder
- * uny(ny, num_al, 3).  put_state also sets u = un (main.f:357-359). */
+ * uny(ny, num_al, 3 + nc) (velocity, then the scalars).  put_state also
+ * sets u = un (main.f:357-359). */
 int psdns_put_state(psdns_ctx *c, const void *uny, int layout);
 int psdns_get_state(psdns_ctx *c, void *uny, int layout);
 
@@ -120,7 +121,7 @@ int psdns_init_field(psdns_ctx *c, unsigned long long seed, double kpeak, double
 /* Per-routine entry points, each the device counterpart of one reference
  * subroutine, on the context's work arrays:
  *   slot "u"  = uy(ny, num_al, 5) cylinder   (PSDNS_FIELD_U)
- *   slot "un" = uny(ny, num_al, 3) cylinder  (PSDNS_FIELD_UN)
+ *   slot "un" = uny(ny, num_al, 3 + nc) cylinder  (PSDNS_FIELD_UN)
  *   slot "ux" = ux(nx, nz, ny, 5) physical   (PSDNS_FIELD_UX) */
 #define PSDNS_FIELD_U 0
 #define PSDNS_FIELD_UN 1
