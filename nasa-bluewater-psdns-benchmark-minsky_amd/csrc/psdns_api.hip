@@ -92,8 +92,7 @@ int check_ctx(const psdns_ctx* c) {
 
 int ensure_ux(psdns_ctx* c) {
   if (c->d_ux) return 0;
-  // 5 slots (products); with scalars 6 (u, v, w and one scalar's gradient)
-  size_t bytes = (size_t)(c->nc > 0 ? 6 : 5) * c->n * c->n * c->n * c->rsize();
+  size_t bytes = (size_t)5 * c->n * c->n * c->n * c->rsize();
   if (hipMalloc(&c->d_ux, bytes) != hipSuccess)
     return fail(PSDNS_ENOMEM, "hipMalloc of the physical array (%zu bytes) failed", bytes);
   HIPCHK(hipMemsetAsync(c->d_ux, 0, bytes, c->c.stream));
@@ -205,33 +204,29 @@ int flush(psdns_ctx* c) {
   return add_time(c, 5, 6, 4);
 }
 
-// The scalars' part of a stage's nonlinear terms (run before the velocity
-// passes, whose work arrays it borrows): the phase-shifted physical velocity
-// (y inverse with phase1, z inverse, x c2r into ux 0..2), then per scalar the
-// shifted gradient (ux 3..5), the advection term (ux 3) and its normalised
-// forward transform into the scalar's term slot of d_sc.
+// The scalars' part of a stage's nonlinear terms, run after the velocity
+// passes, per scalar: the shifted gradient (three cylinder slots of d_sc),
+// its y and z inverse transforms into xi slots 3..5 (X serves as the
+// y-transformed cylinder: W1 holds the velocity's nonlinear terms until the
+// update), the fused x pass with the velocity's z-transformed xi slots 0..2
+// (advection term, r2c / n into X slot 0), then the z and y forward
+// transforms (/ (ny nz), through xi slot 3) into the scalar's term slot.
 int scalar_nonlinear(psdns_ctx* c) {
-  int rc;
-  if ((rc = ensure_ux(c))) return rc;
   const Ctx& k = c->c;
   const size_t vb = c->csize() * c->cyl_elems();
-  const size_t pb = c->rsize() * (size_t)c->n * c->n * c->n;
-  if (psdns::launch_pass(k, Pass::YInvPhase, 3, 0) || psdns::launch_pass(k, Pass::ZInv, 3, 0) ||
-      psdns::launch_pass(k, Pass::XC2R, 3, 0))
-    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;  // one xi / W1 slot
+  void* xg = (char*)c->d_xi + 3 * xb;
   for (int i = 0; i < c->nc; i++) {
     psdns::launch_sc_deriv(k, (char*)c->d_usc + i * vb, c->d_sc);
     Ctx kd = k;
     kd.u = c->d_sc;
-    kd.ux = (char*)c->d_ux + 3 * pb;
-    if (psdns::launch_pass(kd, Pass::YInvPlain, 3, 0) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
-        psdns::launch_pass(kd, Pass::XC2R, 3, 0))
-      return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
-    psdns::launch_sc_term(k, c->grad[i]);
+    kd.w1 = c->d_xs;
+    kd.xi = xg;
     Ctx kf = k;
-    kf.ux = (char*)c->d_ux + 3 * pb;
+    kf.w1 = xg;
     kf.u = (char*)c->d_sc + (3 + i) * vb;
-    if (psdns::launch_pass(kf, Pass::XR2C, 1, 0) || psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0) ||
+    if (psdns::launch_pass(kd, Pass::YInvPlain, 3, 0) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
+        psdns::launch_x_scalar(k, c->d_xi, xg, c->grad[i]) || psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0) ||
         psdns::launch_pass(kf, Pass::YFwd, 1, 0))
       return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   }
@@ -348,7 +343,9 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   }
   const size_t w1_elems = (size_t)5 * c->max_al_x * n * n, xs_elems = (size_t)5 * n * n * c->nxh;
   ALLOC(c->d_w1, c->csize() * w1_elems);
-  ALLOC(c->d_xi, c->csize() * w1_elems);
+  // xi: 3 velocity slots (5 for the per-routine path); with scalars 6 (3
+  // more for a scalar's gradient)
+  ALLOC(c->d_xi, c->csize() * (c->nc > 0 ? 6 * w1_elems / 5 : w1_elems));
   ALLOC(c->d_xs, c->csize() * xs_elems);
   ALLOC(c->d_tw, c->csize() * n);
   ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
@@ -498,7 +495,6 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
     return rc;
   }
   c->c.kstep = kstep - 1;
-  if (c->nc > 0 && (rc = scalar_nonlinear(c))) return rc;
   if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
   if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) return rc;
@@ -509,6 +505,7 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if ((rc = mark(c, 3))) return rc;
   if ((rc = pass(c, Pass::ZFwdConv, 5))) return rc;
   if ((rc = mark(c, 4))) return rc;
+  if (c->nc > 0 && (rc = scalar_nonlinear(c))) return rc;
   if (kstep == 1) {
     if ((rc = read_velmax(c, velmax))) return rc;
   }

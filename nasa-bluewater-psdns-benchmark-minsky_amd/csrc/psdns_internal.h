@@ -93,11 +93,12 @@ void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
 // precision sr (layouts of Ctx::sx/sy/sz and Ctx::sr)
 void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
 // passive scalars, stage c.kstep: derivatives i k (shift phi) of one scalar
-// into three cylinder slots d; the advection term of physical slots 0..5 of
-// c.ux into slot 3; the scalar update (stage 1 or 2) from the normalised
-// transform tr of the term
+// into three cylinder slots d; the scalar update (stage 1 or 2) from the
+// normalised transform tr of the advection term
 void launch_sc_deriv(const Ctx& c, const void* phi, void* d);
-void launch_sc_term(const Ctx& c, const double grad[3]);
+// fused x pass of one scalar: c2r of the velocity xv[0..2] and the gradient
+// xg[0..2] (z-transformed), the advection term, r2c x 1/n into c.xs slot 0
+int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]);
 void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr);
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]

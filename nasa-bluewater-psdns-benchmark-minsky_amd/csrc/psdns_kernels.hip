@@ -724,6 +724,67 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
   if (want_velmax) velmax_commit((double)vm * (double)N, velmax);
 }
 
+// Fused x pass of one passive scalar (itransform.f:522-531 between the x
+// transforms): c2r of u, v, w (xv, the velocity's z-transformed slots 0..2)
+// and of the shifted gradient phi_x, phi_y, phi_z (xg slots 0..2), the
+// advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z) in
+// registers, r2c x 1/n into X slot 0.  Same tiles and staging as k_x_fused
+// (one tile per work-group): the six variables pass through the LDS staging
+// buffer in the order u, phi_x, v, phi_y, w, phi_z, each copied under the
+// previous one's transform.
+template <typename R, int N>
+__global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
+    k_x_scalar(const C<R>* __restrict__ xv, const C<R>* __restrict__ xg, C<R>* __restrict__ xs, Geo g,
+               const C<R>* __restrict__ tw, double g0, double g1, double g2) {
+  using X = XPass<R, N>;
+  constexpr int E = X::E, B = X::B;
+  __shared__ R lds[X::kLdsReals];
+  __shared__ C<R> stw[X::kTw], xtw[X::M];
+  __shared__ __attribute__((aligned(16))) C<R> S[X::M * B];
+  X::fill(stw, xtw, tw);
+  constexpr int NZB = N / B;
+  // one tile per work-group (y, B consecutive z)
+  const int t = xcd_block(blockIdx.x, gridDim.x);
+  const int y = t / NZB, z0 = (t % NZB) * B;
+  C<R> pa[E], pb[E], acc[E];
+  // acc += a (gd + b) on both packed points
+  auto add = [&](R gd, bool first) {
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const C<R> tj = {pa[j].x * (gd + pb[j].x), pa[j].y * (gd + pb[j].y)};
+      acc[j] = first ? tj : acc[j] + tj;
+    }
+  };
+  X::stage_load(S, xv, 0, y, z0, g, true);
+  __syncthreads();
+  X::c2r_staged_pre(pa, S, xtw);  // u
+  X::stage_load(S, xg, 0, y, z0, g, true);
+  X::c2r_staged_fft(pa, lds, stw);
+  __syncthreads();
+  X::c2r_staged_pre(pb, S, xtw);  // phi_x
+  X::stage_load(S, xv, 1, y, z0, g, true);
+  X::c2r_staged_fft(pb, lds, stw);
+  add((R)g0, true);
+  __syncthreads();
+  X::c2r_staged_pre(pa, S, xtw);  // v
+  X::stage_load(S, xg, 1, y, z0, g, true);
+  X::c2r_staged_fft(pa, lds, stw);
+  __syncthreads();
+  X::c2r_staged_pre(pb, S, xtw);  // phi_y
+  X::stage_load(S, xv, 2, y, z0, g, true);
+  X::c2r_staged_fft(pb, lds, stw);
+  add((R)g1, false);
+  __syncthreads();
+  X::c2r_staged_pre(pa, S, xtw);  // w
+  X::stage_load(S, xg, 2, y, z0, g, true);
+  X::c2r_staged_fft(pa, lds, stw);
+  __syncthreads();
+  X::c2r_staged_pre(pb, S, xtw);  // phi_z
+  X::c2r_staged_fft(pb, lds, stw);
+  add((R)g2, false);
+  X::template r2c<true>(acc, xs, 0, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
+}
+
 // realspace_vel alone on physical ux (slots 0..4)
 template <typename R>
 __global__ void __launch_bounds__(256)
@@ -1421,9 +1482,8 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 // Passive scalars (per scalar, beside the velocity stage):
 //  k_sc_deriv   phase1.f:44-64 / itransform.f:190-203: phi_s = sx sy sz phi
 //               on the mask (0 elsewhere); d[0..2] = i kx, i ky, i kz phi_s
-//  k_sc_term    itransform.f:522-531: term = u (G1 + phi_x) + v (G2 + phi_y)
-//               + w (G3 + phi_z) in physical space, into f[3 n3]
-//               (f[0..2] = u, v, w, f[3..5] = the derivatives)
+//  k_x_scalar   (below, with the x passes) itransform.f:522-531: the
+//               advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
 //  k_sc_update  wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz) T with
 //               T the normalised forward transform of term; proc4a.f:506-520
 //               (STAGE 1: F = xdif ydif zdif, xdif = xfac^(1/pr), step.f:
@@ -1446,15 +1506,6 @@ __global__ void __launch_bounds__(256) k_sc_deriv(const C<R>* __restrict__ phi, 
     d[i] = {-(R)p.x * ps.y, (R)p.x * ps.x};
     d[tot + i] = {-(R)p.ky * ps.y, (R)p.ky * ps.x};
     d[2 * tot + i] = {-(R)p.kz * ps.y, (R)p.kz * ps.x};
-  }
-}
-
-template <typename R>
-__global__ void __launch_bounds__(256) k_sc_term(R* __restrict__ f, size_t n3, double g0, double g1, double g2) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
-    const R u = f[i], v = f[n3 + i], w = f[2 * n3 + i];
-    const R px = f[3 * n3 + i], py = f[4 * n3 + i], pz = f[5 * n3 + i];
-    f[3 * n3 + i] = u * ((R)g0 + px) + v * ((R)g1 + py) + w * ((R)g2 + pz);
   }
 }
 
@@ -1935,14 +1986,28 @@ void launch_sc_deriv(const Ctx& c, const void* phi, void* d) {
     hipLaunchKernelGGL((k_sc_deriv<float>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
                        (const C<float>*)phi, (C<float>*)d, c.geo, c.sx[k], c.sy[k], c.sz[k]);
 }
-void launch_sc_term(const Ctx& c, const double grad[3]) {
-  const size_t n3 = (size_t)c.geo.n * c.geo.n * c.geo.n;
-  if (c.prec == 8)
-    hipLaunchKernelGGL((k_sc_term<double>), dim3(grid1d(n3, 256 * 8)), dim3(256), 0, c.stream, (double*)c.ux, n3,
-                       grad[0], grad[1], grad[2]);
-  else
-    hipLaunchKernelGGL((k_sc_term<float>), dim3(grid1d(n3, 256 * 8)), dim3(256), 0, c.stream, (float*)c.ux, n3,
-                       grad[0], grad[1], grad[2]);
+template <typename R, int N>
+static void x_scalar_n(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  using X = XPass<R, N>;
+  const auto kern = k_x_scalar<R, N>;
+  const unsigned grid = N * (N / X::B);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const C<R>*)xv, (const C<R>*)xg, (C<R>*)c.xs,
+                     c.geo, (const C<R>*)c.tw, grad[0], grad[1], grad[2]);
+}
+template <typename R>
+static int x_scalar_r(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  switch (c.geo.n) {
+    case 32: x_scalar_n<R, 32>(c, xv, xg, grad); return 0;
+    case 64: x_scalar_n<R, 64>(c, xv, xg, grad); return 0;
+    case 128: x_scalar_n<R, 128>(c, xv, xg, grad); return 0;
+    case 256: x_scalar_n<R, 256>(c, xv, xg, grad); return 0;
+    case 512: x_scalar_n<R, 512>(c, xv, xg, grad); return 0;
+    case 1024: x_scalar_n<R, 1024>(c, xv, xg, grad); return 0;
+  }
+  return -1;
+}
+int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  return c.prec == 8 ? x_scalar_r<double>(c, xv, xg, grad) : x_scalar_r<float>(c, xv, xg, grad);
 }
 template <typename R>
 static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
