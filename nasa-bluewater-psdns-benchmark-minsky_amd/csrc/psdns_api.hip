@@ -205,10 +205,10 @@ int flush(psdns_ctx* c) {
 }
 
 // The scalars' part of a stage's nonlinear terms, run after the velocity
-// passes, per scalar: the shifted gradient (three cylinder slots of d_sc),
-// its y and z inverse transforms into xi slots 3..5 (X serves as the
-// y-transformed cylinder: W1 holds the velocity's nonlinear terms until the
-// update), the fused x pass with the velocity's z-transformed xi slots 0..2
+// passes, per scalar: the shifted gradient, y-transformed in one pass (X
+// serves as the y-transformed cylinder: W1 holds the velocity's nonlinear
+// terms until the update), its z inverse transforms into xi slots 3..5, the
+// fused x pass with the velocity's z-transformed xi slots 0..2
 // (advection term, r2c / n into X slot 0), then the z and y forward
 // transforms (/ (ny nz), through xi slot 3) into the scalar's term slot.
 int scalar_nonlinear(psdns_ctx* c) {
@@ -217,15 +217,13 @@ int scalar_nonlinear(psdns_ctx* c) {
   const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;  // one xi / W1 slot
   void* xg = (char*)c->d_xi + 3 * xb;
   for (int i = 0; i < c->nc; i++) {
-    psdns::launch_sc_deriv(k, (char*)c->d_usc + i * vb, c->d_sc);
     Ctx kd = k;
-    kd.u = c->d_sc;
     kd.w1 = c->d_xs;
     kd.xi = xg;
     Ctx kf = k;
     kf.w1 = xg;
     kf.u = (char*)c->d_sc + (3 + i) * vb;
-    if (psdns::launch_pass(kd, Pass::YInvPlain, 3, 0) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
+    if (psdns::launch_y_grad(k, (char*)c->d_usc + i * vb, c->d_xs) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
         psdns::launch_x_scalar(k, c->d_xi, xg, c->grad[i]) || psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0) ||
         psdns::launch_pass(kf, Pass::YFwd, 1, 0))
       return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);

@@ -92,10 +92,11 @@ void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
 // phase-shift tables of both stages from gsh(3,2): FP64 s64 and state
 // precision sr (layouts of Ctx::sx/sy/sz and Ctx::sr)
 void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
-// passive scalars, stage c.kstep: derivatives i k (shift phi) of one scalar
-// into three cylinder slots d; the scalar update (stage 1 or 2) from the
-// normalised transform tr of the advection term
-void launch_sc_deriv(const Ctx& c, const void* phi, void* d);
+// passive scalars, stage c.kstep: the shifted gradient i k (sx sy sz phi)
+// of one scalar, y-transformed (y c2c +i), into yw slots 0..2; the scalar
+// update (stage 1 or 2) from the normalised transform tr of the advection
+// term
+int launch_y_grad(const Ctx& c, const void* phi, void* yw);
 // fused x pass of one scalar: c2r of the velocity xv[0..2] and the gradient
 // xg[0..2] (z-transformed), the advection term, r2c x 1/n into c.xs slot 0
 int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]);

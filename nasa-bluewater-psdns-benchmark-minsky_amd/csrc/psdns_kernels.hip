@@ -321,6 +321,57 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 }
 
 // ---------------------------------------------------------------------------
+// Shifted gradient of a passive scalar, y-transformed (phase1.f:44-64,
+// itransform.f:190-203): phi_s = sx sy sz phi on the mask (0 elsewhere);
+// yw slot 0 = i kx Y[phi_s], slot 1 = Y[i ky phi_s], slot 2 = i kz Y[phi_s]
+// with Y the y c2c +i of a column (kx and kz are constant along it, so two
+// transforms give the three derivatives).  One tile of B columns per
+// work-group, natural mapping in and out; yw in the cylinder layout.
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+__global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
+    k_y_grad(const C<R>* __restrict__ phi, C<R>* __restrict__ yw, Geo g, const double2* __restrict__ sx,
+             const double2* __restrict__ sy, const double2* __restrict__ sz, const C<R>* __restrict__ tw,
+             const int* __restrict__ colx, const int* __restrict__ colz) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = YInv<R>::B;
+  using F = TeamFft<R, N, E, B, +1>;
+  __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  const int tid = threadIdx.x;
+  const Slot<R> io = F::natural(lds, tid);
+  const int li = F::natural_line(tid), q = io.q;
+  const int a0 = xcd_block(blockIdx.x, gridDim.x) * B;
+  const bool act = a0 + li < g.na;
+  const int a = act ? a0 + li : g.na - 1;
+  const int x = colx[a], z = colz[a], kz = kwave(z, N);
+  const double2 sxz = zmul(sx[x], sz[z]);
+  const BufView<R> src(phi + (size_t)a * N, kOob);
+  C<R> b[E], b2[E];
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int y = q + T * j, ky = kwave(y, N);
+    const bool keep = in_mask(x, ky, kz, y, z, N);
+    const C<R> v = src.load(keep ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob);
+    b[j] = keep ? cmul(v, to_c<R>(zmul(sxz, sy[y]))) : C<R>{0, 0};
+    b2[j] = {-(R)ky * b[j].y, (R)ky * b[j].x};
+  }
+  __syncthreads();
+  F::template run<false, false, false, false>(b, lds, io, io, stw);
+  F::template run<false, false, false, false>(b2, lds, io, io, stw);
+  const size_t tot = (size_t)g.na * N;
+  const BufView<R> d0(yw + (size_t)a0 * N, kOob), d1(yw + tot + (size_t)a0 * N, kOob),
+      d2(yw + 2 * tot + (size_t)a0 * N, kOob);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const unsigned o = act ? ((unsigned)li * N + q + T * m) * (unsigned)sizeof(C<R>) : kOob;
+    d0.store(o, C<R>{-(R)x * b[m].y, (R)x * b[m].x});
+    d1.store(o, b2[m]);
+    d2.store(o, C<R>{-(R)kz * b[m].y, (R)kz * b[m].x});
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse z pass (itransform_vel.f:80-94, kxtran.f:52-75): y-transformed
 // cylinder yw -> xi[v][y][kx][z].  A tile is one variable, one kx
 // (< max_al_x) and B consecutive y.  Its z lines are gathered from the kx's
@@ -1480,8 +1531,8 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 
 // ---------------------------------------------------------------------------
 // Passive scalars (per scalar, beside the velocity stage):
-//  k_sc_deriv   phase1.f:44-64 / itransform.f:190-203: phi_s = sx sy sz phi
-//               on the mask (0 elsewhere); d[0..2] = i kx, i ky, i kz phi_s
+//  k_y_grad     (with the y passes) phase1.f:44-64 / itransform.f:190-203:
+//               the shifted gradient, y-transformed
 //  k_x_scalar   (below, with the x passes) itransform.f:522-531: the
 //               advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
 //  k_sc_update  wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz) T with
@@ -1491,24 +1542,6 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 //               69-74,183-190,201-205 (STAGE 2: u <- un + N, un <- u, zero
 //               mean); on the mask only.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(256) k_sc_deriv(const C<R>* __restrict__ phi, C<R>* __restrict__ d, Geo g,
-                                                  const double2* __restrict__ sx, const double2* __restrict__ sy,
-                                                  const double2* __restrict__ sz) {
-  const size_t tot = (size_t)g.na * g.n;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const Pt p = point(i, g);
-    C<R> ps = {0, 0};
-    if (p.m) {
-      const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
-      ps = cmul(phi[i], to_c<R>(s));
-    }
-    d[i] = {-(R)p.x * ps.y, (R)p.x * ps.x};
-    d[tot + i] = {-(R)p.ky * ps.y, (R)p.ky * ps.x};
-    d[2 * tot + i] = {-(R)p.kz * ps.y, (R)p.kz * ps.x};
-  }
-}
-
 template <typename R, int STAGE>
 __global__ void __launch_bounds__(256)
     k_sc_update(C<R>* __restrict__ u, C<R>* __restrict__ un, const C<R>* __restrict__ tr, Geo g, double dt2,
@@ -1976,38 +2009,28 @@ void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]) {
     hipLaunchKernelGGL((k_shifts<float>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, s64, (C<float>*)sr,
                        c.geo.n, c.geo.nxh, g);
 }
-void launch_sc_deriv(const Ctx& c, const void* phi, void* d) {
-  const size_t tot = (size_t)c.geo.na * c.geo.n;
-  const int k = c.kstep;
-  if (c.prec == 8)
-    hipLaunchKernelGGL((k_sc_deriv<double>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
-                       (const C<double>*)phi, (C<double>*)d, c.geo, c.sx[k], c.sy[k], c.sz[k]);
-  else
-    hipLaunchKernelGGL((k_sc_deriv<float>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
-                       (const C<float>*)phi, (C<float>*)d, c.geo, c.sx[k], c.sy[k], c.sz[k]);
-}
 template <typename R, int N>
-static void x_scalar_n(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
-  using X = XPass<R, N>;
-  const auto kern = k_x_scalar<R, N>;
-  const unsigned grid = N * (N / X::B);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const C<R>*)xv, (const C<R>*)xg, (C<R>*)c.xs,
-                     c.geo, (const C<R>*)c.tw, grad[0], grad[1], grad[2]);
+static void y_grad_n(const Ctx& c, const void* phi, void* yw) {
+  constexpr int NT = YInv<R>::B * Team<N>::T;
+  const unsigned grid = (unsigned)((c.geo.na + YInv<R>::B - 1) / YInv<R>::B);
+  const int k = c.kstep;
+  hipLaunchKernelGGL((k_y_grad<R, N>), dim3(grid), dim3(NT), 0, c.stream, (const C<R>*)phi, (C<R>*)yw, c.geo,
+                     c.sx[k], c.sy[k], c.sz[k], (const C<R>*)c.tw, c.geo.colx, c.geo.colz);
 }
 template <typename R>
-static int x_scalar_r(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+static int y_grad_r(const Ctx& c, const void* phi, void* yw) {
   switch (c.geo.n) {
-    case 32: x_scalar_n<R, 32>(c, xv, xg, grad); return 0;
-    case 64: x_scalar_n<R, 64>(c, xv, xg, grad); return 0;
-    case 128: x_scalar_n<R, 128>(c, xv, xg, grad); return 0;
-    case 256: x_scalar_n<R, 256>(c, xv, xg, grad); return 0;
-    case 512: x_scalar_n<R, 512>(c, xv, xg, grad); return 0;
-    case 1024: x_scalar_n<R, 1024>(c, xv, xg, grad); return 0;
+    case 32: y_grad_n<R, 32>(c, phi, yw); return 0;
+    case 64: y_grad_n<R, 64>(c, phi, yw); return 0;
+    case 128: y_grad_n<R, 128>(c, phi, yw); return 0;
+    case 256: y_grad_n<R, 256>(c, phi, yw); return 0;
+    case 512: y_grad_n<R, 512>(c, phi, yw); return 0;
+    case 1024: y_grad_n<R, 1024>(c, phi, yw); return 0;
   }
   return -1;
 }
-int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
-  return c.prec == 8 ? x_scalar_r<double>(c, xv, xg, grad) : x_scalar_r<float>(c, xv, xg, grad);
+int launch_y_grad(const Ctx& c, const void* phi, void* yw) {
+  return c.prec == 8 ? y_grad_r<double>(c, phi, yw) : y_grad_r<float>(c, phi, yw);
 }
 template <typename R>
 static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
