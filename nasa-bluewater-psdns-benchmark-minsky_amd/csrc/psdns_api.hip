@@ -237,7 +237,7 @@ int scalar_update(psdns_ctx* c, int kstep, double dt2) {
   const size_t vb = c->csize() * c->cyl_elems();
   for (int i = 0; i < c->nc; i++)
     psdns::launch_sc_update(c->c, kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
-                            (const char*)c->d_sc + (3 + i) * vb, dt2, 1.0 / c->pr[i]);
+                            (const char*)c->d_sc + (3 + i) * vb, dt2, 1.0 / c->pr[i], c->d_red);
   HIPCHK(hipGetLastError());
   return 0;
 }

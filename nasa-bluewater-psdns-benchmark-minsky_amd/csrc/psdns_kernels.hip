@@ -1542,12 +1542,18 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 //               69-74,183-190,201-205 (STAGE 2: u <- un + N, un <- u, zero
 //               mean); on the mask only.
 // ---------------------------------------------------------------------------
+// xdif | ydif | zdif = (xfac | yfac | zfac)^(1/pr) (step.f:387-407), len entries
+__global__ void __launch_bounds__(256) k_sc_dif(const double* __restrict__ fac, int len, double rpr,
+                                                double* __restrict__ dif) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) dif[e] = pow(fac[e], rpr);
+}
+
 template <typename R, int STAGE>
 __global__ void __launch_bounds__(256)
     k_sc_update(C<R>* __restrict__ u, C<R>* __restrict__ un, const C<R>* __restrict__ tr, Geo g, double dt2,
                 const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
-                const double* __restrict__ xfac, const double* __restrict__ yfac, const double* __restrict__ zfac,
-                double rpr) {
+                const double* __restrict__ xdif, const double* __restrict__ ydif, const double* __restrict__ zdif) {
   const size_t tot = (size_t)g.na * g.n;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const Pt p = point(i, g);
@@ -1556,7 +1562,7 @@ __global__ void __launch_bounds__(256)
     const C<R> sn = {(R)(-dt2 * s.x), (R)(dt2 * s.y)};  // -(dt/2) conj(s)
     const C<R> nl = cmul(sn, tr[i]);
     if constexpr (STAGE == 1) {
-      const R f = (R)(pow(xfac[p.x], rpr) * pow(yfac[p.y], rpr) * pow(zfac[p.z], rpr));
+      const R f = (R)(xdif[p.x] * ydif[p.y] * zdif[p.z]);
       const C<R> ut = un[i];
       un[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
       u[i] = {f * (ut.x + nl.x * (R)2), f * (ut.y + nl.y * (R)2)};
@@ -2032,17 +2038,47 @@ static int y_grad_r(const Ctx& c, const void* phi, void* yw) {
 int launch_y_grad(const Ctx& c, const void* phi, void* yw) {
   return c.prec == 8 ? y_grad_r<double>(c, phi, yw) : y_grad_r<float>(c, phi, yw);
 }
+template <typename R, int N>
+static void x_scalar_n(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  using X = XPass<R, N>;
+  const auto kern = k_x_scalar<R, N>;
+  const unsigned grid = N * (N / X::B);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const C<R>*)xv, (const C<R>*)xg, (C<R>*)c.xs,
+                     c.geo, (const C<R>*)c.tw, grad[0], grad[1], grad[2]);
+}
 template <typename R>
-static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
+static int x_scalar_r(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  switch (c.geo.n) {
+    case 32: x_scalar_n<R, 32>(c, xv, xg, grad); return 0;
+    case 64: x_scalar_n<R, 64>(c, xv, xg, grad); return 0;
+    case 128: x_scalar_n<R, 128>(c, xv, xg, grad); return 0;
+    case 256: x_scalar_n<R, 256>(c, xv, xg, grad); return 0;
+    case 512: x_scalar_n<R, 512>(c, xv, xg, grad); return 0;
+    case 1024: x_scalar_n<R, 1024>(c, xv, xg, grad); return 0;
+  }
+  return -1;
+}
+int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
+  return c.prec == 8 ? x_scalar_r<double>(c, xv, xg, grad) : x_scalar_r<float>(c, xv, xg, grad);
+}
+template <typename R>
+static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif) {
   const size_t tot = (size_t)c.geo.na * c.geo.n;
   const int k = c.kstep;
   const auto kern = stage == 1 ? k_sc_update<R, 1> : k_sc_update<R, 2>;
+  const int nxh = c.geo.nxh, n = c.geo.n;
   hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (C<R>*)u, (C<R>*)un,
-                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], c.xfac, c.yfac, c.zfac, rpr);
+                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], dif, dif + nxh, dif + nxh + n);
 }
-void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr) {
-  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, rpr);
-  else sc_update_r<float>(c, stage, u, un, tr, dt2, rpr);
+void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
+                      double* dif) {
+  if (stage == 1) {
+    // c.xfac | c.yfac | c.zfac are one contiguous table of nxh + 2 n entries
+    const int len = c.geo.nxh + 2 * c.geo.n;
+    hipLaunchKernelGGL(k_sc_dif, dim3((len + 255) / 256), dim3(256), 0, c.stream, c.xfac, len, rpr, dif);
+  }
+  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, dif);
+  else sc_update_r<float>(c, stage, u, un, tr, dt2, dif);
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);
