@@ -39,6 +39,7 @@ extern "C" {
 #define PSDNS_EHIP (-2)     /* HIP runtime error */
 #define PSDNS_ENOMEM (-3)   /* device allocation failed */
 #define PSDNS_ESTATE (-4)   /* call out of order (e.g. update before nonlinear) */
+#define PSDNS_EIO (-5)      /* file open, read or write failed */
 
 typedef struct psdns_ctx psdns_ctx;
 
@@ -68,6 +69,28 @@ int psdns_columns(const psdns_ctx *c, int *col_x, int *col_z, int *cut_z);
  * sets u = un (main.f:357-359). */
 int psdns_put_state(psdns_ctx *c, const void *uny, int layout);
 int psdns_get_state(psdns_ctx *c, void *uny, int layout);
+
+/* Checkpoint of un in the reference's pencil file format, replacing
+ * outpen (homogeneous/outpen.f:1-229, record layout outpen_write.f:59-76)
+ * and inpen (homogeneous/inpen.f, records read as inpen_read.f with
+ * slabpenc = 2) for one task.  The file is Fortran sequential-unformatted,
+ * big-endian (-fconvert=big-endian, makefile:251); per variable
+ * i = 1 .. 3 + nc:
+ *   record  i, nx, ny, nz                            4 x int32
+ *   record  xist, xisz, zjst, zjsz, 1, ny            6 x int32 = 1, n/2, 1, n, 1, n
+ *   for z = 1 .. n except n/2+1, two records of complex(file_prec):
+ *     buf(1:n/2, 1:n/2, z), then buf(1:n/2, n/2+2:n, z)   (x fastest)
+ * with buf(x, y, z) = uny(y, a) on cylinder column a = (x, z) and 0 off the
+ * cylinder (kxcomm1.f:586-612).  file_prec 8 = complex*16 (the shipped
+ * DOUBLE_PREC build), 4 = complex*8.  The reference's path for one task is
+ * outpen/0/outpenc.p0 (outpen_write.f:45-60, main.f:196); the caller names
+ * the path.  inpen reads either precision (from the record length), needs
+ * the same grid and 3 + nc variables (no resolution change, no kinit
+ * subset), zeroes the modes the file does not hold (ky = n/2 row, kz = n/2
+ * plane), and sets u = un as put_state does.  Bad headers or record
+ * lengths give PSDNS_EINVAL, open/read/write failures PSDNS_EIO. */
+int psdns_outpen(psdns_ctx *c, const char *path, int file_prec);
+int psdns_inpen(psdns_ctx *c, const char *path);
 
 /* Phase-shift offsets gsh(3,2) in grid cells, as produced by shifts.f
  * (shifts.f:65-84); the factors sx, sy, sz (shifts.f:146-178) are formed

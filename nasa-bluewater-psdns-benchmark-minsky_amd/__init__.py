@@ -30,7 +30,7 @@ EXPORTS = (
     "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_init_field", "psdns_put_field",
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
-    "psdns_stage_times", "psdns_vg_moments", "psdns_last_error",
+    "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_last_error",
 )
 
 
@@ -97,6 +97,8 @@ def load_library():
         "psdns_kxtran": [_vp, i],
         "psdns_xktran": [_vp, i],
         "psdns_stage_times": [_vp, i, _dp],
+        "psdns_outpen": [_vp, ctypes.c_char_p, i],
+        "psdns_inpen": [_vp, ctypes.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -168,6 +170,17 @@ class Solver:
         a = np.zeros((3 + self.nc, self.num_al, self.n), self.cdtype)
         self._chk(self.L.psdns_get_state(self.h, a.ctypes.data, 0))
         return a
+
+    def outpen(self, path, file_prec=8):
+        """Write un as the reference's pencil checkpoint file
+        (homogeneous/outpen_write.f:59-76), complex*16 (file_prec 8) or
+        complex*8 (4), big-endian Fortran records."""
+        self._chk(self.L.psdns_outpen(self.h, os.fsencode(path), file_prec))
+
+    def inpen(self, path):
+        """Read such a file (either precision) into un, and u = un
+        (homogeneous/inpen_read.f)."""
+        self._chk(self.L.psdns_inpen(self.h, os.fsencode(path)))
 
     def put_field(self, which, arr):
         dt = self.rdtype if which == FIELD_UX else self.cdtype

@@ -5,10 +5,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <cstdint>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -783,6 +786,234 @@ int psdns_xktran(psdns_ctx* c, int nv) {
   if ((rc = pass(c, Pass::XR2C, nv))) return rc;
   if ((rc = pass(c, Pass::ZFwdNorm, nv))) return rc;
   if ((rc = pass(c, Pass::YFwd, nv))) return rc;
+  return sync(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Checkpoint files in the reference's pencil format: the writer follows
+// homogeneous/outpen_write.f:59-76, the reader the slabpenc = 2 branch of
+// homogeneous/inpen_read.f.  One GPU is one task (xist = 1, xisz = n/2,
+// zjst = 1, zjsz = n), so the file is the reference's outpen/0/outpenc.p0.
+namespace {
+
+// A Fortran sequential-unformatted record as written under
+// -fconvert=big-endian (PSDNS/makefile:251): big-endian 4-byte byte count,
+// payload, the same count again.
+bool put_record(FILE* f, const void* p, size_t bytes) {
+  const uint32_t m = __builtin_bswap32((uint32_t)bytes);
+  return fwrite(&m, 4, 1, f) == 1 && (bytes == 0 || fwrite(p, bytes, 1, f) == 1) && fwrite(&m, 4, 1, f) == 1;
+}
+
+// Payload length of the next record, -1 on a short read, -2 when the
+// trailing count is not the leading one.
+long get_record(FILE* f, std::vector<unsigned char>& buf) {
+  uint32_t m0, m1;
+  if (fread(&m0, 4, 1, f) != 1) return -1;
+  m0 = __builtin_bswap32(m0);
+  buf.resize(m0);
+  if (m0 && fread(buf.data(), m0, 1, f) != 1) return -1;
+  if (fread(&m1, 4, 1, f) != 1) return -1;
+  return __builtin_bswap32(m1) == m0 ? (long)m0 : -2;
+}
+
+template <typename F>
+inline void store_be(unsigned char* d, double v) {
+  const F w = (F)v;
+  if constexpr (sizeof(F) == 8) {
+    uint64_t b;
+    std::memcpy(&b, &w, 8);
+    b = __builtin_bswap64(b);
+    std::memcpy(d, &b, 8);
+  } else {
+    uint32_t b;
+    std::memcpy(&b, &w, 4);
+    b = __builtin_bswap32(b);
+    std::memcpy(d, &b, 4);
+  }
+}
+
+template <typename F>
+inline F load_be(const unsigned char* s) {
+  F w;
+  if constexpr (sizeof(F) == 8) {
+    uint64_t b;
+    std::memcpy(&b, s, 8);
+    b = __builtin_bswap64(b);
+    std::memcpy(&w, &b, 8);
+  } else {
+    uint32_t b;
+    std::memcpy(&b, s, 4);
+    b = __builtin_bswap32(b);
+    std::memcpy(&w, &b, 4);
+  }
+  return w;
+}
+
+bool put_ints(FILE* f, std::initializer_list<int> v) {
+  std::vector<uint32_t> w;
+  for (int i : v) w.push_back(__builtin_bswap32((uint32_t)i));
+  return put_record(f, w.data(), 4 * w.size());
+}
+
+// The y rows of one z plane's two records: 0 .. n/2-1, then n/2+1 .. n-1
+// (the ky Nyquist row y = n/2 is not stored, outpen_write.f:72-73).
+inline int plane_row(int r, int n) { return r < n / 2 ? r : r + 1; }
+
+// One variable: cylinder uy(ny, num_al) of state precision S on the host ->
+// per z plane (z = n/2, the kz Nyquist plane, skipped) the pencil rows
+// buf(1:n/2, y, z), kx fastest, 0 off the cylinder (kxcomm1.f:586-612).
+template <typename S, typename F>
+bool write_var(FILE* f, const S* cyl, const std::vector<int>& colof, int n, std::vector<unsigned char>& rec) {
+  const int nxh = n / 2;
+  const size_t es = 2 * sizeof(F);
+  for (int z = 0; z < n; z++) {
+    if (z == n / 2) continue;
+    for (int part = 0; part < 2; part++) {
+      const int r0 = part == 0 ? 0 : n / 2, nr = part == 0 ? n / 2 : n / 2 - 1;
+      rec.resize((size_t)nr * nxh * es);
+      unsigned char* d = rec.data();
+      for (int r = 0; r < nr; r++) {
+        const int y = plane_row(r0 + r, n);
+        for (int x = 0; x < nxh; x++, d += es) {
+          const int a = colof[(size_t)x * n + z];
+          const S* v = a >= 0 ? cyl + 2 * ((size_t)a * n + y) : nullptr;
+          store_be<F>(d, v ? (double)v[0] : 0.0);
+          store_be<F>(d + sizeof(F), v ? (double)v[1] : 0.0);
+        }
+      }
+      if (!put_record(f, rec.data(), rec.size())) return false;
+    }
+  }
+  return true;
+}
+
+template <typename S, typename F>
+void read_rows(const unsigned char* s, int part, int z, S* cyl, const std::vector<int>& colof, int n) {
+  const int nxh = n / 2, r0 = part == 0 ? 0 : n / 2, nr = part == 0 ? n / 2 : n / 2 - 1;
+  for (int r = 0; r < nr; r++) {
+    const int y = plane_row(r0 + r, n);
+    for (int x = 0; x < nxh; x++, s += 2 * sizeof(F)) {
+      const int a = colof[(size_t)x * n + z];
+      if (a < 0) continue;  // off the cylinder: not held (zero in a file this library writes)
+      S* v = cyl + 2 * ((size_t)a * n + y);
+      v[0] = (S)load_be<F>(s);
+      v[1] = (S)load_be<F>(s + sizeof(F));
+    }
+  }
+}
+
+std::vector<int> column_index(const psdns_ctx* c) {
+  std::vector<int> colof((size_t)c->nxh * c->n, -1);
+  for (int a = 0; a < c->na; a++) colof[(size_t)c->colx[a] * c->n + c->colz[a]] = a;
+  return colof;
+}
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) fclose(f);
+  }
+};
+
+// u = un for the velocity and the scalar stage copy (main.f:357-359), as
+// after put_state
+int state_to_stage(psdns_ctx* c) {
+  const size_t vb = c->csize() * c->cyl_elems();
+  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, vb * 3, hipMemcpyDeviceToDevice, c->c.stream));
+  if (c->nc > 0)
+    HIPCHK(hipMemcpyAsync(c->d_usc, (char*)c->d_un + 3 * vb, vb * c->nc, hipMemcpyDeviceToDevice, c->c.stream));
+  c->last_nonlinear = 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psdns_outpen(psdns_ctx* c, const char* path, int file_prec) {
+  if (int rc = ready(c)) return rc;
+  if (!path) return fail(PSDNS_EINVAL, "null path");
+  if (file_prec != 4 && file_prec != 8) return fail(PSDNS_EINVAL, "file_prec = %d: 4 or 8", file_prec);
+  FileCloser fc{fopen(path, "wb")};
+  if (!fc.f) return fail(PSDNS_EIO, "%s: %s", path, strerror(errno));
+  const int n = c->n;
+  const size_t vb = c->csize() * c->cyl_elems();
+  const std::vector<int> colof = column_index(c);
+  std::vector<unsigned char> cyl(vb), rec;
+  for (int i = 0; i < 3 + c->nc; i++) {
+    HIPCHK(hipMemcpyAsync(cyl.data(), (char*)c->d_un + i * vb, vb, hipMemcpyDeviceToHost, c->c.stream));
+    if (int rc = sync(c)) return rc;
+    // outpen_write.f:70-71: variable index and grid, then this task's pencil
+    bool ok = put_ints(fc.f, {i + 1, n, n, n}) && put_ints(fc.f, {1, n / 2, 1, n, 1, n});
+    if (ok) {
+      if (c->prec == 8)
+        ok = file_prec == 8 ? write_var<double, double>(fc.f, (const double*)cyl.data(), colof, n, rec)
+                            : write_var<double, float>(fc.f, (const double*)cyl.data(), colof, n, rec);
+      else
+        ok = file_prec == 8 ? write_var<float, double>(fc.f, (const float*)cyl.data(), colof, n, rec)
+                            : write_var<float, float>(fc.f, (const float*)cyl.data(), colof, n, rec);
+    }
+    if (!ok) return fail(PSDNS_EIO, "%s: write of variable %d failed: %s", path, i + 1, strerror(errno));
+  }
+  FILE* f = fc.f;
+  fc.f = nullptr;
+  if (fclose(f) != 0) return fail(PSDNS_EIO, "%s: close failed: %s", path, strerror(errno));
+  return PSDNS_OK;
+}
+
+int psdns_inpen(psdns_ctx* c, const char* path) {
+  if (int rc = ready(c)) return rc;
+  if (!path) return fail(PSDNS_EINVAL, "null path");
+  FileCloser fc{fopen(path, "rb")};
+  if (!fc.f) return fail(PSDNS_EIO, "%s: %s", path, strerror(errno));
+  const int n = c->n, nxh = n / 2;
+  const size_t vb = c->csize() * c->cyl_elems();
+  const std::vector<int> colof = column_index(c);
+  std::vector<unsigned char> cyl(vb), rec;
+  int fprec = 0;
+  for (int i = 0; i < 3 + c->nc; i++) {
+    long len = get_record(fc.f, rec);
+    if (len < 0) return fail(PSDNS_EIO, "%s: variable %d: header record unreadable", path, i + 1);
+    const int want1[4] = {i + 1, n, n, n};
+    if (len != 16) return fail(PSDNS_EINVAL, "%s: variable %d: header record of %ld bytes", path, i + 1, len);
+    for (int k = 0; k < 4; k++)
+      if ((int)__builtin_bswap32(((const uint32_t*)rec.data())[k]) != want1[k])
+        return fail(PSDNS_EINVAL, "%s: variable %d: header (i, nx, ny, nz) is not (%d, %d, %d, %d)", path, i + 1,
+                    i + 1, n, n, n);
+    len = get_record(fc.f, rec);
+    if (len < 0) return fail(PSDNS_EIO, "%s: variable %d: pencil record unreadable", path, i + 1);
+    const int want2[6] = {1, nxh, 1, n, 1, n};
+    if (len != 24) return fail(PSDNS_EINVAL, "%s: variable %d: pencil record of %ld bytes", path, i + 1, len);
+    for (int k = 0; k < 6; k++)
+      if ((int)__builtin_bswap32(((const uint32_t*)rec.data())[k]) != want2[k])
+        return fail(PSDNS_EINVAL, "%s: variable %d: pencil (xist, xisz, zjst, zjsz, 1, ny) is not one task's",
+                    path, i + 1);
+    std::fill(cyl.begin(), cyl.end(), (unsigned char)0);
+    for (int z = 0; z < n; z++) {
+      if (z == n / 2) continue;
+      for (int part = 0; part < 2; part++) {
+        const size_t cells = (size_t)nxh * (part == 0 ? n / 2 : n / 2 - 1);
+        len = get_record(fc.f, rec);
+        if (len < 0) return fail(PSDNS_EIO, "%s: variable %d, z = %d: record unreadable", path, i + 1, z + 1);
+        if (!fprec) fprec = len == (long)(cells * 16) ? 8 : len == (long)(cells * 8) ? 4 : -1;
+        if (fprec < 0 || len != (long)(cells * 2 * fprec))
+          return fail(PSDNS_EINVAL, "%s: variable %d, z = %d: record of %ld bytes for %zu complex values", path,
+                      i + 1, z + 1, len, cells);
+        if (c->prec == 8) {
+          if (fprec == 8) read_rows<double, double>(rec.data(), part, z, (double*)cyl.data(), colof, n);
+          else read_rows<double, float>(rec.data(), part, z, (double*)cyl.data(), colof, n);
+        } else {
+          if (fprec == 8) read_rows<float, double>(rec.data(), part, z, (float*)cyl.data(), colof, n);
+          else read_rows<float, float>(rec.data(), part, z, (float*)cyl.data(), colof, n);
+        }
+      }
+    }
+    HIPCHK(hipMemcpyAsync((char*)c->d_un + i * vb, cyl.data(), vb, hipMemcpyHostToDevice, c->c.stream));
+    if (int rc = sync(c)) return rc;
+  }
+  if (int rc = state_to_stage(c)) return rc;
   return sync(c);
 }
 

@@ -554,6 +554,17 @@ struct XPass {
           0, 0, 0);
     }
   }
+  // Makes the staged variable visible to the whole work-group.  The copies
+  // into S complete asynchronously and count in vmcnt; the waits the
+  // compiler inserts order them only before the same wave's own reads of S,
+  // and those come after the barrier.  Every wave reads rows that other
+  // waves copied, so each wave waits for its own copies first, then the
+  // barrier: otherwise a wave can pass it and read rows still holding the
+  // previous variable (the rows copied last, highest kx, are the ones late).
+  static __device__ __forceinline__ void stage_publish() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
   // c2r of the staged variable: c2r_staged_pre forms Z[k] from S[k] and
   // conj S[m-k] (transposed mapping: 8 lanes read one 64-byte row) and ends
   // with a barrier, after which S may be refilled; c2r_staged_fft then
@@ -737,17 +748,15 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
     for (;;) {
       const int y = t / NZB, z0 = (t % NZB) * B;
       C<R> pu[E], pv[E], pw[E];
-      // the staged variable must be in LDS (its copy's wait is placed by the
-      // compiler, which tracks the LDS writes of the copy) for every wave
-      __syncthreads();
+      X::stage_publish();
       X::c2r_staged_pre(pu, S, xtw);
       X::stage_load(S, xi, 1, y, z0, g, true);
       X::c2r_staged_fft(pu, lds, stw);
-      __syncthreads();
+      X::stage_publish();
       X::c2r_staged_pre(pv, S, xtw);
       X::stage_load(S, xi, 2, y, z0, g, true);
       X::c2r_staged_fft(pv, lds, stw);
-      __syncthreads();
+      X::stage_publish();
       X::c2r_staged_pre(pw, S, xtw);
       const int tn = t + w.step;
       const bool more = tn < w.end;
@@ -807,29 +816,29 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
     }
   };
   X::stage_load(S, xv, 0, y, z0, g, true);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pa, S, xtw);  // u
   X::stage_load(S, xg, 0, y, z0, g, true);
   X::c2r_staged_fft(pa, lds, stw);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pb, S, xtw);  // phi_x
   X::stage_load(S, xv, 1, y, z0, g, true);
   X::c2r_staged_fft(pb, lds, stw);
   add((R)g0, true);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pa, S, xtw);  // v
   X::stage_load(S, xg, 1, y, z0, g, true);
   X::c2r_staged_fft(pa, lds, stw);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pb, S, xtw);  // phi_y
   X::stage_load(S, xv, 2, y, z0, g, true);
   X::c2r_staged_fft(pb, lds, stw);
   add((R)g1, false);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pa, S, xtw);  // w
   X::stage_load(S, xg, 2, y, z0, g, true);
   X::c2r_staged_fft(pa, lds, stw);
-  __syncthreads();
+  X::stage_publish();
   X::c2r_staged_pre(pb, S, xtw);  // phi_z
   X::c2r_staged_fft(pb, lds, stw);
   add((R)g2, false);

@@ -9,7 +9,10 @@
 ! usage: psdns_driver key=value ...
 !   n=64 prec=8 nsteps=20 cfl=3.14159265 viscos=0.01 kshift=1 kf_shell=3
 !   seed=122929298 ic=<file> out=<file> shifts=<file> log=<file>
+!   inpen=<file> outpen=<file>
 ! ic/out: raw uny(ny,num_al,3) complex at precision prec (stream access)
+! inpen/outpen: pencil checkpoint files of the reference (incond.f:120-133
+!   reads one with inpen, main.f:604-608 writes un with outpen at the end)
 ! shifts: raw gsh(3,2) per step, real(8), overriding the kshift rule
 program psdns_driver
   use iso_c_binding
@@ -21,6 +24,7 @@ program psdns_driver
   real(c_double) :: cfl = 3.14159265358979d0, viscos = 0.01d0, velmax, velmax2, dt, time
   real(c_double) :: gsh(3, 2), tke, eps, r3(3), stimes(6)
   character(len=512) :: arg, key, val, ic_file = '', out_file = '', shift_file = '', log_file = 'log'
+  character(len=512) :: inpen_file = '', outpen_file = ''
   type(psdns_params) :: p
   type(c_ptr) :: ctx
   integer(c_int) :: num_al, max_al_x, kfor
@@ -53,6 +57,8 @@ program psdns_driver
     case ('out'); out_file = val
     case ('shifts'); shift_file = val
     case ('log'); log_file = val
+    case ('inpen'); inpen_file = val
+    case ('outpen'); outpen_file = val
     end select
   end do
 
@@ -63,8 +69,11 @@ program psdns_driver
   write (6, '(a,i0,a,i0,a,i0,a,i0)') 'psdns_driver: n=', n, ' prec=', prec, ' num_al=', num_al, &
       ' max_al_x=', max_al_x
 
-  ! initial condition: file (reference cylinder layout) or the device generator
-  if (len_trim(ic_file) > 0) then
+  ! initial condition: pencil checkpoint, file (reference cylinder layout)
+  ! or the device generator
+  if (len_trim(inpen_file) > 0) then
+    call psdns_check(psdns_inpen(ctx, trim(inpen_file)//c_null_char), 'psdns_inpen')
+  else if (len_trim(ic_file) > 0) then
     if (prec == 8) then
       allocate (u8(n, num_al, 3))
       open (11, file=trim(ic_file), access='stream', form='unformatted', status='old')
@@ -193,6 +202,8 @@ program psdns_driver
     end if
     close (13)
   end if
+  if (len_trim(outpen_file) > 0) &
+    call psdns_check(psdns_outpen(ctx, trim(outpen_file)//c_null_char, 8), 'psdns_outpen')
   call psdns_check(psdns_destroy(ctx), 'psdns_destroy')
   first = .true.
 end program psdns_driver

@@ -102,6 +102,21 @@ module psdns_gpu
       real(c_double), intent(out) :: dudx(8), dwdz(8)
     end function psdns_vg_moments
 
+    ! pencil checkpoint of un (homogeneous/outpen_write.f:59-76, inpen_read.f);
+    ! path is a NUL-terminated string: trim(name)//c_null_char
+    integer(c_int) function psdns_outpen(ctx, path, file_prec) bind(C, name="psdns_outpen")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: file_prec
+    end function psdns_outpen
+
+    integer(c_int) function psdns_inpen(ctx, path) bind(C, name="psdns_inpen")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function psdns_inpen
+
     integer(c_int) function psdns_init_field(ctx, seed, kpeak, tke) bind(C, name="psdns_init_field")
       import :: c_int, c_ptr, c_double, c_long_long
       type(c_ptr), value :: ctx

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import kgrid, rel
+from conftest import REPO, kgrid, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -259,11 +259,38 @@ def test_fortran_driver_matches_oracle(O, P, tmp_path):
     np.ascontiguousarray(shifts, np.float64).tofile(tmp_path / "shifts.bin")
     cmd = [str(P.DRIVER_PATH), f"n={n}", "prec=8", f"nsteps={nsteps}", "cfl=3.14159265358979",
            "viscos=0.01", "kshift=1", "kf_shell=3", f"ic={tmp_path / 'ic.bin'}",
-           f"shifts={tmp_path / 'shifts.bin'}", f"out={tmp_path / 'out.bin'}", f"log={tmp_path / 'log'}"]
+           f"shifts={tmp_path / 'shifts.bin'}", f"out={tmp_path / 'out.bin'}", f"log={tmp_path / 'log'}",
+           f"outpen={tmp_path / 'outpenc.p0'}"]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     out = np.fromfile(tmp_path / "out.bin", np.complex128).reshape(3, g.num_al, n)
     assert rel(out, ro2.uny, g.mask()[None]) < 1e-11
+    # the checkpoint the driver wrote at the end (main.f:604-608) holds the same state
+    sys.path.insert(0, str(REPO / "oracle"))
+    import pencil
+    chk, fprec = pencil.read_inpen(tmp_path / "outpenc.p0", n, g.col_x, g.col_z, 3)
+    assert fprec == 8 and np.array_equal(chk[:, pencil.held(n, g.col_z)], out[:, pencil.held(n, g.col_z)])
     log = (tmp_path / "log").read_text().splitlines()
     assert len([l for l in log if l.startswith("cpu at step")]) == nsteps
+
+
+@pytest.mark.parametrize("n,prec", [(32, 8), (256, 8), (256, 4)])
+def test_fused_step_reproducible(P, O, n, prec):
+    """The same state stepped twice on one context gives the same bits.
+    The x pass stages each variable into LDS with asynchronous copies that
+    several waves read, so every wave waits for its own copies before the
+    publishing barrier; a missing wait shows up here as run-to-run
+    differences in the highest kx modes."""
+    print("[gpu case] reproducible", n, prec, flush=True)
+    g = O.Grid(n)
+    u0 = g.field(seed=9)
+    gsh = [0.1, 0.2, 0.3, 0.6, 0.7, 0.8]
+    s = P.Solver(n, prec=prec, viscos=0.01)
+    out = []
+    for _ in range(3):
+        s.put_state(u0)
+        P.Run(s, 0.01, np.pi).step(gsh)
+        out.append(s.get_state())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[2])
+    s.close()
