@@ -116,6 +116,23 @@ int psdns_stage_nonlinear(psdns_ctx *c, int kstep, double *velmax);
 int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
                        const double *yfac, const double *zfac, const void *forcing);
 
+/* RK4 (rkmethod = 4, timestep_module.f:293-310), velocity only (nc = 0):
+ * per substep kstep = 1..4, stage_nonlinear(kstep) (stages 3 and 4 use the
+ * shift offsets of stages 1 and 2, shifts.f:133-137; velmax on kstep 1) and
+ * then rk4_update(kstep), which replaces the y part of transform +
+ * wavespace_vel + advanc_kstep (homogeneous/advanc_1.f .. advanc_4.f,
+ * non-EPFOR, modpress = 1).  With N the nonlinear term (dt/2 attached), un
+ * the state at the start of the step and u1 an accumulator the context
+ * allocates on first use (zeroed, comp_set.f:72-74):
+ *   1: u1 = F (un + N/3), u = P F (un + N), F = xfac yfac zfac (full dt)
+ *   2: u1 += 2N/3, u = P (un + N)
+ *   3: u1 += 2N/3, u = P (un + 2N)
+ *   4: u1 = P (u1 + N/3); un = u = u1; zero mean
+ * on the mask, P the projection with k = 0 excluded.  The factors are read
+ * on kstep 1 only (NULL allowed after). */
+int psdns_rk4_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
+                     const double *yfac, const double *zfac);
+
 /* FEK forcing (ek_force.f:1-76 with ek_shell.f:81-176): shell energies of
  * un for ik = int(|k| + 1.5) <= kfor, then u and un scaled by
  * sqrt(ekinf(ik)/ek(ik)) for ik <= kf_shell (ratio 0 for ik = 1).

@@ -30,7 +30,8 @@ EXPORTS = (
     "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_init_field", "psdns_put_field",
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
-    "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_last_error",
+    "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_rk4_update",
+    "psdns_last_error",
 )
 
 
@@ -99,6 +100,7 @@ def load_library():
         "psdns_stage_times": [_vp, i, _dp],
         "psdns_outpen": [_vp, ctypes.c_char_p, i],
         "psdns_inpen": [_vp, ctypes.c_char_p],
+        "psdns_rk4_update": [_vp, i, d, _dp, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -215,6 +217,12 @@ class Solver:
             xf, yf, zf = (np.ascontiguousarray(f, np.float64) for f in (xfac, yfac, zfac))
             self._chk(self.L.psdns_stage_update(self.h, kstep, dt, _dptr(xf), _dptr(yf), _dptr(zf), None))
 
+    def rk4_update(self, kstep, dt, xfac=None, yfac=None, zfac=None):
+        """advanc_kstep (homogeneous/advanc_1.f .. advanc_4.f) after
+        stage_nonlinear(kstep); factors on kstep 1 only."""
+        args = [None if a is None else _dptr(np.ascontiguousarray(a, np.float64)) for a in (xfac, yfac, zfac)]
+        self._chk(self.L.psdns_rk4_update(self.h, kstep, dt, *args))
+
     def step_factors(self, dt):
         """step.f:366-385 integrating factors (host side, as the kept driver)."""
         n = self.n
@@ -283,6 +291,36 @@ class Solver:
 
     def xktran(self, nv):
         self._chk(self.L.psdns_xktran(self.h, nv))
+
+
+class Run4:
+    """RK4 time loop (rkmethod = 4, timestep_module.f:293-310): four
+    substeps, dt and the integrating factors from substep 1, FEK forcing
+    after the step.  Shift offsets gsh(3,2) from the caller serve stages
+    1, 2 and again 3, 4 (shifts.f:133-137)."""
+
+    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None):
+        self.s, self.viscos, self.cfl = solver, viscos, cfl
+        self.kf_shell = kf_shell
+        self.kfor = kf_shell + 1 if kf_shell > 0 else 0
+        self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
+        self.dts = []
+
+    def step(self, gsh):
+        s = self.s
+        s.set_shifts(gsh)
+        for k in range(1, 5):
+            vm = s.stage_nonlinear(k)
+            if k == 1:
+                self.velmax = vm
+                self.dt = self.cfl / vm
+                xf, yf, zf = s.step_factors(self.dt)
+                s.rk4_update(1, self.dt, xf, yf, zf)
+            else:
+                s.rk4_update(k, self.dt)
+        if self.kfor:
+            s.ek_force(self.kf_shell, self.kfor, self.ekinf)
+        self.dts.append(self.dt)
 
 
 class Run:

@@ -55,6 +55,8 @@ struct psdns_ctx {
   int nc = 0;
   double pr[8] = {}, grad[8][3] = {};
   void *d_usc = nullptr, *d_sc = nullptr;
+  // RK4 accumulator u1(ny, num_al, 3), allocated on the first RK4 substep
+  void* d_u1 = nullptr;
   std::vector<int> colx, colz, cutz;
   // device allocations
   int *d_colx = nullptr, *d_colz = nullptr, *d_cutz = nullptr, *d_colz0 = nullptr, *d_col0 = nullptr;
@@ -425,7 +427,7 @@ int psdns_destroy(psdns_ctx* c) {
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
-                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc};
+                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc, c->d_u1};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_fac) (void)hipHostFree(c->h_fac);
@@ -484,7 +486,8 @@ int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
 
 int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if (int rc = check_ctx(c)) return rc;
-  if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
+  if (kstep < 1 || kstep > 4) return fail(PSDNS_EINVAL, "kstep = %d: stages 1..2 (RK2) or 1..4 (RK4)", kstep);
+  if (kstep > 2 && c->nc > 0) return fail(PSDNS_EINVAL, "RK4 with passive scalars is not supported");
   int rc;
   // stage 2 right after stage_update(1): that update and this stage's phase
   // shift and inverse y pass are one kernel, whose yw the z pass reads from u
@@ -495,7 +498,9 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   } else if ((rc = flush(c))) {
     return rc;
   }
-  c->c.kstep = kstep - 1;
+  // RK4's stages 3 and 4 use the shifts of stages 1 and 2 (shifts.f:133-137:
+  // gsh(:,3) = gsh(:,1), gsh(:,4) = gsh(:,2))
+  c->c.kstep = (kstep - 1) % 2;
   if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
   if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) return rc;
@@ -542,6 +547,37 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
   }
   if ((rc = y_update(c, kstep, norm, false))) return rc;
   if (c->nc > 0 && (rc = scalar_update(c, kstep, dt2))) return rc;
+  if ((rc = sync(c))) return rc;
+  return add_time(c, 5, 6, 4);
+}
+
+int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, const double* yfac,
+                     const double* zfac) {
+  if (int rc = ready(c)) return rc;
+  if (kstep < 1 || kstep > 4) return fail(PSDNS_EINVAL, "kstep = %d: RK4 has stages 1..4", kstep);
+  if (c->nc > 0) return fail(PSDNS_EINVAL, "RK4 with passive scalars is not supported");
+  if (c->last_nonlinear != kstep)
+    return fail(PSDNS_ESTATE, "rk4_update(%d) without a preceding stage_nonlinear(%d)", kstep, kstep);
+  if (!(dt > 0.0)) return fail(PSDNS_EINVAL, "dt = %g must be positive", dt);
+  int rc;
+  if (!c->d_u1) {
+    const size_t bytes = 3 * c->csize() * c->cyl_elems();
+    if (hipMalloc(&c->d_u1, bytes) != hipSuccess)
+      return fail(PSDNS_ENOMEM, "hipMalloc of the RK4 accumulator (%zu bytes) failed", bytes);
+    HIPCHK(hipMemsetAsync(c->d_u1, 0, bytes, c->c.stream));  // comp_set.f:74
+  }
+  // advanc_1 is the only substep with integrating factors
+  if (kstep == 1 && (rc = upload_factors(c, xfac, yfac, zfac))) return rc;
+  c->last_nonlinear = 0;
+  c->c.kstep = (kstep - 1) % 2;
+  if ((rc = mark(c, 5))) return rc;
+  // the y part of transform (W1 -> u slots 0..3), then wavespace_vel with
+  // dt/2 attached (wavespace_vel.f:80) and advanc_kstep in one pass; W1 from
+  // the fused x pass carries its unnormalized r2c (x 2n), taken out here
+  if ((rc = pass(c, Pass::YFwd, 4))) return rc;
+  psdns::launch_wavespace_advanc(c->c, dt / 2.0 / c->n / c->n / (2.0 * c->n), kstep, c->d_u1);
+  HIPCHK(hipGetLastError());
+  if ((rc = mark(c, 6))) return rc;
   if ((rc = sync(c))) return rc;
   return add_time(c, 5, 6, 4);
 }

@@ -1215,6 +1215,74 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// RK4 substeps (rkmethod = 4): wavespace_vel, then advanc_K with N the
+// nonlinear term (dt/2 attached), un the state at the start of the step and
+// u1 the accumulator, on the mask only:
+//   K = 1 (advanc_1.f:88-99): F = xfac yfac zfac; u1 = F (un + N/3);
+//         u = F (un + N); project u
+//   K = 2 (advanc_2.f, loop over i): u1 += N 2/3; u = un + N; project u
+//   K = 3 (advanc_3.f): u1 += N 2/3; u = un + 2 N; project u
+//   K = 4 (advanc_4.f:46-83,149-176): u1 += N/3; project u1; un = u = u1;
+//         zero mean
+// The projection skips k = 0 (advanc_2.f: y = z = x = 1), as in proc4a.
+template <typename R, int K>
+__global__ void __launch_bounds__(256)
+    k_wavespace_advanc(C<R>* __restrict__ u, C<R>* __restrict__ un, C<R>* __restrict__ u1, Geo g, double norm,
+                       const double2* __restrict__ sx, const double2* __restrict__ sy,
+                       const double2* __restrict__ sz, const double* __restrict__ xfac,
+                       const double* __restrict__ yfac, const double* __restrict__ zfac) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    if (!p.m) {
+      // wavespace zeroes u off the mask; advanc_K writes the mask only
+#pragma unroll
+      for (int v = 0; v < 3; v++) u[v * tot + i] = {0, 0};
+      continue;
+    }
+    C<R> nl[4] = {u[i], u[tot + i], u[2 * tot + i], u[3 * tot + i]};
+    wavespace_point<R>(p, nl, nl + 1, nl + 2, nl + 3, nullptr, norm, sx, sy, sz);
+    C<R> a[3] = {u1[i], u1[tot + i], u1[2 * tot + i]};
+    C<R> o[3];
+    if constexpr (K == 1) {
+      const R f = (R)(xfac[p.x] * yfac[p.y] * zfac[p.z]);
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        const C<R> b = un[v * tot + i];
+        a[v] = {f * (b.x + nl[v].x / (R)3), f * (b.y + nl[v].y / (R)3)};
+        o[v] = {f * (b.x + nl[v].x), f * (b.y + nl[v].y)};
+      }
+      project(o, p);
+    } else if constexpr (K == 2 || K == 3) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        const C<R> b = un[v * tot + i];
+        a[v] = {a[v].x + nl[v].x * (R)2 / (R)3, a[v].y + nl[v].y * (R)2 / (R)3};
+        o[v] = K == 2 ? C<R>{b.x + nl[v].x, b.y + nl[v].y} : C<R>{b.x + nl[v].x * (R)2, b.y + nl[v].y * (R)2};
+      }
+      project(o, p);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 3; v++) a[v] = {a[v].x + nl[v].x / (R)3, a[v].y + nl[v].y / (R)3};
+      project(a, p);
+      if (p.x == 0 && p.z == 0 && p.y == 0) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) a[v] = {0, 0};
+      }
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        o[v] = a[v];
+        un[v * tot + i] = a[v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      u[v * tot + i] = o[v];
+      u1[v * tot + i] = a[v];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2):
 // gathers P2, C1, P4, C3 from W (transposed mapping), y c2c -i each
@@ -1878,6 +1946,15 @@ struct LaunchP {
                          (CR*)c.u, (CR*)c.un, c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac,
                          c.yfac, c.zfac);
   }
+  static void wavespace_advanc(const Ctx& c, double norm, int k, void* u1) {
+    size_t tot = (size_t)c.geo.na * c.geo.n;
+    const auto kern = k == 1   ? k_wavespace_advanc<R, 1>
+                      : k == 2 ? k_wavespace_advanc<R, 2>
+                      : k == 3 ? k_wavespace_advanc<R, 3>
+                               : k_wavespace_advanc<R, 4>;
+    hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u, (CR*)c.un, (CR*)u1,
+                       c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac, c.yfac, c.zfac);
+  }
   static void ek_shell(const Ctx& c, int kfor, double* d_ek) {
     hipLaunchKernelGGL((k_ek_shell<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (const CR*)c.un,
                        c.geo, c.force_list, c.nforce, kfor, d_ek);
@@ -1992,6 +2069,10 @@ void launch_update(const Ctx& c, int stage) {
 void launch_wavespace_update(const Ctx& c, double norm, int stage) {
   if (c.prec == 8) LaunchP<double>::wavespace_update(c, norm, stage);
   else LaunchP<float>::wavespace_update(c, norm, stage);
+}
+void launch_wavespace_advanc(const Ctx& c, double norm, int k, void* u1) {
+  if (c.prec == 8) LaunchP<double>::wavespace_advanc(c, norm, k, u1);
+  else LaunchP<float>::wavespace_advanc(c, norm, k, u1);
 }
 void launch_ek_shell(const Ctx& c, int kfor, double* d_ek) {
   if (c.prec == 8) LaunchP<double>::ek_shell(c, kfor, d_ek);

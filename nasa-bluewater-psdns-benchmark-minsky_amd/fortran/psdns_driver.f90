@@ -9,7 +9,7 @@
 ! usage: psdns_driver key=value ...
 !   n=64 prec=8 nsteps=20 cfl=3.14159265 viscos=0.01 kshift=1 kf_shell=3
 !   seed=122929298 ic=<file> out=<file> shifts=<file> log=<file>
-!   inpen=<file> outpen=<file>
+!   inpen=<file> outpen=<file> rk=2|4
 ! ic/out: raw uny(ny,num_al,3) complex at precision prec (stream access)
 ! inpen/outpen: pencil checkpoint files of the reference (incond.f:120-133
 !   reads one with inpen, main.f:604-608 writes un with outpen at the end)
@@ -19,7 +19,7 @@ program psdns_driver
   use psdns_gpu
   implicit none
 
-  integer :: n = 64, prec = 8, nsteps = 10, kshift = 1, kf_shell = 0, istep, i, k, nargs
+  integer :: n = 64, prec = 8, nsteps = 10, kshift = 1, kf_shell = 0, rk = 2, istep, i, k, nargs
   integer(c_long_long) :: seed = 122929298_c_long_long
   real(c_double) :: cfl = 3.14159265358979d0, viscos = 0.01d0, velmax, velmax2, dt, time
   real(c_double) :: gsh(3, 2), tke, eps, r3(3), stimes(6)
@@ -59,6 +59,7 @@ program psdns_driver
     case ('log'); log_file = val
     case ('inpen'); inpen_file = val
     case ('outpen'); outpen_file = val
+    case ('rk'); read (val, *) rk
     end select
   end do
 
@@ -162,11 +163,20 @@ program psdns_driver
       yfac(i) = exp(-viscos * dble(k)**2 * dt)
       zfac(i) = yfac(i)
     end do
-    call psdns_check(psdns_stage_update(ctx, 1, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(1)')
+    if (rk == 4) then
+      ! rkmethod = 4 (timestep_module.f:293-310): advanc_1 .. advanc_4
+      call psdns_check(psdns_rk4_update(ctx, 1, dt, xfac, yfac, zfac), 'psdns_rk4_update(1)')
+      do k = 2, 4
+        call psdns_check(psdns_stage_nonlinear(ctx, k, velmax2), 'psdns_stage_nonlinear(rk4)')
+        call psdns_check(psdns_rk4_update(ctx, k, dt, xfac, yfac, zfac), 'psdns_rk4_update')
+      end do
+    else
+      call psdns_check(psdns_stage_update(ctx, 1, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(1)')
 
-    ! substep 2: corrector
-    call psdns_check(psdns_stage_nonlinear(ctx, 2, velmax2), 'psdns_stage_nonlinear(2)')
-    call psdns_check(psdns_stage_update(ctx, 2, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(2)')
+      ! substep 2: corrector
+      call psdns_check(psdns_stage_nonlinear(ctx, 2, velmax2), 'psdns_stage_nonlinear(2)')
+      call psdns_check(psdns_stage_update(ctx, 2, dt, xfac, yfac, zfac, c_null_ptr), 'psdns_stage_update(2)')
+    end if
     if (kfor > 0) call psdns_check(psdns_ek_force(ctx, kf_shell, kfor, ekinf, ekout), 'psdns_ek_force')
     time = time + dt
 

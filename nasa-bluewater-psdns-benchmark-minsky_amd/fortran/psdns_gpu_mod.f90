@@ -102,6 +102,16 @@ module psdns_gpu
       real(c_double), intent(out) :: dudx(8), dwdz(8)
     end function psdns_vg_moments
 
+    ! RK4 substep kstep = 1..4 after psdns_stage_nonlinear(ctx, kstep, ...)
+    ! (homogeneous/advanc_1.f .. advanc_4.f); factors read on kstep 1 only
+    integer(c_int) function psdns_rk4_update(ctx, kstep, dt, xfac, yfac, zfac) bind(C, name="psdns_rk4_update")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kstep
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: xfac(*), yfac(*), zfac(*)
+    end function psdns_rk4_update
+
     ! pencil checkpoint of un (homogeneous/outpen_write.f:59-76, inpen_read.f);
     ! path is a NUL-terminated string: trim(name)//c_null_char
     integer(c_int) function psdns_outpen(ctx, path, file_prec) bind(C, name="psdns_outpen")

@@ -270,6 +270,49 @@ def scalar_update(g, phi_u, phi_n, terms, kstep, dt, gsh, factors=None, pr=None)
             phi_n[i, 0, 0] = 0
 
 
+def advanc(g, k, uy, uny, u1, factors=None):
+    """RK4 substep k = 1..4 (rkmethod = 4) on the cylinder, in place on
+    uy[:3] (the nonlinear term N on entry, dt/2 attached), uny and u1, on
+    the mask only.  Restates PSDNS/homogeneous/advanc_1.f:88-153 (the
+    integrating factor xfac(x) yfac(y) zfac(z) and the modpress projection,
+    k = 0 skipped), advanc_2.f and advanc_3.f (u1 += N 2/3; u = un + N, resp.
+    un + 2 N; projection) and advanc_4.f:46-83,149-176 (u1 += N/3; projection
+    of u1; un = u = u1 on the mask; zero mean).  EPFOR forcing is not part of
+    it.  factors = (xfac, yfac, zfac) for k = 1."""
+    n = g.n
+    m = g.mask()
+    kx = g.col_x.astype(np.float64)[:, None]
+    kz = np.where(g.col_z > n // 2, g.col_z - n, g.col_z).astype(np.float64)[:, None]
+    y = np.arange(n)
+    ky = np.where(y > n // 2, y - n, y).astype(np.float64)[None, :]
+    k2 = kx * kx + ky * ky + kz * kz
+    k0 = k2 == 0
+    kk = (np.broadcast_to(kx, k2.shape), np.broadcast_to(ky, k2.shape), np.broadcast_to(kz, k2.shape))
+
+    def project(v):
+        psi = -(kk[0] * v[0] + kk[1] * v[1] + kk[2] * v[2]) / np.where(k0, 1.0, k2)
+        psi[k0] = 0
+        return np.array([v[i] + kk[i] * psi for i in range(3)])
+
+    nl = uy[:3]
+    if k == 1:
+        xf, yf, zf = factors
+        f = xf[g.col_x][:, None] * yf[None, :] * zf[g.col_z][:, None]
+        a = f * (uny + nl / 3.0)
+        u = project(f * (uny + nl))
+    elif k in (2, 3):
+        a = u1 + nl * 2.0 / 3.0
+        u = project(uny + nl * (1.0 if k == 2 else 2.0))
+    else:
+        a = project(u1 + nl * 1.0 / 3.0)
+        a[:, k0] = 0
+        u = a
+    u1[:, m] = a[:, m]
+    uy[:3][:, m] = u[:, m]
+    if k == 4:
+        uny[:, m] = a[:, m]
+
+
 def fft_c2c(x, sign):
     a = np.ascontiguousarray(x, np.complex128).copy()
     lib().orc_fft_c2c(_p(a), a.shape[-1], a.size // a.shape[-1], sign)
@@ -354,6 +397,36 @@ class Run:
                 g.proc4b(self.uy, self.uny)
             if self.phi_n is not None:
                 scalar_update(g, self.phi_u, self.phi_n, terms, k, self.dt, gsh_now, self.factors, self.pr)
+        if self.kfor:
+            g.ek_force(self.uy, self.uny, self.kf_shell, self.kfor, self.ekinf)
+        self.istep += 1
+        self.time += self.dt
+        self.dts.append(self.dt)
+        return self.uny
+
+    def step_rk4(self, gsh=None):
+        """RK4 step (rkmethod = 4, PSDNS/timestep_module.f:293-310): four
+        substeps through the same nonlinear term, stages 3 and 4 with the
+        shifts of stages 1 and 2 (PSDNS/shifts.f:133-137), dt and the
+        integrating factors from substep 1, advanc_1..4, then FEK forcing.
+        Velocity only."""
+        g = self.g
+        g.set_shifts(self.next_shifts() if gsh is None else gsh)
+        if not hasattr(self, "u1"):
+            self.u1 = g.zeros_cyl(3)  # comp_set.f:72-74
+        for k in (1, 2, 3, 4):
+            ks = 1 if k in (1, 3) else 2
+            g.phshift_inplace(self.uy, 3, ks)
+            ux = g.itransform_vel(self.uy)
+            vm = g.realspace_vel(ux, 1 if k == 1 else 2)
+            nl = g.transform(ux)
+            if k == 1:
+                self.velmax = vm
+                self.dt = self.cfl / vm if self.dt_fixed is None else self.dt_fixed
+                self.factors = g.step_factors(self.viscos, self.dt)
+            g.wavespace_vel(nl, ks, self.dt / 2.0)
+            self.uy = nl
+            advanc(g, k, self.uy, self.uny, self.u1, self.factors)
         if self.kfor:
             g.ek_force(self.uy, self.uny, self.kf_shell, self.kfor, self.ekinf)
         self.istep += 1

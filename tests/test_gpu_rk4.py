@@ -1,0 +1,82 @@
+"""RK4 (rkmethod = 4) through the C ABI against the FP64 oracle: four
+substeps of stage_nonlinear + rk4_update (homogeneous/advanc_1.f ..
+advanc_4.f, stages 3 and 4 with the shifts of stages 1 and 2) from the same
+seeded field and shift offsets, with and without FEK forcing."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = {8: 1e-11, 4: 5e-6}
+
+
+@pytest.mark.parametrize("n,prec,kf_shell,nsteps", [(32, 8, 0, 3), (64, 8, 3, 3), (64, 4, 3, 3), (128, 4, 0, 2)])
+def test_rk4_steps_match_oracle(P, O, n, prec, kf_shell, nsteps):
+    print("[gpu case] rk4", n, prec, kf_shell, nsteps, flush=True)
+    g = O.Grid(n)
+    u0 = g.field(seed=7)
+    nu = 0.02
+    ro = O.Run(g, u0, nu, cfl=np.pi, kshift=1, kf_shell=kf_shell, shift_seed=3)
+    shifts = ro.shift_sequence(nsteps)
+    ro = O.Run(g, u0, nu, cfl=np.pi, kshift=1, kf_shell=kf_shell, shift_seed=3)
+    s = P.Solver(n, prec=prec, viscos=nu)
+    s.put_state(u0)
+    rg = P.Run4(s, nu, np.pi, kf_shell=kf_shell, ekinf=ro.ekinf)
+    m = g.mask()[None]
+    for k in range(nsteps):
+        ro.step_rk4(shifts[k])
+        rg.step(shifts[k])
+        assert abs(rg.dt - ro.dt) <= 1e-6 * ro.dt
+    assert rel(s.get_state(), ro.uny, m) < TOL[prec]
+    # u = un after the step (advanc_4.f:149-163)
+    assert rel(s.get_field(P.FIELD_U, 3), ro.uny, m) < TOL[prec]
+    s.close()
+
+
+def test_rk4_call_order(P, O):
+    print("[gpu case] rk4 order", flush=True)
+    g = O.Grid(32)
+    s = P.Solver(32, prec=8)
+    s.put_state(g.field(seed=1))
+    s.set_shifts([0.0] * 6)
+    with pytest.raises(P.PsdnsError, match="error -4"):
+        s.rk4_update(1, 0.01, *s.step_factors(0.01))
+    s.stage_nonlinear(1)
+    with pytest.raises(P.PsdnsError, match="error -4"):
+        s.rk4_update(2, 0.01)
+    with pytest.raises(P.PsdnsError, match="error -1"):
+        s.rk4_update(5, 0.01)
+    s.close()
+    sc = P.Solver(32, prec=8, nc=1)
+    with pytest.raises(P.PsdnsError, match="error -1"):
+        sc.stage_nonlinear(3)
+    sc.close()
+
+
+def test_fortran_driver_rk4(P, O, tmp_path):
+    """The Fortran driver with rk=4 (rkmethod = 4) through the ISO_C_BINDING
+    module, against the oracle's RK4 from the same field and shifts."""
+    import subprocess
+    print("[gpu case] driver rk4", flush=True)
+    if not P.DRIVER_PATH.exists():
+        pytest.skip("Fortran driver not built")
+    n, nsteps = 32, 3
+    g = O.Grid(n)
+    u0 = g.field(seed=5)
+    ro = O.Run(g, u0, 0.01, cfl=np.pi, kshift=1, kf_shell=3, shift_seed=5)
+    shifts = ro.shift_sequence(nsteps)
+    ro = O.Run(g, u0, 0.01, cfl=np.pi, kshift=1, kf_shell=3, shift_seed=5)
+    for k in range(nsteps):
+        ro.step_rk4(shifts[k])
+    np.ascontiguousarray(u0, np.complex128).tofile(tmp_path / "ic.bin")
+    np.ascontiguousarray(shifts, np.float64).tofile(tmp_path / "shifts.bin")
+    cmd = [str(P.DRIVER_PATH), f"n={n}", "prec=8", f"nsteps={nsteps}", "cfl=3.14159265358979", "viscos=0.01",
+           "kshift=1", "kf_shell=3", "rk=4", f"ic={tmp_path / 'ic.bin'}", f"shifts={tmp_path / 'shifts.bin'}",
+           f"out={tmp_path / 'out.bin'}", f"log={tmp_path / 'log'}"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = np.fromfile(tmp_path / "out.bin", np.complex128).reshape(3, g.num_al, n)
+    assert rel(out, ro.uny, g.mask()[None]) < 1e-11
