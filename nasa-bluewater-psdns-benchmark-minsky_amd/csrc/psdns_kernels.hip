@@ -1242,7 +1242,12 @@ __global__ void __launch_bounds__(256)
     }
     C<R> nl[4] = {u[i], u[tot + i], u[2 * tot + i], u[3 * tot + i]};
     wavespace_point<R>(p, nl, nl + 1, nl + 2, nl + 3, nullptr, norm, sx, sy, sz);
-    C<R> a[3] = {u1[i], u1[tot + i], u1[2 * tot + i]};
+    // substep 1 sets u1, the others accumulate into it
+    C<R> a[3];
+    if constexpr (K != 1) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) a[v] = u1[v * tot + i];
+    }
     C<R> o[3];
     if constexpr (K == 1) {
       const R f = (R)(xfac[p.x] * yfac[p.y] * zfac[p.z]);
