@@ -181,9 +181,9 @@ def main():
     per_step = {nm: 2 for nm in names}
     per_step["y_inverse"] = 1 if nc == 0 else 2
     if args.rk == 4:
-        # four substeps, none fused across substeps; the last timer holds
-        # the y forward pass plus the pointwise wavespace + advanc pass,
-        # whose bytes the y-update model does not describe
+        # four substeps, none fused across substeps; the last timer is the
+        # fused y pass with advanc_K, whose accumulator traffic the y-update
+        # byte model does not include
         per_step = {nm: 4 for nm in names}
     passes = {}
     for i, nm in enumerate(names):

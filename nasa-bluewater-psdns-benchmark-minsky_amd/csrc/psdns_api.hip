@@ -571,11 +571,10 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
   c->last_nonlinear = 0;
   c->c.kstep = (kstep - 1) % 2;
   if ((rc = mark(c, 5))) return rc;
-  // the y part of transform (W1 -> u slots 0..3), then wavespace_vel with
-  // dt/2 attached (wavespace_vel.f:80) and advanc_kstep in one pass; W1 from
-  // the fused x pass carries its unnormalized r2c (x 2n), taken out here
-  if ((rc = pass(c, Pass::YFwd, 4))) return rc;
-  psdns::launch_wavespace_advanc(c->c, dt / 2.0 / c->n / c->n / (2.0 * c->n), kstep, c->d_u1);
+  // the y part of transform, wavespace_vel with dt/2 attached
+  // (wavespace_vel.f:80) and advanc_kstep in one kernel, from W1
+  if (psdns::launch_y_rk4(c->c, kstep, dt / 2.0 / c->n / c->n, c->d_u1) != 0)
+    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   HIPCHK(hipGetLastError());
   if ((rc = mark(c, 6))) return rc;
   if ((rc = sync(c))) return rc;

@@ -84,8 +84,8 @@ void launch_velmax_reduce(const Ctx& c);
 void launch_wavespace(const Ctx& c, double norm);
 void launch_update(const Ctx& c, int stage);
 void launch_wavespace_update(const Ctx& c, double norm, int stage);
-// RK4 substep k = 1..4: wavespace_vel + advanc_k on u (N), un and the accumulator u1
-void launch_wavespace_advanc(const Ctx& c, double norm, int k, void* u1);
+// RK4 substep k = 1..4 in the fused y forward kernel (W1 -> u, un, u1); -1 for an unsupported grid
+int launch_y_rk4(const Ctx& c, int k, double norm, void* u1);
 void launch_ek_shell(const Ctx& c, int kfor, double* d_ek);
 void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
 void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot);

@@ -1215,81 +1215,9 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// RK4 substeps (rkmethod = 4): wavespace_vel, then advanc_K with N the
-// nonlinear term (dt/2 attached), un the state at the start of the step and
-// u1 the accumulator, on the mask only:
-//   K = 1 (advanc_1.f:88-99): F = xfac yfac zfac; u1 = F (un + N/3);
-//         u = F (un + N); project u
-//   K = 2 (advanc_2.f, loop over i): u1 += N 2/3; u = un + N; project u
-//   K = 3 (advanc_3.f): u1 += N 2/3; u = un + 2 N; project u
-//   K = 4 (advanc_4.f:46-83,149-176): u1 += N/3; project u1; un = u = u1;
-//         zero mean
-// The projection skips k = 0 (advanc_2.f: y = z = x = 1), as in proc4a.
-template <typename R, int K>
-__global__ void __launch_bounds__(256)
-    k_wavespace_advanc(C<R>* __restrict__ u, C<R>* __restrict__ un, C<R>* __restrict__ u1, Geo g, double norm,
-                       const double2* __restrict__ sx, const double2* __restrict__ sy,
-                       const double2* __restrict__ sz, const double* __restrict__ xfac,
-                       const double* __restrict__ yfac, const double* __restrict__ zfac) {
-  const size_t tot = (size_t)g.na * g.n;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const Pt p = point(i, g);
-    if (!p.m) {
-      // wavespace zeroes u off the mask; advanc_K writes the mask only
-#pragma unroll
-      for (int v = 0; v < 3; v++) u[v * tot + i] = {0, 0};
-      continue;
-    }
-    C<R> nl[4] = {u[i], u[tot + i], u[2 * tot + i], u[3 * tot + i]};
-    wavespace_point<R>(p, nl, nl + 1, nl + 2, nl + 3, nullptr, norm, sx, sy, sz);
-    // substep 1 sets u1, the others accumulate into it
-    C<R> a[3];
-    if constexpr (K != 1) {
-#pragma unroll
-      for (int v = 0; v < 3; v++) a[v] = u1[v * tot + i];
-    }
-    C<R> o[3];
-    if constexpr (K == 1) {
-      const R f = (R)(xfac[p.x] * yfac[p.y] * zfac[p.z]);
-#pragma unroll
-      for (int v = 0; v < 3; v++) {
-        const C<R> b = un[v * tot + i];
-        a[v] = {f * (b.x + nl[v].x / (R)3), f * (b.y + nl[v].y / (R)3)};
-        o[v] = {f * (b.x + nl[v].x), f * (b.y + nl[v].y)};
-      }
-      project(o, p);
-    } else if constexpr (K == 2 || K == 3) {
-#pragma unroll
-      for (int v = 0; v < 3; v++) {
-        const C<R> b = un[v * tot + i];
-        a[v] = {a[v].x + nl[v].x * (R)2 / (R)3, a[v].y + nl[v].y * (R)2 / (R)3};
-        o[v] = K == 2 ? C<R>{b.x + nl[v].x, b.y + nl[v].y} : C<R>{b.x + nl[v].x * (R)2, b.y + nl[v].y * (R)2};
-      }
-      project(o, p);
-    } else {
-#pragma unroll
-      for (int v = 0; v < 3; v++) a[v] = {a[v].x + nl[v].x / (R)3, a[v].y + nl[v].y / (R)3};
-      project(a, p);
-      if (p.x == 0 && p.z == 0 && p.y == 0) {
-#pragma unroll
-        for (int v = 0; v < 3; v++) a[v] = {0, 0};
-      }
-#pragma unroll
-      for (int v = 0; v < 3; v++) {
-        o[v] = a[v];
-        un[v * tot + i] = a[v];
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-      u[v * tot + i] = o[v];
-      u1[v * tot + i] = a[v];
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2):
+// Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2)
+// or the RK4 substep advanc_K (STAGE 10 + K):
 // gathers P2, C1, P4, C3 from W (transposed mapping), y c2c -i each
 // (xkcomm2.f:229), accumulates N1, N2, N3 (wavespace_vel.f:162-174) in
 // registers (natural mapping), then the integrating-factor update and
@@ -1308,20 +1236,27 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
                    const C<R>* __restrict__ sr, const C<R>* __restrict__ sr2, const double* __restrict__ xfac,
                    const double* __restrict__ yfac, const double* __restrict__ zfac, const C<R>* __restrict__ tw,
-                   const int2* __restrict__ grp, const int* __restrict__ colx, const int* __restrict__ colz) {
+                   const int2* __restrict__ grp, const int* __restrict__ colx, const int* __restrict__ colz,
+                   C<R>* __restrict__ u1) {
   static_assert(!INV2 || STAGE == 1, "the next stage's y pass follows stage 1 only");
+  // STAGE 1, 2: RK2 (proc4a, proc4b); 11..14: RK4 substep K = STAGE - 10
+  // (advanc_K) with the accumulator u1
+  constexpr bool RK4 = STAGE > 10;
+  constexpr int K = STAGE - 10;
+  constexpr bool IF = STAGE == 1 || STAGE == 11;  // integrating factors
+  constexpr bool LAST = STAGE == 2 || STAGE == 14;  // writes un, zero mean
   constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   using Buf = Regs<R, E>;
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
   __shared__ C<R> ssy[N];
-  __shared__ R syf[STAGE == 1 ? N : 1];  // yfac in the state precision
+  __shared__ R syf[IF ? N : 1];  // yfac in the state precision
   __shared__ C<R> ssy2[INV2 ? N : 1];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
   for (int k = threadIdx.x; k < N; k += F::NT) {
     ssy[k] = sr[g.nxh + k];
-    if constexpr (STAGE == 1) syf[k] = (R)yfac[k];
+    if constexpr (IF) syf[k] = (R)yfac[k];
     if constexpr (INV2) ssy2[k] = sr2[g.nxh + k];
   }
   const int tid = threadIdx.x;
@@ -1414,7 +1349,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   // integrating factor xfac yfac zfac as (xfac zfac) yfac in the state
   // precision
   R fxz = 0;
-  if constexpr (STAGE == 1) fxz = (R)(xfac[x] * zfac[z]);
+  if constexpr (IF) fxz = (R)(xfac[x] * zfac[z]);
   auto component = [&](C<R>* nl, const Buf& b, const BufView<R>& dn) {
 #pragma unroll
     for (int m = 0; m < E; m++) {
@@ -1429,21 +1364,63 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
       }
     }
   };
+  if constexpr (!RK4) {
 #pragma unroll
-  for (int m = 0; m < E; m++) U0.v[m] = dn0.load(act ? ofs(m) : kOob);
-  issue_fence();
-  component(a1, U0, dn0);
-  issue_fence();
+    for (int m = 0; m < E; m++) U0.v[m] = dn0.load(act ? ofs(m) : kOob);
+    issue_fence();
+    component(a1, U0, dn0);
+    issue_fence();
 #pragma unroll
-  for (int m = 0; m < E; m++) U0.v[m] = dn1.load(act ? ofs(m) : kOob);
-  issue_fence();
-  component(a2, U0, dn1);
-  issue_fence();
+    for (int m = 0; m < E; m++) U0.v[m] = dn1.load(act ? ofs(m) : kOob);
+    issue_fence();
+    component(a2, U0, dn1);
+    issue_fence();
 #pragma unroll
-  for (int m = 0; m < E; m++) U0.v[m] = dn2.load(act ? ofs(m) : kOob);
-  issue_fence();
-  component(a3, U0, dn2);
-  issue_fence();
+    for (int m = 0; m < E; m++) U0.v[m] = dn2.load(act ? ofs(m) : kOob);
+    issue_fence();
+    component(a3, U0, dn2);
+    issue_fence();
+  } else {
+    // advanc_K per component (advanc_1.f:88-99, advanc_2.f, advanc_3.f,
+    // advanc_4.f:46-52): K 1..3 store u1 (on the mask) and leave the
+    // pre-projection u in nl; K 4 leaves u1 + N/3 in nl for the projection
+    // (the accumulator is not needed again: substep 1 sets it anew)
+    const BufView<R> d10(u1 + col, kOob), d11(u1 + tot + col, kOob), d12(u1 + 2 * tot + col, kOob);
+    auto rk4 = [&](C<R>* nl, const BufView<R>& dn, const BufView<R>& d1) {
+      Buf A;
+      if constexpr (K != 4) {
+#pragma unroll
+        for (int m = 0; m < E; m++) U0.v[m] = dn.load(act ? ofs(m) : kOob);
+      }
+      if constexpr (K != 1) {
+#pragma unroll
+        for (int m = 0; m < E; m++) A.v[m] = d1.load(act ? ofs(m) : kOob);
+      }
+      issue_fence();
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        const unsigned so = act && ((on >> m) & 1u) ? ofs(m) : kOob;
+        if constexpr (K == 1) {
+          const R f = fxz * syf[out.q + T * m];
+          const C<R> ut = U0.v[m];
+          d1.store(so, C<R>{f * (ut.x + nl[m].x / (R)3), f * (ut.y + nl[m].y / (R)3)});
+          nl[m] = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
+        } else if constexpr (K == 2 || K == 3) {
+          d1.store(so, C<R>{A.v[m].x + nl[m].x * (R)2 / (R)3, A.v[m].y + nl[m].y * (R)2 / (R)3});
+          const R w = K == 2 ? (R)1 : (R)2;
+          nl[m] = {U0.v[m].x + nl[m].x * w, U0.v[m].y + nl[m].y * w};
+        } else {
+          nl[m] = {A.v[m].x + nl[m].x / (R)3, A.v[m].y + nl[m].y / (R)3};
+        }
+      }
+    };
+    rk4(a1, dn0, d10);
+    issue_fence();
+    rk4(a2, dn1, d11);
+    issue_fence();
+    rk4(a3, dn2, d12);
+    issue_fence();
+  }
   // projection (proc4a.f:540-571 / proc4b.f:85-106, k = 0 excluded; zeroed
   // in STAGE 2, proc4b.f:176-205); u off the mask is zero
 #pragma unroll
@@ -1460,7 +1437,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
       n0 = {n0.x + kxr * psi.x, n0.y + kxr * psi.y};
       n1 = {n1.x + kyr * psi.x, n1.y + kyr * psi.y};
       n2 = {n2.x + kzr * psi.x, n2.y + kzr * psi.y};
-    } else if constexpr (STAGE == 2) {
+    } else if constexpr (LAST) {
       n0 = n1 = n2 = {0, 0};
     }
     if (!o) n0 = n1 = n2 = {0, 0};
@@ -1473,7 +1450,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
       du0.store(ou, n0);
       du1.store(ou, n1);
       du2.store(ou, n2);
-      if constexpr (STAGE == 2) {
+      if constexpr (LAST) {
         dn0.store(on2, n0);
         dn1.store(on2, n1);
         dn2.store(on2, n2);
@@ -1901,15 +1878,30 @@ struct Launch {
     if (stage == 1 && inv2)
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, true>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz);
+                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
     else if (stage == 1)
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz);
+                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
     else
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz);
+                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
+  }
+  // RK4 substep k: y forward pass + wavespace_vel + advanc_k, shifts of
+  // stage (k - 1) % 2 + 1
+  static void y_rk4(const Ctx& c, int k, double norm, void* u1) {
+    constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
+    dim3 grid(c.geo.ngrp);
+    norm /= 2.0 * N;  // W1 from the fused x pass: x 2n
+    const CR* sr = (const CR*)c.sr[(k - 1) % 2];
+    const auto kern = k == 1   ? k_y_fwd_update<R, N, 11, false>
+                      : k == 2 ? k_y_fwd_update<R, N, 12, false>
+                      : k == 3 ? k_y_fwd_update<R, N, 13, false>
+                               : k_y_fwd_update<R, N, 14, false>;
+    hipLaunchKernelGGL(kern, grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo, norm,
+                       sr, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz,
+                       (CR*)u1);
   }
 };
 
@@ -1950,15 +1942,6 @@ struct LaunchP {
       hipLaunchKernelGGL((k_wavespace_update<R, 2>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac,
                          c.yfac, c.zfac);
-  }
-  static void wavespace_advanc(const Ctx& c, double norm, int k, void* u1) {
-    size_t tot = (size_t)c.geo.na * c.geo.n;
-    const auto kern = k == 1   ? k_wavespace_advanc<R, 1>
-                      : k == 2 ? k_wavespace_advanc<R, 2>
-                      : k == 3 ? k_wavespace_advanc<R, 3>
-                               : k_wavespace_advanc<R, 4>;
-    hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (CR*)c.u, (CR*)c.un, (CR*)u1,
-                       c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac, c.yfac, c.zfac);
   }
   static void ek_shell(const Ctx& c, int kfor, double* d_ek) {
     hipLaunchKernelGGL((k_ek_shell<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (const CR*)c.un,
@@ -2042,6 +2025,24 @@ static int y_update_n(const Ctx& c, int stage, double norm, bool inv2) {
   return -1;
 }
 
+template <typename R>
+static int y_rk4_n(const Ctx& c, int k, double norm, void* u1) {
+  switch (c.geo.n) {
+    case 32: Launch<R, 32>::y_rk4(c, k, norm, u1); return 0;
+    case 64: Launch<R, 64>::y_rk4(c, k, norm, u1); return 0;
+    case 128: Launch<R, 128>::y_rk4(c, k, norm, u1); return 0;
+    case 256: Launch<R, 256>::y_rk4(c, k, norm, u1); return 0;
+    case 512: Launch<R, 512>::y_rk4(c, k, norm, u1); return 0;
+    case 1024: Launch<R, 1024>::y_rk4(c, k, norm, u1); return 0;
+  }
+  return -1;
+}
+
+int launch_y_rk4(const Ctx& c, int k, double norm, void* u1) {
+  if (k < 1 || k > 4) return -1;
+  return c.prec == 8 ? y_rk4_n<double>(c, k, norm, u1) : y_rk4_n<float>(c, k, norm, u1);
+}
+
 int launch_y_update(const Ctx& c, int stage, double norm, bool inv2) {
   if (inv2 && stage != 1) return -1;
   return c.prec == 8 ? y_update_n<double>(c, stage, norm, inv2) : y_update_n<float>(c, stage, norm, inv2);
@@ -2074,10 +2075,6 @@ void launch_update(const Ctx& c, int stage) {
 void launch_wavespace_update(const Ctx& c, double norm, int stage) {
   if (c.prec == 8) LaunchP<double>::wavespace_update(c, norm, stage);
   else LaunchP<float>::wavespace_update(c, norm, stage);
-}
-void launch_wavespace_advanc(const Ctx& c, double norm, int k, void* u1) {
-  if (c.prec == 8) LaunchP<double>::wavespace_advanc(c, norm, k, u1);
-  else LaunchP<float>::wavespace_advanc(c, norm, k, u1);
 }
 void launch_ek_shell(const Ctx& c, int kfor, double* d_ek) {
   if (c.prec == 8) LaunchP<double>::ek_shell(c, kfor, d_ek);
