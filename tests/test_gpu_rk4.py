@@ -5,7 +5,7 @@ seeded field and shift offsets, with and without FEK forcing."""
 import numpy as np
 import pytest
 
-from conftest import rel
+from conftest import kgrid, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -80,3 +80,37 @@ def test_fortran_driver_rk4(P, O, tmp_path):
     assert r.returncode == 0
     out = np.fromfile(tmp_path / "out.bin", np.complex128).reshape(3, g.num_al, n)
     assert rel(out, ro.uny, g.mask()[None]) < 1e-11
+
+
+@pytest.mark.parametrize("n,nsteps", [(256, 2), (1024, 1)])
+def test_rk4_size_independent_properties(P, n, nsteps):
+    """FP32 RK4 steps at 256^3 and the full 1024^3 size: the state after
+    advanc_4 is divergence-free, has zero mean and a Hermitian kx = kz = 0
+    column, u = un, and the TKE stays inside the unforced decay budget."""
+    print("[gpu case] rk4 properties", n, nsteps, flush=True)
+    s = P.Solver(n, 4, viscos=0.01)
+    s.init_field()
+    cx, cz, _ = s.columns()
+    kx, ky, kz = kgrid(n, cx, cz)
+    rg = P.Run4(s, 0.01, np.pi)
+    t0 = s.shell_stats(0)[2]
+    rng = np.random.default_rng(1)
+    for _ in range(nsteps):
+        rg.step(rng.random(6))
+    un = s.get_state()
+    div = np.abs(kx * un[0] + ky * un[1] + kz * un[2])
+    kmag = np.sqrt(kx * kx + ky * ky + kz * kz)
+    scale = np.max(kmag * np.abs(un).max(axis=0))
+    print(f"max |k.u| / max |k||u| = {div.max() / scale:.2e}", flush=True)
+    assert div.max() < 1e-5 * scale
+    del div, kmag
+    assert np.abs(un[:, 0, 0]).max() == 0.0
+    a0 = int(np.nonzero((cx == 0) & (cz == 0))[0][0])
+    col = un[:, a0, :]
+    mirror = np.conj(col[:, (n - np.arange(n)) % n])
+    assert np.abs(col - mirror).max() < 1e-5 * np.abs(col).max()
+    assert np.array_equal(s.get_field(P.FIELD_U, 3), un)
+    _, _, t1, eps = s.shell_stats(n)
+    assert 0 < t1 < t0
+    assert t0 - t1 < 1.2 * eps * sum(rg.dts) + 1e-3 * t0
+    s.close()
