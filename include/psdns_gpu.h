@@ -88,7 +88,9 @@ int psdns_get_state(psdns_ctx *c, void *uny, int layout);
  * the same grid and 3 + nc variables (no resolution change, no kinit
  * subset), zeroes the modes the file does not hold (ky = n/2 row, kz = n/2
  * plane), and sets u = un as put_state does.  Bad headers or record
- * lengths give PSDNS_EINVAL, open/read/write failures PSDNS_EIO. */
+ * lengths give PSDNS_EINVAL, open/read/write failures PSDNS_EIO; a failed
+ * inpen may have replaced the variables before the one that failed (put a
+ * state or read a good file before stepping again). */
 int psdns_outpen(psdns_ctx *c, const char *path, int file_prec);
 int psdns_inpen(psdns_ctx *c, const char *path);
 
