@@ -278,7 +278,13 @@ def advanc(g, k, uy, uny, u1, factors=None):
     k = 0 skipped), advanc_2.f and advanc_3.f (u1 += N 2/3; u = un + N, resp.
     un + 2 N; projection) and advanc_4.f:46-83,149-176 (u1 += N/3; projection
     of u1; un = u = u1 on the mask; zero mean).  EPFOR forcing is not part of
-    it.  factors = (xfac, yfac, zfac) for k = 1."""
+    it.  factors = (xfac, yfac, zfac) for k = 1.
+
+    Pinned by the exact Stokes decay of a Beltrami field (the factor on
+    un and N/3 in u1), fourth order in dt at nu = 0 (the substep weights)
+    and first order at nu > 0 (no factor after substep 1).  Parity unpinned:
+    the factor on substep 1's intermediate u (u = F (un + N)) is fixed by
+    the reference text alone; none of these checks would fail without it."""
     n = g.n
     m = g.mask()
     kx = g.col_x.astype(np.float64)[:, None]

@@ -264,6 +264,26 @@ def test_rk4_fourth_order_in_dt_inviscid(O):
     assert 12.0 < e1 / e2 < 20.0
 
 
+def test_rk4_first_order_in_dt_viscous(O):
+    """With nu > 0 advanc_2 and advanc_3 add the nonlinear term to the
+    unfactored un (no integrating factor after substep 1), so the scheme is
+    first order in dt: the error halves with dt (a consistent
+    integrating-factor RK4 would fall 16-fold)."""
+    g = O.Grid(32)
+    u0 = tg_field(g)
+    T, nu = 0.4, 0.05
+
+    def run(dt):
+        r = O.Run(g, u0, nu, cfl=-1.0, kshift=4, dt_fixed=dt)
+        for _ in range(int(round(T / dt))):
+            r.step_rk4()
+        return r.uny
+
+    ref = run(0.00625)
+    e = [np.linalg.norm((run(dt) - ref).ravel()) for dt in (0.05, 0.025)]
+    assert 1.8 < e[0] / e[1] < 3.0
+
+
 def sinusoidal_field(g):
     """pfield.f (hooks/initial_conditions): au = av = 1, aw = -2"""
     n = g.n
