@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--nc", type=int, default=0,
                     help="passive scalars (Sc = 1, unit mean gradient along x, zero initial fluctuation)")
     ap.add_argument("--rk", type=int, default=2, choices=(2, 4),
-                    help="time scheme: 2 (proc4a/proc4b) or 4 (advanc_1..4, velocity only)")
+                    help="time scheme: 2 (proc4a/proc4b) or 4 (advanc_1..4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -144,9 +144,6 @@ def main():
     solver.viscos = viscos
     kf_shell = 3
     ekinf = solver.ek_shell(kf_shell + 1)
-    if args.rk == 4 and nc:
-        print("bench.py: --rk 4 runs the velocity only (nc = 0)", file=sys.stderr)
-        sys.exit(2)
     RunK = P.Run4 if args.rk == 4 else P.Run
     run = RunK(solver, viscos, math.pi, kf_shell=kf_shell, ekinf=ekinf)
     rng = np.random.default_rng(seed)

@@ -118,7 +118,7 @@ int psdns_stage_nonlinear(psdns_ctx *c, int kstep, double *velmax);
 int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
                        const double *yfac, const double *zfac, const void *forcing);
 
-/* RK4 (rkmethod = 4, timestep_module.f:293-310), velocity only (nc = 0):
+/* RK4 (rkmethod = 4, timestep_module.f:293-310):
  * per substep kstep = 1..4, stage_nonlinear(kstep) (stages 3 and 4 use the
  * shift offsets of stages 1 and 2, shifts.f:133-137; velmax on kstep 1) and
  * then rk4_update(kstep), which replaces the y part of transform +
@@ -130,8 +130,10 @@ int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
  *   2: u1 += 2N/3, u = P (un + N)
  *   3: u1 += 2N/3, u = P (un + 2N)
  *   4: u1 = P (u1 + N/3); un = u = u1; zero mean
- * on the mask, P the projection with k = 0 excluded.  The factors are read
- * on kstep 1 only (NULL allowed after). */
+ * on the mask, P the projection with k = 0 excluded.  Passive scalars take
+ * the same steps without the projection, with xdif ydif zdif in place of F
+ * (advanc_1.f:101-111).  The factors are read on kstep 1 only (NULL
+ * allowed after). */
 int psdns_rk4_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
                      const double *yfac, const double *zfac);
 

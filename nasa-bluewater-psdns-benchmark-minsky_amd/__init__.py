@@ -297,7 +297,8 @@ class Run4:
     """RK4 time loop (rkmethod = 4, timestep_module.f:293-310): four
     substeps, dt and the integrating factors from substep 1, FEK forcing
     after the step.  Shift offsets gsh(3,2) from the caller serve stages
-    1, 2 and again 3, 4 (shifts.f:133-137)."""
+    1, 2 and again 3, 4 (shifts.f:133-137).  Passive scalars of the solver
+    step with the velocity."""
 
     def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None):
         self.s, self.viscos, self.cfl = solver, viscos, cfl

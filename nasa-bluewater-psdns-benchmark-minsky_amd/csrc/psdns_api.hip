@@ -487,7 +487,6 @@ int psdns_get_state(psdns_ctx* c, void* uny, int layout) {
 int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if (int rc = check_ctx(c)) return rc;
   if (kstep < 1 || kstep > 4) return fail(PSDNS_EINVAL, "kstep = %d: stages 1..2 (RK2) or 1..4 (RK4)", kstep);
-  if (kstep > 2 && c->nc > 0) return fail(PSDNS_EINVAL, "RK4 with passive scalars is not supported");
   int rc;
   // stage 2 right after stage_update(1): that update and this stage's phase
   // shift and inverse y pass are one kernel, whose yw the z pass reads from u
@@ -555,13 +554,13 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
                      const double* zfac) {
   if (int rc = ready(c)) return rc;
   if (kstep < 1 || kstep > 4) return fail(PSDNS_EINVAL, "kstep = %d: RK4 has stages 1..4", kstep);
-  if (c->nc > 0) return fail(PSDNS_EINVAL, "RK4 with passive scalars is not supported");
   if (c->last_nonlinear != kstep)
     return fail(PSDNS_ESTATE, "rk4_update(%d) without a preceding stage_nonlinear(%d)", kstep, kstep);
   if (!(dt > 0.0)) return fail(PSDNS_EINVAL, "dt = %g must be positive", dt);
   int rc;
+  const size_t vb = c->csize() * c->cyl_elems();
   if (!c->d_u1) {
-    const size_t bytes = 3 * c->csize() * c->cyl_elems();
+    const size_t bytes = (3 + (size_t)c->nc) * vb;
     if (hipMalloc(&c->d_u1, bytes) != hipSuccess)
       return fail(PSDNS_ENOMEM, "hipMalloc of the RK4 accumulator (%zu bytes) failed", bytes);
     HIPCHK(hipMemsetAsync(c->d_u1, 0, bytes, c->c.stream));  // comp_set.f:74
@@ -575,6 +574,12 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
   // (wavespace_vel.f:80) and advanc_kstep in one kernel, from W1
   if (psdns::launch_y_rk4(c->c, kstep, dt / 2.0 / c->n / c->n, c->d_u1) != 0)
     return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  // the scalars' advanc_kstep (the same loops over i = 4 .. 3 + nc), from
+  // their transformed advection terms of stage_nonlinear
+  for (int i = 0; i < c->nc; i++)
+    psdns::launch_sc_update(c->c, 10 + kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
+                            (const char*)c->d_sc + (3 + i) * vb, dt / 2.0, 1.0 / c->pr[i], c->d_red,
+                            (char*)c->d_u1 + (3 + i) * vb);
   HIPCHK(hipGetLastError());
   if ((rc = mark(c, 6))) return rc;
   if ((rc = sync(c))) return rc;

@@ -102,9 +102,10 @@ int launch_y_grad(const Ctx& c, const void* phi, void* yw);
 // fused x pass of one scalar: c2r of the velocity xv[0..2] and the gradient
 // xg[0..2] (z-transformed), the advection term, r2c x 1/n into c.xs slot 0
 int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]);
-// (dif: nxh + 2 n doubles of scratch for the scalar's integrating factors)
+// (dif: nxh + 2 n doubles of scratch for the scalar's integrating factors;
+// stage 1 | 2 RK2, 11 .. 14 the RK4 substeps with the accumulator u1)
 void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
-                      double* dif);
+                      double* dif, void* u1 = nullptr);
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]
 void launch_vg_deriv(const Ctx& c, void* d);

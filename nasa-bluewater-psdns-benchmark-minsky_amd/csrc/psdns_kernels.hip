@@ -1612,7 +1612,8 @@ template <typename R, int STAGE>
 __global__ void __launch_bounds__(256)
     k_sc_update(C<R>* __restrict__ u, C<R>* __restrict__ un, const C<R>* __restrict__ tr, Geo g, double dt2,
                 const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
-                const double* __restrict__ xdif, const double* __restrict__ ydif, const double* __restrict__ zdif) {
+                const double* __restrict__ xdif, const double* __restrict__ ydif, const double* __restrict__ zdif,
+                C<R>* __restrict__ u1) {
   const size_t tot = (size_t)g.na * g.n;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const Pt p = point(i, g);
@@ -1625,9 +1626,28 @@ __global__ void __launch_bounds__(256)
       const C<R> ut = un[i];
       un[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
       u[i] = {f * (ut.x + nl.x * (R)2), f * (ut.y + nl.y * (R)2)};
-    } else {
+    } else if constexpr (STAGE == 2) {
       const bool k0 = p.a == 0 && p.y == 0 && p.x == 0 && p.z == 0;
       const C<R> v = k0 ? C<R>{0, 0} : un[i] + nl;
+      u[i] = v;
+      un[i] = v;
+    } else if constexpr (STAGE == 11) {
+      // advanc_1.f:101-111: factor xdif ydif zdif; u1 = F (un + N/3), u = F (un + N)
+      const R f = (R)(xdif[p.x] * ydif[p.y] * zdif[p.z]);
+      const C<R> ut = un[i];
+      u1[i] = {f * (ut.x + nl.x / (R)3), f * (ut.y + nl.y / (R)3)};
+      u[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
+    } else if constexpr (STAGE == 12 || STAGE == 13) {
+      // advanc_2.f / advanc_3.f: u1 += N 2/3; u = un + N (resp. + 2 N)
+      const C<R> a = u1[i], b = un[i];
+      u1[i] = {a.x + nl.x * (R)2 / (R)3, a.y + nl.y * (R)2 / (R)3};
+      const R w = STAGE == 12 ? (R)1 : (R)2;
+      u[i] = {b.x + nl.x * w, b.y + nl.y * w};
+    } else {
+      // advanc_4.f:51,149-176: u1 += N/3; un = u = u1; zero mean
+      const bool k0 = p.a == 0 && p.y == 0 && p.x == 0 && p.z == 0;
+      const C<R> a = u1[i];
+      const C<R> v = k0 ? C<R>{0, 0} : C<R>{a.x + nl.x / (R)3, a.y + nl.y / (R)3};
       u[i] = v;
       un[i] = v;
     }
@@ -2154,23 +2174,30 @@ int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double g
   return c.prec == 8 ? x_scalar_r<double>(c, xv, xg, grad) : x_scalar_r<float>(c, xv, xg, grad);
 }
 template <typename R>
-static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif) {
+static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif,
+                        void* u1) {
   const size_t tot = (size_t)c.geo.na * c.geo.n;
   const int k = c.kstep;
-  const auto kern = stage == 1 ? k_sc_update<R, 1> : k_sc_update<R, 2>;
+  const auto kern = stage == 1    ? k_sc_update<R, 1>
+                    : stage == 2  ? k_sc_update<R, 2>
+                    : stage == 11 ? k_sc_update<R, 11>
+                    : stage == 12 ? k_sc_update<R, 12>
+                    : stage == 13 ? k_sc_update<R, 13>
+                                  : k_sc_update<R, 14>;
   const int nxh = c.geo.nxh, n = c.geo.n;
   hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (C<R>*)u, (C<R>*)un,
-                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], dif, dif + nxh, dif + nxh + n);
+                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], dif, dif + nxh, dif + nxh + n,
+                     (C<R>*)u1);
 }
 void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
-                      double* dif) {
-  if (stage == 1) {
+                      double* dif, void* u1) {
+  if (stage == 1 || stage == 11) {
     // c.xfac | c.yfac | c.zfac are one contiguous table of nxh + 2 n entries
     const int len = c.geo.nxh + 2 * c.geo.n;
     hipLaunchKernelGGL(k_sc_dif, dim3((len + 255) / 256), dim3(256), 0, c.stream, c.xfac, len, rpr, dif);
   }
-  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, dif);
-  else sc_update_r<float>(c, stage, u, un, tr, dt2, dif);
+  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, dif, u1);
+  else sc_update_r<float>(c, stage, u, un, tr, dt2, dif, u1);
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);

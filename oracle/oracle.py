@@ -270,6 +270,38 @@ def scalar_update(g, phi_u, phi_n, terms, kstep, dt, gsh, factors=None, pr=None)
             phi_n[i, 0, 0] = 0
 
 
+def scalar_advanc(g, k, phi_u, phi_n, phi_1, terms, ks, dt, gsh, factors=None, pr=None):
+    """Scalar loops of the RK4 substep k (i = 4 .. 3 + nc of
+    PSDNS/homogeneous/advanc_1.f:101-111, advanc_2.f, advanc_3.f,
+    advanc_4.f:46-52,149-176), in place on phi_u, phi_n and the accumulator
+    phi_1, with N = -(dt/2) conj(sx sy sz) terms (wavespace_module.f:347-352,
+    shifts of stage ks) on the mask and, for k = 1, the factor
+    xdif ydif zdif with xdif = xfac^(1/pr) (step.f:387-407).  No
+    projection; zero mean after k = 4.  Pinned like advanc (the same weights
+    and placement, checked through the velocity)."""
+    m = g.mask()
+    nsh = -(dt / 2.0) * np.conj(scalar_shift(g, gsh, ks))
+    for i in range(len(phi_u)):
+        nl = nsh * terms[i]
+        if k == 1:
+            xf, yf, zf = factors
+            r = 1.0 / pr[i]
+            f = (xf[g.col_x] ** r * zf[g.col_z] ** r)[:, None] * (yf ** r)[None, :]
+            a = f * (phi_n[i] + nl / 3.0)
+            u = f * (phi_n[i] + nl)
+        elif k in (2, 3):
+            a = phi_1[i] + nl * 2.0 / 3.0
+            u = phi_n[i] + nl * (1.0 if k == 2 else 2.0)
+        else:
+            a = phi_1[i] + nl * 1.0 / 3.0
+            a[0, 0] = 0
+            u = a
+        phi_1[i] = np.where(m, a, phi_1[i])
+        phi_u[i] = np.where(m, u, phi_u[i])
+        if k == 4:
+            phi_n[i] = np.where(m, a, phi_n[i])
+
+
 def advanc(g, k, uy, uny, u1, factors=None):
     """RK4 substep k = 1..4 (rkmethod = 4) on the cylinder, in place on
     uy[:3] (the nonlinear term N on entry, dt/2 attached), uny and u1, on
@@ -415,15 +447,20 @@ class Run:
         substeps through the same nonlinear term, stages 3 and 4 with the
         shifts of stages 1 and 2 (PSDNS/shifts.f:133-137), dt and the
         integrating factors from substep 1, advanc_1..4, then FEK forcing.
-        Velocity only."""
+        Passive scalars (phi) go through scalar_terms and scalar_advanc."""
         g = self.g
         g.set_shifts(self.next_shifts() if gsh is None else gsh)
+        gsh_now = self.gsh if gsh is None else np.asarray(gsh)
         if not hasattr(self, "u1"):
             self.u1 = g.zeros_cyl(3)  # comp_set.f:72-74
+            if self.phi_n is not None:
+                self.phi_1 = np.zeros_like(self.phi_n)
         for k in (1, 2, 3, 4):
             ks = 1 if k in (1, 3) else 2
             g.phshift_inplace(self.uy, 3, ks)
             ux = g.itransform_vel(self.uy)
+            if self.phi_n is not None:
+                terms = scalar_terms(g, ux, self.phi_u, ks, self.grad, gsh_now)
             vm = g.realspace_vel(ux, 1 if k == 1 else 2)
             nl = g.transform(ux)
             if k == 1:
@@ -433,6 +470,9 @@ class Run:
             g.wavespace_vel(nl, ks, self.dt / 2.0)
             self.uy = nl
             advanc(g, k, self.uy, self.uny, self.u1, self.factors)
+            if self.phi_n is not None:
+                scalar_advanc(g, k, self.phi_u, self.phi_n, self.phi_1, terms, ks, self.dt, gsh_now, self.factors,
+                              self.pr)
         if self.kfor:
             g.ek_force(self.uy, self.uny, self.kf_shell, self.kfor, self.ekinf)
         self.istep += 1

@@ -49,10 +49,38 @@ def test_rk4_call_order(P, O):
     with pytest.raises(P.PsdnsError, match="error -1"):
         s.rk4_update(5, 0.01)
     s.close()
-    sc = P.Solver(32, prec=8, nc=1)
-    with pytest.raises(P.PsdnsError, match="error -1"):
-        sc.stage_nonlinear(3)
-    sc.close()
+
+
+def scalar_ic(g, nc, seed):
+    rng = np.random.default_rng(seed)
+    phi = g.xktran(rng.standard_normal((nc, g.n, g.n, g.n)), nc)
+    phi[:, ~g.mask()] = 0
+    return phi
+
+
+@pytest.mark.parametrize("n,prec,nsteps", [(32, 8, 2), (64, 4, 2)])
+def test_rk4_scalars_match_oracle(P, O, n, prec, nsteps):
+    """RK4 with two passive scalars (advanc_K's loops over i = 4 .. 3 + nc)."""
+    print("[gpu case] rk4 scalars", n, prec, nsteps, flush=True)
+    g = O.Grid(n)
+    u0 = g.field(seed=31)
+    phi0 = scalar_ic(g, 2, 7)
+    pr, grad = [0.7, 1.3], [(1.0, 0.0, 0.0), (0.0, 0.5, -0.25)]
+    nu = 0.02
+    ro = O.Run(g, u0, nu, cfl=np.pi, kshift=1, shift_seed=3, phi=phi0, pr=pr, grad=grad)
+    s = P.Solver(n, prec, viscos=nu, nc=2, pr=pr, grad=grad)
+    s.put_state(np.concatenate([u0, phi0]))
+    rg = P.Run4(s, nu, np.pi)
+    m = g.mask()[None]
+    for _ in range(nsteps):
+        gsh = ro.next_shifts()
+        ro.step_rk4(gsh)
+        rg.step(gsh)
+    st = s.get_state()
+    ev, es = rel(st[:3], ro.uny, m), rel(st[3:], ro.phi_n, m)
+    print(f"velocity rel L2 {ev:.2e}, scalars rel L2 {es:.2e}", flush=True)
+    assert ev < TOL[prec] and es < TOL[prec]
+    s.close()
 
 
 def test_fortran_driver_rk4(P, O, tmp_path):

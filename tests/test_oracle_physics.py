@@ -284,6 +284,23 @@ def test_rk4_first_order_in_dt_viscous(O):
     assert 1.8 < e[0] / e[1] < 3.0
 
 
+def test_rk4_scalar_pure_diffusion_is_exact(O):
+    """u = 0, G = 0: the scalar terms vanish and RK4 (factor in substep 1
+    only) gives each mode exp(-nu/pr k^2 dt) per step exactly."""
+    n, nu, pr, dt = 32, 0.05, 0.7, 0.01
+    g = O.Grid(n)
+    rng = np.random.default_rng(9)
+    phi0 = g.xktran(rng.standard_normal((1, n, n, n)), 1)
+    phi0[:, ~g.mask()] = 0
+    phi0[0, 0, 0] = 0
+    ro = O.Run(g, g.zeros_cyl(3), nu, cfl=np.pi, kshift=4, dt_fixed=dt, phi=phi0, pr=[pr], grad=[(0, 0, 0)])
+    for _ in range(3):
+        ro.step_rk4()
+    kx, ky, kz = kgrid(n, g.col_x, g.col_z)
+    ref = phi0[0] * np.exp(-nu / pr * (kx * kx + ky * ky + kz * kz) * 3 * dt)
+    assert rel(ro.phi_n[0], ref, g.mask()) < 1e-13
+
+
 def sinusoidal_field(g):
     """pfield.f (hooks/initial_conditions): au = av = 1, aw = -2"""
     n = g.n
