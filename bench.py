@@ -9,6 +9,13 @@ kmax*eta = 1.5.  Synthetic seeded initial field (no datasets).  A "step" is
 one full RK2 step: 2 x (stage_nonlinear + stage_update) + FEK forcing, all
 state resident in HBM.  Prints one JSON line.
 
+After a velocity-only RK2 run the same grid is run again with two passive
+scalars (Sc = 1, unit mean gradient along x, zero initial fluctuation):
+its step time, per-pass device times per RK substep and the roofline of the
+scalar path's dominant kernel go into the line's "config3_nc2" object
+(--no-config3 skips it).  Step times are also reported as min/mean/max over
+steps 2..K.
+
 The CPU baseline times the FP64 oracle (a port of the reference algorithm)
 on a bounded sample (RK2 steps at 256^3 for about 10-20 s, all host cores)
 and scales it to n^3 by n^3 log2 n.
@@ -46,6 +53,13 @@ def pass_bytes(n, s, num_al, max_al_x):
       x fused     3 spectral in, 5 spectral out (kx < max_al_x)     8 * n^2 * max_al_x * 2s
       z-forward   5 lines in, 4 cylinder columns out                (5 * n * max_al_x * n + 4 * num_al * n) * 2s
       y-fwd+upd   4 vars gathered + un read, u and un written       (4 + 3 + 3 + 3) * num_al * n * 2s
+    and per passive scalar:
+      gradient y  1 cylinder in, 3 transformed derivatives out      4 * num_al * n * 2s
+      z-inverse   3 derivatives, as the velocity's z-inverse         3 * (num_al * n + n * max_al_x * n) * 2s
+      x pass      6 spectral in (u, v, w, 3 derivatives), 1 out      7 * n^2 * max_al_x * 2s
+      z-forward   1 line set in, 1 cylinder out                      (n * max_al_x * n + num_al * n) * 2s
+      y-forward   1 cylinder in, 1 out                               2 * num_al * n * 2s
+      update      term and un in, u and un out                       4 * num_al * n * 2s
     (nkept * max_al_x = num_al)."""
     c = 2 * s
     return {
@@ -54,7 +68,107 @@ def pass_bytes(n, s, num_al, max_al_x):
         "x_fused": 8 * n * n * max_al_x * c,
         "z_forward": (5 * n * max_al_x * n + 4 * num_al * n) * c,
         "y_forward_update": 13 * num_al * n * c,
+        "sc_grad_y": 4 * num_al * n * c,
+        "sc_z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
+        "sc_x_pass": 7 * n * n * max_al_x * c,
+        "sc_z_forward": (n * max_al_x * n + num_al * n) * c,
+        "sc_y_forward": 2 * num_al * n * c,
+        "sc_update": 4 * num_al * n * c,
     }
+
+
+VEL = ["y_inverse", "z_inverse", "x_fused", "z_forward", "y_forward_update"]
+SC = ["sc_grad_y", "sc_z_inverse", "sc_x_pass", "sc_z_forward", "sc_y_forward", "sc_update"]
+
+
+def launches_per_step(nc, rk):
+    """launches of each pass per step: RK2 has two stages (the stage-2 y
+    inverse of a velocity-only run is done by the stage-1 update kernel),
+    RK4 four substeps; every scalar pass runs once per scalar and stage"""
+    st = 2 if rk == 2 else 4
+    per = {nm: st for nm in VEL}
+    if rk == 2 and nc == 0:
+        per["y_inverse"] = 1
+    for nm in SC:
+        per[nm] = st * nc
+    return per
+
+
+def run_case(P, n, prec, nc, rk, steps, warmup, seed=122929298):
+    """One timed run: seeded field, viscosity from kmax eta = 1.5, FEK
+    forcing of shells |k| <= 2, kshift 1 shifts; returns timings and the
+    per-pass device times (HIP events on the library's stream)."""
+    s_bytes = 4 if prec == 4 else 8
+    solver = P.Solver(n, prec, viscos=1.0, nc=nc, pr=[1.0] * nc, grad=[(1.0, 0.0, 0.0)] * nc)
+    try:
+        solver.init_field(seed=seed, kpeak=3.0, tke=0.5)
+        # viscosity from kmax eta = 1.5: eps = 2 nu D, eta = (nu^3/eps)^(1/4)
+        _, _, tke0, eps_nu1 = solver.shell_stats(0)
+        D = eps_nu1 / 2.0
+        kmax = math.sqrt(2.0) / 3.0 * n
+        viscos = math.sqrt(2.0 * D) * (1.5 / kmax) ** 2
+        solver.viscos = viscos
+        kf_shell = 3
+        ekinf = solver.ek_shell(kf_shell + 1)
+        RunK = P.Run4 if rk == 4 else P.Run
+        run = RunK(solver, viscos, math.pi, kf_shell=kf_shell, ekinf=ekinf)
+        rng = np.random.default_rng(seed)
+        gsh = np.zeros(6)
+        gsh[3:] = rng.random(3)
+
+        def next_gsh():
+            r = rng.random(3)
+            gsh[:3] = np.mod(gsh[3:] + 0.5, 1.0)  # shifts.f kshift = 1
+            gsh[3:] = r
+            return gsh.copy()
+
+        for _ in range(warmup):
+            run.step(next_gsh())
+        print(f"[bench] n={n} nc={nc} rk={rk}: warmup done", file=sys.stderr, flush=True)
+        solver.pass_times(True)
+        step_s = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            run.step(next_gsh())  # every entry point is synchronous at return
+            step_s.append(time.perf_counter() - a)
+        t1 = time.perf_counter()
+        tab = solver.pass_times(False)  # [substep][pass] ms over the timed steps
+        print(f"[bench] n={n} nc={nc} rk={rk}: timed steps done", file=sys.stderr, flush=True)
+        _, _, tke, eps = solver.shell_stats(0)
+        num_al, max_al_x = solver.num_al, solver.max_al_x
+        mean_dt = float(np.mean(run.dts[-steps:]))
+    finally:
+        solver.close()
+    sps = (t1 - t0) / steps
+    nb = pass_bytes(n, s_bytes, num_al, max_al_x)
+    per = launches_per_step(nc, rk)
+    names = VEL + SC
+    passes = {}
+    for i, nm in enumerate(names):
+        if per[nm] == 0:
+            continue
+        ms_step = float(tab[:, i].sum()) / steps
+        ms = ms_step / per[nm]
+        bytes_ok = not (rk == 4 and nm == "y_forward_update")  # RK4 adds the accumulator u1
+        passes[nm] = {"ms": round(ms, 4), "ms_per_step": round(ms_step, 3), "launches_per_step": per[nm],
+                      "GBps": round(nb[nm] / (ms * 1e-3) / 1e9, 1) if bytes_ok and ms > 0 else None}
+    rest = step_s[1:] if len(step_s) > 1 else step_s
+    return {
+        "n": n, "nc": nc, "rk": rk, "prec": prec, "s_bytes": s_bytes, "sps": sps, "viscos": viscos,
+        "tke": tke, "eps": eps, "mean_dt": mean_dt, "nb": nb, "per": per, "passes": passes, "tab": tab,
+        "step_ms": {"min": round(min(rest) * 1e3, 3), "mean": round(float(np.mean(rest)) * 1e3, 3),
+                    "max": round(max(rest) * 1e3, 3), "over": f"steps 2..{steps}"},
+        "by_substep_ms": [[round(float(v) / steps, 3) for v in row] for row in tab[:(2 if rk == 2 else 4)]],
+    }
+
+
+def fft_summary(r, names):
+    nb, per, passes = r["nb"], r["per"], r["passes"]
+    timed = [nm for nm in names if nm in passes and passes[nm]["GBps"] is not None]
+    fft_bytes = sum(nb[nm] * per[nm] for nm in timed)
+    fft_ms = sum(passes[nm]["ms_per_step"] for nm in timed)
+    return timed, fft_bytes, fft_ms
 
 
 def cpu_baseline(n_target, budget_s=15.0):
@@ -117,6 +231,9 @@ def main():
                     help="passive scalars (Sc = 1, unit mean gradient along x, zero initial fluctuation)")
     ap.add_argument("--rk", type=int, default=2, choices=(2, 4),
                     help="time scheme: 2 (proc4a/proc4b) or 4 (advanc_1..4)")
+    ap.add_argument("--no-config3", dest="config3", action="store_false",
+                    help="skip the two-scalar run that follows a velocity-only RK2 run")
+    ap.add_argument("--config3-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -129,78 +246,23 @@ def main():
         sys.exit(2)
 
     P = load_package()
-    n, prec = args.n, args.prec
-    s_bytes = 4 if prec == 4 else 8
-    seed = 122929298
-    print(f"[bench] n={n} prec=fp{8 * s_bytes} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    nc = args.nc
-    solver = P.Solver(n, prec, viscos=1.0, nc=nc, pr=[1.0] * nc, grad=[(1.0, 0.0, 0.0)] * nc)
-    solver.init_field(seed=seed, kpeak=3.0, tke=0.5)
-    # viscosity from kmax eta = 1.5: eps = 2 nu D, eta = (nu^3/eps)^(1/4)
-    _, _, tke0, eps_nu1 = solver.shell_stats(0)
-    D = eps_nu1 / 2.0
-    kmax = math.sqrt(2.0) / 3.0 * n
-    viscos = math.sqrt(2.0 * D) * (1.5 / kmax) ** 2
-    solver.viscos = viscos
-    kf_shell = 3
-    ekinf = solver.ek_shell(kf_shell + 1)
-    RunK = P.Run4 if args.rk == 4 else P.Run
-    run = RunK(solver, viscos, math.pi, kf_shell=kf_shell, ekinf=ekinf)
-    rng = np.random.default_rng(seed)
-    gsh = np.zeros(6)
-    gsh[3:] = rng.random(3)
-
-    def next_gsh():
-        r = rng.random(3)
-        gsh[:3] = np.mod(gsh[3:] + 0.5, 1.0)  # shifts.f kshift = 1
-        gsh[3:] = r
-        return gsh.copy()
-
-    for _ in range(args.warmup):
-        run.step(next_gsh())
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-    solver.stage_times(True)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        run.step(next_gsh())
-    t1 = time.perf_counter()
-    times = solver.stage_times(False)
-    print("[bench] timed steps done", file=sys.stderr, flush=True)
-    sps = (t1 - t0) / args.steps
-    _, _, tke, eps = solver.shell_stats(0)
-
-    names = ["y_inverse", "z_inverse", "x_fused", "z_forward", "y_forward_update"]
-    nb = pass_bytes(n, s_bytes, solver.num_al, solver.max_al_x)
-    # launches per RK2 step: two of each pass, except the y inverse of
-    # stage 2, which the stage-1 y update kernel performs (same bytes there:
-    # it writes yw in place of u) when there are no scalars; the scalars'
-    # own transforms are not in these pass timers
-    per_step = {nm: 2 for nm in names}
-    per_step["y_inverse"] = 1 if nc == 0 else 2
-    if args.rk == 4:
-        # four substeps, none fused across substeps; the last timer is the
-        # fused y pass with advanc_K, whose accumulator traffic the y-update
-        # byte model does not include
-        per_step = {nm: 4 for nm in names}
-    passes = {}
-    for i, nm in enumerate(names):
-        ms = times[i] / (per_step[nm] * args.steps)
-        gbps = None if args.rk == 4 and nm == "y_forward_update" else round(nb[nm] / (ms * 1e-3) / 1e9, 1)
-        passes[nm] = {"ms": round(ms, 4), "GBps": gbps}
-    timed = names if args.rk == 2 else names[:4]
-    fft_bytes = sum(nb[nm] * per_step[nm] for nm in timed)
-    fft_ms = sum(times[names.index(nm)] for nm in timed) / args.steps
-    dominant = max(timed, key=lambda k: passes[k]["ms"])
-    dom_ms = times[names.index(dominant)] / (per_step[dominant] * args.steps)
+    n, prec, nc = args.n, args.prec, args.nc
+    print(f"[bench] n={n} prec=fp{8 * (4 if prec == 4 else 8)} steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+    r = run_case(P, n, prec, nc, args.rk, args.steps, args.warmup)
+    sps, nb, passes = r["sps"], r["nb"], r["passes"]
+    timed, fft_bytes, fft_ms = fft_summary(r, VEL + SC)
+    dominant = max(timed, key=lambda k: passes[k]["ms_per_step"])
+    dom_ms = passes[dominant]["ms"]
     achieved = nb[dominant] / (dom_ms * 1e-3) / 1e9
 
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE counter
     # passes over this same workload (scaled by a plain copy's readings)
-    traffic = None
-    tfile = REPO / "profiles" / f"traffic_{n}_fp{8 * s_bytes}.json"
-    if tfile.exists():
-        kern = json.loads(tfile.read_text()).get("kernels", {})
-        traffic = kern.get(dominant, {}).get("bytes_per_launch")
+    def traffic_of(kernel, n, s_bytes, nc):
+        tfile = REPO / "profiles" / f"traffic_{n}_fp{8 * s_bytes}{'_nc%d' % nc if nc else ''}.json"
+        if tfile.exists():
+            return json.loads(tfile.read_text()).get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
+        return None
 
     line = {
         "metric": ("s/RK2 step & FFT HBM-roofline % at 1024^3 FP32; LES-closure speedup vs DNS" if args.rk == 2
@@ -221,10 +283,11 @@ def main():
                         + ("velocity only (nc=0), " if nc == 0 else
                            f"{nc} passive scalars (Sc=1, mean gradient along x, zero initial fluctuation), ")
                         + f"RK{args.rk} + phase shifts (kshift 1), FEK forcing kf_shell 3, CFL pi",
-            "n": n, "nc": nc, "viscos": viscos, "cfl": math.pi, "kmax_eta": 1.5,
+            "n": n, "nc": nc, "viscos": r["viscos"], "cfl": math.pi, "kmax_eta": 1.5,
             "grid_points_per_s": n ** 3 / sps,
-            "tke": tke, "epsilon": eps, "mean_dt": float(np.mean(run.dts[-args.steps:])),
+            "tke": r["tke"], "epsilon": r["eps"], "mean_dt": r["mean_dt"],
         },
+        "step_ms": r["step_ms"],
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
@@ -232,7 +295,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": traffic_of(dominant, n, r["s_bytes"], nc),
             "bytes_per_launch": nb[dominant],
             "ms_per_launch": round(dom_ms, 4),
         },
@@ -241,14 +304,36 @@ def main():
             "alg_GB_per_step": round(fft_bytes / 1e9, 3),
             "frac": round(fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "passes": passes,
+            "by_substep_ms": r["by_substep_ms"],
         },
     }
+    if args.config3 and nc == 0 and args.rk == 2:
+        # the configuration with two passive scalars (Sc = 1, unit mean
+        # gradient along x, zero initial fluctuation), same grid and precision
+        r3 = run_case(P, n, prec, 2, 2, args.config3_steps, args.warmup)
+        t3, b3, ms3 = fft_summary(r3, VEL + SC)
+        sc_dom = max(SC, key=lambda k: r3["passes"][k]["ms_per_step"])
+        sd_ms = r3["passes"][sc_dom]["ms"]
+        sd = r3["nb"][sc_dom] / (sd_ms * 1e-3) / 1e9
+        line["config3_nc2"] = {
+            "workload": f"forced isotropic DNS {n}^3 {'FP32' if prec == 4 else 'FP64'} with 2 passive scalars "
+                        "(Sc=1, mean gradient along x, zero initial fluctuation), RK2",
+            "s_per_step": r3["sps"], "ms_per_step": round(r3["sps"] * 1e3, 3), "steps": args.config3_steps,
+            "step_ms": r3["step_ms"],
+            "grid_points_per_s": n ** 3 / r3["sps"],
+            "fft_passes": {"per_step_ms": round(ms3, 3), "alg_GB_per_step": round(b3 / 1e9, 3),
+                           "frac": round(b3 / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "passes": r3["passes"], "by_substep_ms": r3["by_substep_ms"]},
+            "scalar_roofline": {"bound": "hbm", "kernel": sc_dom, "achieved": round(sd, 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(sd / HBM_PEAK_GBS, 4),
+                                "traffic": traffic_of(sc_dom, n, r3["s_bytes"], 2),
+                                "bytes_per_launch": r3["nb"][sc_dom], "ms_per_launch": round(sd_ms, 4)},
+        }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n)
     else:
         line["cpu_baseline"] = None
-    solver.close()
     print(json.dumps(line), flush=True)
 
 

@@ -26,6 +26,11 @@
  *    psdns_last_error().  The Fortran caller maps non-zero to abrt().
  *  - One host thread per context.  Host pointers are borrowed for the call
  *    only.  The context owns all device memory.
+ *  - stage_update / rk4_update need the nonlinear terms of the
+ *    stage_nonlinear just before them: any call in between that writes
+ *    the work arrays or the shift tables (set_shifts, put_field,
+ *    vg_moments, the per-routine entry points, put_state, init_field,
+ *    inpen) discards them, and the update then fails with PSDNS_ESTATE.
  */
 #ifndef PSDNS_GPU_H
 #define PSDNS_GPU_H
@@ -106,11 +111,15 @@ int psdns_set_shifts(psdns_ctx *c, const double gsh[6]);
  * stage_update = the y-forward part of transform + wavespace_vel + proc4a
  * (kstep 1) or proc4b (kstep 2) (timestep_module.f:244-317).  xfac(n/2),
  * yfac(n), zfac(n) are the integrating factors of step.f:366-385, used by
- * kstep 1 only (proc4b has none; they may be NULL on kstep 2); forcing
- * must be NULL (EPFOR forcing increments are not supported; FEK forcing is
- * psdns_ek_force).
- * stage_update(1) only records its arguments: the update itself runs in
- * the next call.  When that call is stage_nonlinear(2), the update and stage
+ * kstep 1 only (proc4b has none; they may be NULL on kstep 2).  forcing
+ * is NULL or the EPFOR increments for(kfor, k2fo, k2fo, 3), k2fo =
+ * 2 kfor - 1, as the reference's force.f forms them (FP64 complex,
+ * Fortran order; needs psdns_set_epfor first): added after the projection,
+ * on kstep 1 where |k| <= kforce with the efkz tally (proc4a.f:588-640),
+ * on kstep 2 in the whole box before un <- u (proc4b.f:118-163).  FEK
+ * forcing is psdns_ek_force.
+ * stage_update(1) without forcing only records its arguments: the update
+ * itself runs in the next call.  When that call is stage_nonlinear(2), the update and stage
  * 2's phase shift and inverse y transform are one kernel; any other call
  * runs the update on its own first, so every result the caller can observe
  * is the same either way. */
@@ -136,6 +145,22 @@ int psdns_stage_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
  * allowed after). */
 int psdns_rk4_update(psdns_ctx *c, int kstep, double dt, const double *xfac,
                      const double *yfac, const double *zfac);
+
+/* EPFOR forcing (the -DEPFOR build, makefile:69): set_epfor gives the
+ * forcing box kfor (param_set.f:33: kfor = kforce + 1.5 for a cube) and the
+ * radius kforce (input.f:186); kfor = 0 turns it off.  The increments
+ * come with each stage_update (RK2) or, for RK4, with rk4_forcing, which
+ * keeps them for the following rk4_update calls until replaced or cleared
+ * (NULL): advanc_1 adds them with the tally, advanc_2 and advanc_3 in the
+ * whole box; advanc_4's add is overwritten by u1 on the mask in the
+ * reference (advanc_4.f:96-170), so substep 4 has none.
+ * epfor_tally returns efkz(nxh, 3) of the last proc4a / advanc_1 (zeroed
+ * when that stage had no increments, proc4a.f:66): efkz[(i-1) nxh + k-1]
+ * = sum over shell k of 2 Re(u conj(for_i)) tfact, FP64, summed in a fixed
+ * order (the caller's forcdt.f divides by dt). */
+int psdns_set_epfor(psdns_ctx *c, int kfor, double kforce);
+int psdns_epfor_tally(psdns_ctx *c, double *efkz);
+int psdns_rk4_forcing(psdns_ctx *c, const void *forcing);
 
 /* FEK forcing (ek_force.f:1-76 with ek_shell.f:81-176): shell energies of
  * un for ik = int(|k| + 1.5) <= kfor, then u and un scaled by
@@ -185,10 +210,26 @@ int psdns_xktran(psdns_ctx *c, int nv);                    /* xktran.f:1-88: ux(
 
 /* Device time of the last stage calls, per pass (ms), the reference's
  * t_itrans/t_trans split (timestep_module.f:98-437); times[0..5] =
- * y-inverse, z-inverse, x-fused, z-forward, y-forward+update, total.  A
- * stage 2 that follows stage_update(1) directly has no y-inverse of its own:
- * that transform is timed with the y-forward+update of stage 1. */
+ * y-inverse, z-inverse, x-fused, z-forward, y-forward+update, total, for the
+ * velocity passes (psdns_pass_times has the scalar passes too).  A stage 2
+ * that follows stage_update(1) directly has no y-inverse of its own: that
+ * transform is timed with the y-forward+update of stage 1. */
 int psdns_stage_times(psdns_ctx *c, int enable, double *times);
+
+/* Device time of the passes per RK substep, the reference's per-substep
+ * tt(1..10) / t_itrans / t_trans timers (timestep_module.f:98-437) at the
+ * granularity of this library's passes, scalars included: times[11 st + p]
+ * for substep st = 0..3 (RK2 uses 0 and 1) and pass p =
+ *   0 y-inverse (+ phase1)   1 z-inverse   2 x pass (+ realspace_vel)
+ *   3 z-forward   4 y-forward + wavespace + update (RK4: + advanc_k)
+ *   5 scalar gradient y pass   6 scalar z-inverse   7 scalar x pass
+ *   8 scalar z-forward   9 scalar y-forward   10 scalar updates
+ * summed over the scalars and over the calls since the last read (a stage 2
+ * that follows stage_update(1) directly has its y-inverse inside substep
+ * 0's pass 4).  ntimes >= 44.  Reading clears the table; enable turns the
+ * timing on or off (psdns_stage_times shares the table: its times[0..4]
+ * are passes 0..4 summed over the substeps, times[5] their total). */
+int psdns_pass_times(psdns_ctx *c, int enable, double *times, int ntimes);
 
 const char *psdns_last_error(void);
 

@@ -31,7 +31,7 @@ EXPORTS = (
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
     "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_rk4_update",
-    "psdns_last_error",
+    "psdns_set_epfor", "psdns_epfor_tally", "psdns_rk4_forcing", "psdns_pass_times", "psdns_last_error",
 )
 
 
@@ -101,6 +101,10 @@ def load_library():
         "psdns_outpen": [_vp, ctypes.c_char_p, i],
         "psdns_inpen": [_vp, ctypes.c_char_p],
         "psdns_rk4_update": [_vp, i, d, _dp, _dp, _dp],
+        "psdns_set_epfor": [_vp, i, d],
+        "psdns_epfor_tally": [_vp, _dp],
+        "psdns_rk4_forcing": [_vp, _vp],
+        "psdns_pass_times": [_vp, i, _dp, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -210,12 +214,39 @@ class Solver:
         self._chk(self.L.psdns_stage_nonlinear(self.h, kstep, _dptr(vm)))
         return float(vm[0])
 
-    def stage_update(self, kstep, dt, xfac=None, yfac=None, zfac=None):
+    def stage_update(self, kstep, dt, xfac=None, yfac=None, zfac=None, forcing=None):
+        """forcing: EPFOR increments for(kfor, k2fo, k2fo, 3) as a (3, k2fo,
+        k2fo, kfor) complex array (needs set_epfor)"""
+        f = None if forcing is None else self._forcing(forcing)
+        fp = None if f is None else f.ctypes.data
         if xfac is None:
-            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, None, None, None, None))
+            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, None, None, None, fp))
         else:
             xf, yf, zf = (np.ascontiguousarray(f, np.float64) for f in (xfac, yfac, zfac))
-            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, _dptr(xf), _dptr(yf), _dptr(zf), None))
+            self._chk(self.L.psdns_stage_update(self.h, kstep, dt, _dptr(xf), _dptr(yf), _dptr(zf), fp))
+
+    def _forcing(self, forcing):
+        f = np.ascontiguousarray(forcing, np.complex128)
+        k = getattr(self, "epf_kfor", 0)
+        if k == 0 or f.shape != (3, 2 * k - 1, 2 * k - 1, k):
+            raise PsdnsError(f"forcing needs set_epfor and shape (3, 2kfor-1, 2kfor-1, kfor), got {f.shape}")
+        return f
+
+    def set_epfor(self, kfor, kforce):
+        """EPFOR forcing box kfor and radius kforce (0 turns it off)"""
+        self._chk(self.L.psdns_set_epfor(self.h, kfor, kforce))
+        self.epf_kfor = kfor
+
+    def epfor_tally(self):
+        """efkz(nxh, 3) of the last proc4a / advanc_1 as a (3, n/2) array"""
+        e = np.zeros((3, self.n // 2))
+        self._chk(self.L.psdns_epfor_tally(self.h, _dptr(e)))
+        return e
+
+    def rk4_forcing(self, forcing):
+        """EPFOR increments for the following rk4_update calls (None clears)"""
+        f = None if forcing is None else self._forcing(forcing)
+        self._chk(self.L.psdns_rk4_forcing(self.h, None if f is None else f.ctypes.data))
 
     def rk4_update(self, kstep, dt, xfac=None, yfac=None, zfac=None):
         """advanc_kstep (homogeneous/advanc_1.f .. advanc_4.f) after
@@ -261,6 +292,15 @@ class Solver:
         self._chk(self.L.psdns_stage_times(self.h, int(enable), _dptr(t)))
         return t
 
+    PASS_NAMES = ("y_inverse", "z_inverse", "x_pass", "z_forward", "y_forward_update", "sc_grad_y",
+                  "sc_z_inverse", "sc_x_pass", "sc_z_forward", "sc_y_forward", "sc_update")
+
+    def pass_times(self, enable=True):
+        """device ms per RK substep (4 rows) and pass (PASS_NAMES), cleared on read"""
+        t = np.zeros(44)
+        self._chk(self.L.psdns_pass_times(self.h, int(enable), _dptr(t), 44))
+        return t.reshape(4, 11)
+
     # per-routine entry points
     def phase1(self, kstep):
         self._chk(self.L.psdns_phase1(self.h, kstep))
@@ -300,16 +340,22 @@ class Run4:
     1, 2 and again 3, 4 (shifts.f:133-137).  Passive scalars of the solver
     step with the velocity."""
 
-    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None):
+    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None, epfor=None):
         self.s, self.viscos, self.cfl = solver, viscos, cfl
         self.kf_shell = kf_shell
         self.kfor = kf_shell + 1 if kf_shell > 0 else 0
         self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
         self.dts = []
+        # EPFOR: (kfor, kforce, forcing) with forcing(istep) the increments of step istep
+        self.epfor = epfor
+        if epfor is not None:
+            solver.set_epfor(epfor[0], epfor[1])
 
     def step(self, gsh):
         s = self.s
         s.set_shifts(gsh)
+        if self.epfor is not None:
+            s.rk4_forcing(self.epfor[2](len(self.dts)))
         for k in range(1, 5):
             vm = s.stage_nonlinear(k)
             if k == 1:
@@ -330,12 +376,17 @@ class Run:
     stage_nonlinear -> (stage 1: dt = cfl/velmax, integrating factors) ->
     stage_update, then FEK forcing.  Shift offsets come from the caller."""
 
-    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None):
+    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None, epfor=None):
         self.s, self.viscos, self.cfl = solver, viscos, cfl
         self.kf_shell = kf_shell
         self.kfor = kf_shell + 1 if kf_shell > 0 else 0
         self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
         self.dts = []
+        # EPFOR: (kfor, kforce, forcing) with forcing(istep) the increments
+        # for(kfor, k2fo, k2fo, 3) of step istep (force.f, called after step)
+        self.epfor = epfor
+        if epfor is not None:
+            solver.set_epfor(epfor[0], epfor[1])
 
     def step(self, gsh):
         s = self.s
@@ -344,9 +395,10 @@ class Run:
         self.velmax = vm
         self.dt = self.cfl / vm
         xf, yf, zf = s.step_factors(self.dt)
-        s.stage_update(1, self.dt, xf, yf, zf)
+        f = None if self.epfor is None else self.epfor[2](len(self.dts))
+        s.stage_update(1, self.dt, xf, yf, zf, forcing=f)
         s.stage_nonlinear(2)
-        s.stage_update(2, self.dt, xf, yf, zf)
+        s.stage_update(2, self.dt, xf, yf, zf, forcing=f)
         if self.kfor:
             s.ek_force(self.kf_shell, self.kfor, self.ekinf)
         self.dts.append(self.dt)

@@ -71,7 +71,20 @@ struct psdns_ctx {
   double* d_red = nullptr;     // reductions scratch: [2 + 2*n + n]
   unsigned long long* d_velmax = nullptr;
   int2* d_force = nullptr;
+  double* d_force_terms = nullptr;  // 2 doubles per force-list entry (fixed-order shell sums)
   int force_kfor = -1, nforce = 0;
+  double* d_shell_part = nullptr;   // partial rows of psdns_shell_stats, allocated on first use
+  int dev = 0;                      // device of the context's stream
+  // EPFOR forcing increments (psdns_set_epfor): the cylinder entries of the
+  // forcing box, the caller's for(kfor, k2fo, k2fo, 3) on the device, the
+  // per-entry tally terms and efkz(nxh, 3) of the last proc4a / advanc_1
+  int epf_kfor = 0, n_epf = 0;
+  double epf_kforce = 0.0;
+  int4* d_epf = nullptr;
+  double2* d_epf_for = nullptr;
+  double* d_epf_terms = nullptr;
+  double* d_efkz = nullptr;
+  bool rk4_forcing = false;  // psdns_rk4_forcing set the increments of this step
   int last_nonlinear = 0;
   // stage 1's update is deferred to the next call: stage_nonlinear(2) runs
   // it in one kernel with its own phase shift and inverse y pass (yw left in
@@ -81,7 +94,12 @@ struct psdns_ctx {
   // pass timers
   bool timing = false;
   hipEvent_t ev[8] = {};
-  double times[6] = {0, 0, 0, 0, 0, 0};
+  // device ms per RK substep (0..3) and pass (psdns_pass_times order); the
+  // scalar passes are bracketed by evs: 6 boundaries per scalar for the
+  // nonlinear passes, then nc + 1 around the scalar updates
+  static constexpr int kPasses = 11, kStages = 4, kMaxScalars = 8;
+  double tt[kStages][kPasses] = {};
+  std::vector<hipEvent_t> evs;
 
   size_t csize() const { return prec == 8 ? 16 : 8; }  // bytes per complex
   size_t rsize() const { return prec == 8 ? 8 : 4; }
@@ -129,10 +147,13 @@ int ensure_force_list(psdns_ctx* c, int kfor) {
     }
   }
   if (c->d_force) HIPCHK(hipFree(c->d_force));
+  if (c->d_force_terms) HIPCHK(hipFree(c->d_force_terms));
   c->d_force = nullptr;
+  c->d_force_terms = nullptr;
   c->nforce = (int)list.size();
   if (!list.empty()) {
     HIPCHK(hipMalloc(&c->d_force, sizeof(int2) * list.size()));
+    HIPCHK(hipMalloc(&c->d_force_terms, 2 * sizeof(double) * list.size()));
     HIPCHK(hipMemcpy(c->d_force, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice));
   }
   c->c.force_list = c->d_force;
@@ -179,14 +200,27 @@ int mark(psdns_ctx* c, int i) {
   return 0;
 }
 
-// adds the time between events a and b to pass timer t (after a sync)
-int add_time(psdns_ctx* c, int a, int b, int t) {
+// adds the time between events a and b to pass t of RK substep st (after a sync)
+int add_time(psdns_ctx* c, int a, int b, int t, int st) {
   if (!c->timing) return 0;
   float ms;
   HIPCHK(hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
-  c->times[t] += ms;
+  c->tt[st][t] += ms;
   return 0;
 }
+int mark_sc(psdns_ctx* c, int i) {
+  if (c->timing) HIPCHK(hipEventRecord(c->evs[i], c->c.stream));
+  return 0;
+}
+int add_time_sc(psdns_ctx* c, int a, int b, int t, int st) {
+  if (!c->timing) return 0;
+  float ms;
+  HIPCHK(hipEventElapsedTime(&ms, c->evs[a], c->evs[b]));
+  c->tt[st][t] += ms;
+  return 0;
+}
+// passes of the timer table
+enum { kTYInv, kTZInv, kTX, kTZFwd, kTYUpd, kTScGrad, kTScZInv, kTScX, kTScZFwd, kTScYFwd, kTScUpd };
 
 // the fused forward y pass + update of a stage (timer 4, events 5 and 6);
 // inv2 also does the inverse y pass of stage 2
@@ -206,7 +240,7 @@ int flush(psdns_ctx* c) {
   int rc;
   if ((rc = y_update(c, 1, c->pending_norm, false))) return rc;
   if ((rc = sync(c))) return rc;
-  return add_time(c, 5, 6, 4);
+  return add_time(c, 5, 6, kTYUpd, 0);
 }
 
 // The scalars' part of a stage's nonlinear terms, run after the velocity
@@ -228,29 +262,77 @@ int scalar_nonlinear(psdns_ctx* c) {
     Ctx kf = k;
     kf.w1 = xg;
     kf.u = (char*)c->d_sc + (3 + i) * vb;
-    if (psdns::launch_y_grad(k, (char*)c->d_usc + i * vb, c->d_xs) || psdns::launch_pass(kd, Pass::ZInv, 3, 0) ||
-        psdns::launch_x_scalar(k, c->d_xi, xg, c->grad[i]) || psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0) ||
-        psdns::launch_pass(kf, Pass::YFwd, 1, 0))
-      return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+    int bad = 0, rc;
+    if ((rc = mark_sc(c, 6 * i))) return rc;
+    bad |= psdns::launch_y_grad(k, (char*)c->d_usc + i * vb, c->d_xs);
+    if ((rc = mark_sc(c, 6 * i + 1))) return rc;
+    bad |= psdns::launch_pass(kd, Pass::ZInv, 3, 0);
+    if ((rc = mark_sc(c, 6 * i + 2))) return rc;
+    bad |= psdns::launch_x_scalar(k, c->d_xi, xg, c->grad[i]);
+    if ((rc = mark_sc(c, 6 * i + 3))) return rc;
+    bad |= psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0);
+    if ((rc = mark_sc(c, 6 * i + 4))) return rc;
+    bad |= psdns::launch_pass(kf, Pass::YFwd, 1, 0);
+    if ((rc = mark_sc(c, 6 * i + 5))) return rc;
+    if (bad) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the scalar passes' times of substep st, after a sync
+int scalar_times(psdns_ctx* c, int st) {
+  for (int i = 0; i < c->nc; i++)
+    for (int p = 0; p < 5; p++)
+      if (int rc = add_time_sc(c, 6 * i + p, 6 * i + p + 1, kTScGrad + p, st)) return rc;
   return 0;
 }
 
 // the scalars' update of stage kstep (after the velocity's)
 int scalar_update(psdns_ctx* c, int kstep, double dt2) {
   const size_t vb = c->csize() * c->cyl_elems();
+  const int e0 = 6 * psdns_ctx::kMaxScalars;
+  int rc;
+  if ((rc = mark_sc(c, e0))) return rc;
   for (int i = 0; i < c->nc; i++)
     psdns::launch_sc_update(c->c, kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
                             (const char*)c->d_sc + (3 + i) * vb, dt2, 1.0 / c->pr[i], c->d_red);
   HIPCHK(hipGetLastError());
-  return 0;
+  return mark_sc(c, e0 + 1);
 }
 
 // a valid context with no deferred work left
 int ready(psdns_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   return flush(c);
+}
+
+// as ready, for an entry point that overwrites the work arrays (u slots,
+// W1, xi, X, ux) or the shift tables: the nonlinear terms of a preceding
+// stage_nonlinear are gone, so the next stage_update / rk4_update must
+// fail with PSDNS_ESTATE instead of updating with overwritten terms
+int ready_clobber(psdns_ctx* c) {
+  if (int rc = ready(c)) return rc;
+  c->last_nonlinear = 0;
+  return 0;
+}
+
+// for(kfor, k2fo, k2fo, 3) (FP64 complex) to the device
+int upload_forcing(psdns_ctx* c, const void* forcing) {
+  const int k2fo = 2 * c->epf_kfor - 1;
+  const size_t bytes = sizeof(double2) * 3 * (size_t)c->epf_kfor * k2fo * k2fo;
+  HIPCHK(hipMemcpyAsync(c->d_epf_for, forcing, bytes, hipMemcpyHostToDevice, c->c.stream));
+  return 0;
+}
+
+// the forcing add of mode 1 (proc4a, advanc_1: tally into d_efkz), 2
+// (proc4b) or 3 (advanc_2, advanc_3) on the uploaded increments
+int epfor(psdns_ctx* c, int mode) {
+  const int k2fo = 2 * c->epf_kfor - 1;
+  psdns::launch_epfor(c->c, mode, c->d_epf, c->n_epf, c->d_epf_for, c->epf_kfor * k2fo * k2fo, c->d_epf_terms,
+                      c->d_efkz);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 }  // namespace
@@ -328,6 +410,7 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
       return bail(fail(PSDNS_ENOMEM, "hipMalloc(%zu) failed for " #ptr, (size_t)(bytes))); \
   } while (0)
 
+  CHK(hipGetDevice(&c->dev));
   CHK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
   const size_t cyl = c->cyl_elems();
   ALLOC(c->d_colx, sizeof(int) * c->na);
@@ -412,6 +495,8 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   k.force_list = nullptr;
   k.nforce = 0;
   for (auto& e : c->ev) CHK(hipEventCreate(&e));
+  c->evs.assign(6 * psdns_ctx::kMaxScalars + 2, nullptr);
+  for (auto& e : c->evs) CHK(hipEventCreate(&e));
   CHK(hipStreamSynchronize(k.stream));
 #undef CHK
 #undef ALLOC
@@ -427,11 +512,14 @@ int psdns_destroy(psdns_ctx* c) {
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
-                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc, c->d_u1};
+                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc, c->d_u1, c->d_epf, c->d_epf_for,
+                  c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_fac) (void)hipHostFree(c->h_fac);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->evs)
     if (e) (void)hipEventDestroy(e);
   if (c->c.stream) (void)hipStreamDestroy(c->c.stream);
   delete c;
@@ -454,7 +542,7 @@ int psdns_columns(const psdns_ctx* c, int* col_x, int* col_z, int* cut_z) {
 }
 
 int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (!gsh) return fail(PSDNS_EINVAL, "null gsh");
   // shifts.f:146-178: exp(i 2 pi k gsh(dir, stage) / n), formed on the
   // device in FP64 (and rounded to the state precision for the passes)
@@ -516,8 +604,10 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   }
   if ((rc = sync(c))) return rc;
   for (int i = 0; i < 4; i++)
-    if ((rc = add_time(c, i, i + 1, i))) return rc;
-  if (fused && (rc = add_time(c, 5, 6, 4))) return rc;
+    if ((rc = add_time(c, i, i + 1, kTYInv + i, kstep - 1))) return rc;
+  // the fused kernel is stage 1's update (with stage 2's y inverse)
+  if (fused && (rc = add_time(c, 5, 6, kTYUpd, 0))) return rc;
+  if (c->nc > 0 && (rc = scalar_times(c, kstep - 1))) return rc;
   c->last_nonlinear = kstep;
   return PSDNS_OK;
 }
@@ -526,7 +616,8 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
                        const double* zfac, const void* forcing) {
   if (int rc = ready(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d: RK2 has stages 1 and 2", kstep);
-  if (forcing) return fail(PSDNS_EINVAL, "EPFOR forcing increments are not supported (pass NULL)");
+  if (forcing && c->epf_kfor == 0)
+    return fail(PSDNS_EINVAL, "EPFOR forcing increments need psdns_set_epfor (kfor, kforce) first");
   if (c->last_nonlinear != kstep)
     return fail(PSDNS_ESTATE, "stage_update(%d) without a preceding stage_nonlinear(%d)", kstep, kstep);
   if (!(dt > 0.0)) return fail(PSDNS_EINVAL, "dt = %g must be positive", dt);
@@ -538,16 +629,29 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
   const double dt2 = dt / 2.0;
   const double norm = dt2 / c->n / c->n;  // wavespace_vel.f:80
   c->last_nonlinear = 0;
-  if (kstep == 1 && c->nc == 0) {
-    // deferred (the factors are on the device already)
+  if (kstep == 1 && c->nc == 0 && !forcing) {
+    // deferred (the factors are on the device already); proc4a zeroes
+    // efkz when forcing is on (proc4a.f:66)
+    if (c->epf_kfor) HIPCHK(hipMemsetAsync(c->d_efkz, 0, sizeof(double) * 3 * c->nxh, c->c.stream));
     c->pending_update = true;
     c->pending_norm = norm;
     return sync(c);
   }
+  if (forcing && (rc = upload_forcing(c, forcing))) return rc;
   if ((rc = y_update(c, kstep, norm, false))) return rc;
+  // EPFOR after the projection (proc4a.f:588-640 with the tally, proc4b.f:
+  // 118-163 before un <- u)
+  if (forcing) {
+    if ((rc = epfor(c, kstep))) return rc;
+  } else if (kstep == 1 && c->epf_kfor) {
+    HIPCHK(hipMemsetAsync(c->d_efkz, 0, sizeof(double) * 3 * c->nxh, c->c.stream));
+  }
   if (c->nc > 0 && (rc = scalar_update(c, kstep, dt2))) return rc;
   if ((rc = sync(c))) return rc;
-  return add_time(c, 5, 6, 4);
+  if (c->nc > 0 && (rc = add_time_sc(c, 6 * psdns_ctx::kMaxScalars, 6 * psdns_ctx::kMaxScalars + 1, kTScUpd,
+                                     kstep - 1)))
+    return rc;
+  return add_time(c, 5, 6, kTYUpd, kstep - 1);
 }
 
 int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, const double* yfac,
@@ -574,16 +678,29 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
   // (wavespace_vel.f:80) and advanc_kstep in one kernel, from W1
   if (psdns::launch_y_rk4(c->c, kstep, dt / 2.0 / c->n / c->n, c->d_u1) != 0)
     return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  // EPFOR (advanc_1.f:165-210 with the tally, advanc_2.f, advanc_3.f): the
+  // increments added to u; advanc_4's add is overwritten by u1 on the mask
+  // (advanc_4.f:96-170), so substep 4 has none
+  if (c->rk4_forcing && kstep < 4) {
+    if ((rc = epfor(c, kstep == 1 ? 1 : 3))) return rc;
+  } else if (kstep == 1 && c->epf_kfor) {
+    HIPCHK(hipMemsetAsync(c->d_efkz, 0, sizeof(double) * 3 * c->nxh, c->c.stream));
+  }
   // the scalars' advanc_kstep (the same loops over i = 4 .. 3 + nc), from
   // their transformed advection terms of stage_nonlinear
+  if ((rc = mark(c, 6))) return rc;
+  if ((rc = mark_sc(c, 6 * psdns_ctx::kMaxScalars))) return rc;
   for (int i = 0; i < c->nc; i++)
     psdns::launch_sc_update(c->c, 10 + kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
                             (const char*)c->d_sc + (3 + i) * vb, dt / 2.0, 1.0 / c->pr[i], c->d_red,
                             (char*)c->d_u1 + (3 + i) * vb);
   HIPCHK(hipGetLastError());
-  if ((rc = mark(c, 6))) return rc;
+  if ((rc = mark_sc(c, 6 * psdns_ctx::kMaxScalars + 1))) return rc;
   if ((rc = sync(c))) return rc;
-  return add_time(c, 5, 6, 4);
+  if (c->nc > 0 && (rc = add_time_sc(c, 6 * psdns_ctx::kMaxScalars, 6 * psdns_ctx::kMaxScalars + 1, kTScUpd,
+                                     kstep - 1)))
+    return rc;
+  return add_time(c, 5, 6, kTYUpd, kstep - 1);
 }
 
 int psdns_stage_times(psdns_ctx* c, int enable, double* times) {
@@ -591,13 +708,97 @@ int psdns_stage_times(psdns_ctx* c, int enable, double* times) {
   if (times) {
     double tot = 0;
     for (int i = 0; i < 5; i++) {
-      times[i] = c->times[i];
-      tot += c->times[i];
+      times[i] = 0;
+      for (int st = 0; st < psdns_ctx::kStages; st++) times[i] += c->tt[st][i];
+      tot += times[i];
     }
     times[5] = tot;
   }
-  for (double& t : c->times) t = 0;
+  for (auto& row : c->tt)
+    for (double& t : row) t = 0;
   c->timing = enable != 0;
+  return PSDNS_OK;
+}
+
+int psdns_pass_times(psdns_ctx* c, int enable, double* times, int ntimes) {
+  if (int rc = ready(c)) return rc;
+  constexpr int kN = psdns_ctx::kStages * psdns_ctx::kPasses;
+  if (times) {
+    if (ntimes < kN) return fail(PSDNS_EINVAL, "ntimes = %d: the table has %d entries", ntimes, kN);
+    for (int st = 0; st < psdns_ctx::kStages; st++)
+      for (int p = 0; p < psdns_ctx::kPasses; p++) times[st * psdns_ctx::kPasses + p] = c->tt[st][p];
+  }
+  for (auto& row : c->tt)
+    for (double& t : row) t = 0;
+  c->timing = enable != 0;
+  return PSDNS_OK;
+}
+
+int psdns_set_epfor(psdns_ctx* c, int kfor, double kforce) {
+  if (int rc = ready(c)) return rc;
+  if (kfor < 0 || kfor > c->n / 2) return fail(PSDNS_EINVAL, "kfor = %d: need 0 (off) .. n/2", kfor);
+  if (kfor > 0 && !(kforce >= 0.0)) return fail(PSDNS_EINVAL, "kforce = %g must be >= 0", kforce);
+  for (void* p : {(void*)c->d_epf, (void*)c->d_epf_for, (void*)c->d_epf_terms, (void*)c->d_efkz})
+    if (p) HIPCHK(hipFree(p));
+  c->d_epf = nullptr;
+  c->d_epf_for = nullptr;
+  c->d_epf_terms = nullptr;
+  c->d_efkz = nullptr;
+  c->epf_kfor = 0;
+  c->n_epf = 0;
+  c->rk4_forcing = false;
+  if (kfor == 0) return PSDNS_OK;
+  // the mask entries of the box kx < kfor, |ky| < kfor, |kz| < kfor in (a, y)
+  // order, with their index into for(kfor, k2fo, k2fo, 3) (proc4a.f:596-626)
+  // and the shell of those inside |k| <= kforce (proc4a.f:633-635)
+  const int n = c->n, k2fo = 2 * kfor - 1;
+  std::vector<int4> list;
+  for (int a = 0; a < c->na; a++) {
+    const int x = c->colx[a], z = c->colz[a];
+    const int iz = z + 1 <= kfor ? z + 1 : (z + 1 >= n + 2 - kfor ? z + 1 - n + k2fo : 0);
+    if (iz == 0 || x + 1 > kfor) continue;
+    for (int y = 0; y < n; y++) {
+      const int ky = kwave(y, n), kz = kwave(z, n);
+      if (9 * (x * x + ky * ky + kz * kz) > 2 * n * n || y == n / 2 || z == n / 2) continue;
+      const int iy = y + 1 <= kfor ? y + 1 : (y + 1 >= n + 2 - kfor ? y + 1 - n + k2fo : 0);
+      if (iy == 0) continue;
+      const double rks = (double)x * x + (double)ky * ky + (double)kz * kz;
+      const int shell = rks <= kforce * kforce ? (int)(std::sqrt(rks) + 1.5) : 0;
+      list.push_back(make_int4(a, y, ((iz - 1) * k2fo + (iy - 1)) * kfor + x, shell));
+    }
+  }
+  c->n_epf = (int)list.size();
+  const size_t nfor = 3 * (size_t)kfor * k2fo * k2fo;
+  if (hipMalloc(&c->d_epf, sizeof(int4) * std::max<size_t>(list.size(), 1)) != hipSuccess ||
+      hipMalloc(&c->d_epf_for, sizeof(double2) * nfor) != hipSuccess ||
+      hipMalloc(&c->d_epf_terms, sizeof(double) * 3 * std::max<size_t>(list.size(), 1)) != hipSuccess ||
+      hipMalloc(&c->d_efkz, sizeof(double) * 3 * c->nxh) != hipSuccess)
+    return fail(PSDNS_ENOMEM, "hipMalloc of the EPFOR tables failed");
+  if (!list.empty())
+    HIPCHK(hipMemcpy(c->d_epf, list.data(), sizeof(int4) * list.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_efkz, 0, sizeof(double) * 3 * c->nxh));
+  c->epf_kfor = kfor;
+  c->epf_kforce = kforce;
+  return PSDNS_OK;
+}
+
+int psdns_epfor_tally(psdns_ctx* c, double* efkz) {
+  if (int rc = ready(c)) return rc;
+  if (!efkz) return fail(PSDNS_EINVAL, "null efkz");
+  if (!c->epf_kfor) return fail(PSDNS_EINVAL, "EPFOR forcing is not set (psdns_set_epfor)");
+  HIPCHK(hipMemcpyAsync(efkz, c->d_efkz, sizeof(double) * 3 * c->nxh, hipMemcpyDeviceToHost, c->c.stream));
+  return sync(c);
+}
+
+int psdns_rk4_forcing(psdns_ctx* c, const void* forcing) {
+  if (int rc = ready(c)) return rc;
+  if (forcing && !c->epf_kfor)
+    return fail(PSDNS_EINVAL, "EPFOR forcing increments need psdns_set_epfor (kfor, kforce) first");
+  c->rk4_forcing = forcing != nullptr;
+  if (forcing) {
+    if (int rc = upload_forcing(c, forcing)) return rc;
+    return sync(c);
+  }
   return PSDNS_OK;
 }
 
@@ -606,8 +807,7 @@ int psdns_ek_shell(psdns_ctx* c, int kfor, double* ek) {
   if (kfor < 1 || kfor > 64 || !ek) return fail(PSDNS_EINVAL, "kfor = %d: need 1..64 and an output array", kfor);
   int rc;
   if ((rc = ensure_force_list(c, kfor))) return rc;
-  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(double) * kfor, c->c.stream));
-  if (c->nforce) psdns::launch_ek_shell(c->c, kfor, c->d_red);
+  psdns::launch_ek_shell(c->c, kfor, c->d_red, c->d_force_terms);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(ek, c->d_red, sizeof(double) * kfor, hipMemcpyDeviceToHost, c->c.stream));
   return sync(c);
@@ -634,31 +834,51 @@ int psdns_ek_force(psdns_ctx* c, int kf_shell, int kfor, const double* ekinf, do
   return PSDNS_OK;
 }
 
-int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* tke, double* eps) {
-  if (int rc = ready(c)) return rc;
-  if (nshell < 0 || nshell > c->n) return fail(PSDNS_EINVAL, "nshell = %d out of range", nshell);
-  if (nshell > 0 && (!ek || !dk)) return fail(PSDNS_EINVAL, "null spectrum arrays");
-  double* d_tot = c->d_red;
-  double* d_ek = c->d_red + 2;
-  double* d_dk = c->d_red + 2 + c->n;
-  HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(double) * (2 + 2 * (size_t)c->n), c->c.stream));
-  psdns::launch_spectrum(c->c, c->viscos, nshell, d_ek, d_dk, d_tot);
+}  // extern "C"
+
+namespace {
+
+// shell statistics into part (partial rows) and out (2 nshell + 2 doubles),
+// then to the host
+int shell_stats_into(psdns_ctx* c, double* part, double* out, double* ek, double* dk, int nshell, double* tke,
+                     double* eps) {
+  psdns::launch_shell_stats(c->c, c->viscos, nshell, part, out);
   HIPCHK(hipGetLastError());
   double tot[2];
-  HIPCHK(hipMemcpyAsync(tot, d_tot, sizeof tot, hipMemcpyDeviceToHost, c->c.stream));
+  HIPCHK(hipMemcpyAsync(tot, out + 2 * nshell, sizeof tot, hipMemcpyDeviceToHost, c->c.stream));
   if (nshell > 0) {
-    HIPCHK(hipMemcpyAsync(ek, d_ek, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
-    HIPCHK(hipMemcpyAsync(dk, d_dk, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
+    HIPCHK(hipMemcpyAsync(ek, out, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
+    HIPCHK(hipMemcpyAsync(dk, out + nshell, sizeof(double) * nshell, hipMemcpyDeviceToHost, c->c.stream));
   }
-  int rc;
-  if ((rc = sync(c))) return rc;
+  if (int rc = sync(c)) return rc;
   if (tke) *tke = tot[0];
   if (eps) *eps = tot[1];
   return PSDNS_OK;
 }
 
-int psdns_vg_moments(psdns_ctx* c, double* dudx, double* dwdz) {
+int shell_args(const psdns_ctx* c, double* ek, double* dk, int nshell) {
+  if (nshell < 0 || nshell > c->n) return fail(PSDNS_EINVAL, "nshell = %d out of range", nshell);
+  if (nshell > 0 && (!ek || !dk)) return fail(PSDNS_EINVAL, "null spectrum arrays");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* tke, double* eps) {
   if (int rc = ready(c)) return rc;
+  if (int rc = shell_args(c, ek, dk, nshell)) return rc;
+  if (!c->d_shell_part) {
+    const size_t bytes = sizeof(double) * psdns::shell_part_doubles(c->n);
+    if (hipMalloc(&c->d_shell_part, bytes) != hipSuccess)
+      return fail(PSDNS_ENOMEM, "hipMalloc of the shell partial rows (%zu bytes) failed", bytes);
+  }
+  return shell_stats_into(c, c->d_shell_part, c->d_red, ek, dk, nshell, tke, eps);
+}
+
+int psdns_vg_moments(psdns_ctx* c, double* dudx, double* dwdz) {
+  if (int rc = ready_clobber(c)) return rc;
   if (!dudx || !dwdz) return fail(PSDNS_EINVAL, "null moment arrays");
   int rc;
   if ((rc = ensure_ux(c))) return rc;
@@ -703,7 +923,7 @@ int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double
 }
 
 int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (!host) return fail(PSDNS_EINVAL, "null host array");
   size_t bytes;
   void* dst;
@@ -747,7 +967,7 @@ int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
 }
 
 int psdns_phase1(psdns_ctx* c, int kstep) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_phshift(c->c, 3);
@@ -755,7 +975,7 @@ int psdns_phase1(psdns_ctx* c, int kstep) {
 }
 
 int psdns_itransform_vel(psdns_ctx* c) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::YInvPlain, 3))) return rc;
@@ -765,7 +985,7 @@ int psdns_itransform_vel(psdns_ctx* c) {
 }
 
 int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
@@ -776,7 +996,7 @@ int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
 }
 
 int psdns_transform(psdns_ctx* c) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::XR2C, 5))) return rc;
@@ -786,7 +1006,7 @@ int psdns_transform(psdns_ctx* c) {
 }
 
 int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_wavespace(c->c, dt2 / c->n / c->n);
@@ -794,7 +1014,7 @@ int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
 }
 
 int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   int rc;
   if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
   psdns::launch_update(c->c, 1);
@@ -802,13 +1022,13 @@ int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const dou
 }
 
 int psdns_proc4b(psdns_ctx* c) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   psdns::launch_update(c->c, 2);
   return sync(c);
 }
 
 int psdns_kxtran(psdns_ctx* c, int nv) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;
@@ -819,7 +1039,7 @@ int psdns_kxtran(psdns_ctx* c, int nv) {
 }
 
 int psdns_xktran(psdns_ctx* c, int nv) {
-  if (int rc = ready(c)) return rc;
+  if (int rc = ready_clobber(c)) return rc;
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;

@@ -86,9 +86,23 @@ void launch_update(const Ctx& c, int stage);
 void launch_wavespace_update(const Ctx& c, double norm, int stage);
 // RK4 substep k = 1..4 in the fused y forward kernel (W1 -> u, un, u1); -1 for an unsupported grid
 int launch_y_rk4(const Ctx& c, int k, double norm, void* u1);
-void launch_ek_shell(const Ctx& c, int kfor, double* d_ek);
+// FEK shell energies of the force list: per-entry terms (2 doubles each)
+// summed per shell in list order
+void launch_ek_shell(const Ctx& c, int kfor, double* d_ek, double* terms);
 void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
-void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot);
+// FP64 shell statistics in a fixed order: kShellBlocks partial rows of
+// 2 shell_rows(n) doubles in part, then out[0..nshell) = E,
+// out[nshell..2 nshell) = D, out[2 nshell] = TKE, out[2 nshell + 1] = eps
+constexpr int kShellBlocks = 1024;
+// shells a mask mode can reach: ik = int(|k| + 1.5) with 9 |k|^2 <= 2 n^2
+inline int shell_rows(int n) {
+  int k2max = 2 * n * n / 9;
+  int r = 0;
+  while ((long long)(r + 1) * (r + 1) <= k2max) r++;  // floor(sqrt(k2max))
+  return r + 2;
+}
+inline size_t shell_part_doubles(int n) { return (size_t)kShellBlocks * 2 * shell_rows(n); }
+void launch_shell_stats(const Ctx& c, double viscos, int nshell, double* part, double* out);
 void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0);
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
 // phase-shift tables of both stages from gsh(3,2): FP64 s64 and state
@@ -109,6 +123,11 @@ void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]
 void launch_vg_deriv(const Ctx& c, void* d);
+// EPFOR forcing increments on u (and un): mode 1 proc4a/advanc_1 (sphere,
+// per-entry tally terms[3 nlist], then efkz[3 nxh] folded in list order),
+// 2 proc4b, 3 advanc_2/advanc_3; list entries (a, y, for index, shell | 0)
+void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
+                  double* terms, double* efkz);
 void launch_moments(const Ctx& c, double* d_out);
 
 }  // namespace psdns

@@ -1486,24 +1486,36 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
 // FEK forcing over a short list of (a, y) entries with |k| < kf_shell + 1.5
 // (ek_shell.f:131-168, ek_force.f:47-66), FP64 sums.
 // ---------------------------------------------------------------------------
+// per entry of the list: terms[2 i] = tfact |u|^2 / 2, terms[2 i + 1] = its
+// shell (0 past kfor); k_ek_fold sums each shell in list order
 template <typename R>
-__global__ void k_ek_shell(const C<R>* __restrict__ un, Geo g, const int2* __restrict__ list, int nlist,
-                           int kfor, double* __restrict__ ek) {
+__global__ void __launch_bounds__(256)
+    k_ek_terms(const C<R>* __restrict__ un, Geo g, const int2* __restrict__ list, int nlist, int kfor,
+               double* __restrict__ terms) {
   const size_t tot = (size_t)g.na * g.n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nlist; i += gridDim.x * blockDim.x) {
     const int a = list[i].x, y = list[i].y;
     const int x = g.colx[a], z = g.colz[a];
     const double kx = x, ky = kwave(y, g.n), kz = kwave(z, g.n);
     const int ik = (int)(sqrt(kx * kx + ky * ky + kz * kz) + 1.5);
-    if (ik > kfor) continue;
     const size_t j = (size_t)a * g.n + y;
     double s = 0;
     for (int v = 0; v < 3; v++) {
       const C<R> q = un[v * tot + j];
       s += (double)q.x * q.x + (double)q.y * q.y;
     }
-    atomicAdd(ek + ik - 1, s / 2.0 * (x == 0 ? 1.0 : 2.0));
+    terms[2 * (size_t)i] = ik <= kfor ? s / 2.0 * (x == 0 ? 1.0 : 2.0) : 0.0;
+    terms[2 * (size_t)i + 1] = ik <= kfor ? (double)ik : 0.0;
   }
+}
+__global__ void __launch_bounds__(64)
+    k_ek_fold(const double* __restrict__ terms, int nlist, int kfor, double* __restrict__ ek) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > kfor) return;
+  double s = 0;
+  for (int i = 0; i < nlist; i++)
+    if (terms[2 * (size_t)i + 1] == (double)k) s += terms[2 * (size_t)i];
+  ek[k - 1] = s;
 }
 
 template <typename R>
@@ -1525,39 +1537,195 @@ __global__ void k_ek_scale(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, c
   }
 }
 
-// sptr.f:139-176 style FP64 shell sums: E(ik), D(ik) = 2 nu k^2 E, per block
-// partials reduced with FP64 atomics.
-template <typename R>
+// ---------------------------------------------------------------------------
+// EPFOR forcing increments (proc4a.f:588-640, proc4b.f:118-163,
+// advanc_1.f:165-210, advanc_2.f/advanc_3.f): list entry e = (a, y, index of
+// for(x, iy, iz, 1) in the for(kfor, k2fo, k2fo, 3) array, shell k of the
+// entry inside |k| <= kforce or 0 outside), in (a, y) order; forc holds the
+// three components fstride apart (FP64 complex, as the caller's array).
+// MODE 1 (proc4a, advanc_1): inside the sphere u += for and
+//        terms[3 e + i] = 2 Re(u conj for) tfact (u before the add), else 0
+// MODE 2 (proc4b): u += for in the whole box, un = u; k = 0 left zero (the
+//        zero mean that follows in proc4b)
+// MODE 3 (advanc_2, advanc_3): u += for in the whole box
+// The sum is formed in FP64 and rounded to the state precision, as the
+// reference adds its complex(8) for to uy.
+// ---------------------------------------------------------------------------
+template <typename R, int MODE>
 __global__ void __launch_bounds__(256)
-    k_spectrum(const C<R>* __restrict__ un, Geo g, double viscos, int nshell, double* __restrict__ ek,
-               double* __restrict__ dk, double* __restrict__ tot_out) {
+    k_epfor(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, const int4* __restrict__ list, int nlist,
+            const double2* __restrict__ forc, int fstride, double* __restrict__ terms) {
   const size_t tot = (size_t)g.na * g.n;
-  double te = 0, td = 0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const Pt p = point(i, g);
-    if (!p.m) continue;
-    double s = 0;
-    for (int v = 0; v < 3; v++) {
-      const C<R> q = un[v * tot + i];
-      s += (double)q.x * q.x + (double)q.y * q.y;
-    }
-    const double rk2 = (double)p.x * p.x + (double)p.ky * p.ky + (double)p.kz * p.kz;
-    const double en = 0.5 * (p.x == 0 ? 1.0 : 2.0) * s;
-    const int ik = (int)(sqrt(rk2) + 1.5);
-    te += en;
-    td += 2.0 * viscos * rk2 * en;
-    if (ik <= nshell) {
-      atomicAdd(ek + ik - 1, en);
-      atomicAdd(dk + ik - 1, 2.0 * viscos * rk2 * en);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nlist; e += gridDim.x * blockDim.x) {
+    const int4 en = list[e];
+    const size_t j = (size_t)en.x * g.n + en.y;
+    const bool k0 = en.x == 0 && en.y == 0 && g.colx[0] == 0 && g.colz[0] == 0;
+    const double tf = g.colx[en.x] == 0 ? 1.0 : 2.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double2 f = forc[(size_t)i * fstride + en.z];
+      C<R>* p = u + i * tot + j;
+      const C<R> v = *p;
+      const C<R> s = {(R)((double)v.x + f.x), (R)((double)v.y + f.y)};
+      if constexpr (MODE == 1) {
+        double t = 0;
+        if (en.w > 0) {
+          t = 2.0 * ((double)v.x * f.x + (double)v.y * f.y) * tf;
+          *p = s;
+        }
+        terms[3 * (size_t)e + i] = t;
+      } else if constexpr (MODE == 2) {
+        if (!k0) {
+          *p = s;
+          un[i * tot + j] = s;
+        }
+      } else {
+        *p = s;
+      }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    te += __shfl_xor(te, off);
-    td += __shfl_xor(td, off);
+}
+
+// efkz(k, i) of proc4a.f:640 as efkz[i * nxh + k - 1]: the entries' terms of
+// shell k summed in list order (fixed order: reproducible FP64 sums)
+__global__ void __launch_bounds__(256)
+    k_epfor_fold(const int4* __restrict__ list, int nlist, const double* __restrict__ terms, int nxh,
+                 double* __restrict__ efkz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nxh) return;
+  const int i = t / nxh, k = t % nxh + 1;
+  double s = 0;
+  for (int e = 0; e < nlist; e++)
+    if (list[e].w == k) s += terms[3 * (size_t)e + i];
+  efkz[t] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Shell sums in a fixed order (sptr.f:139-176,235): E(ik) = sum tfact |u|^2/2,
+// D(ik) = 2 nu k^2 E over the mask, ik = int(|k| + 1.5), FP64, with the same
+// bits on every run.  Work-group b owns the fixed column range
+// [b na / B, (b + 1) na / B); each wave takes one column at a time and walks
+// its two y halves (ky >= 0 ascending, then ky < 0 with |ky| descending), in
+// which the shell index is monotone, so the lanes of one shell are
+// contiguous: a segmented scan over the 64 lanes sums each shell's run in a
+// fixed tree, and only the run's last lane adds it to the wave's own LDS
+// histogram (one writer per shell per instruction, in program order).  The
+// work-group then adds its wave histograms in wave order into its partial
+// row part[b][0..nh) (E) and part[b][nh..2 nh) (D); k_shell_fold sums the
+// rows in block order.  No atomics: the result depends on B only.
+// ---------------------------------------------------------------------------
+constexpr int kShellWaves = 8;      // waves per work-group of k_shell_partial
+constexpr int kShellChunks = 8;     // 64-entry chunks of a column half per pass
+
+template <typename R>
+__global__ void __launch_bounds__(64 * kShellWaves)
+    k_shell_partial(const C<R>* __restrict__ un, Geo g, double viscos, int nh, double* __restrict__ part) {
+  extern __shared__ double hist[];  // [kShellWaves][2][nh]
+  const int n = g.n, nb = gridDim.x, b = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kShellWaves * 2 * nh; i += blockDim.x) hist[i] = 0.0;
+  __syncthreads();
+  double* he = hist + (size_t)w * 2 * nh;
+  double* hd = he + nh;
+  const size_t tot = (size_t)g.na * n;
+  const int a0 = (int)((long long)b * g.na / nb), a1 = (int)((long long)(b + 1) * g.na / nb);
+  for (int a = a0 + w; a < a1; a += kShellWaves) {
+    const int x = g.colx[a], z = g.colz[a], kz = kwave(z, n);
+    const double tf = x == 0 ? 1.0 : 2.0;
+    const BufView<R> s0(un + (size_t)a * n, kOob), s1(un + tot + (size_t)a * n, kOob),
+        s2(un + 2 * tot + (size_t)a * n, kOob);
+    for (int half = 0; half < 2; half++) {
+      // half 0: y = 0 .. n/2 - 1; half 1: y = n/2 + 1 .. n - 1
+      const int ylo = half == 0 ? 0 : n / 2 + 1, yhi = half == 0 ? n / 2 : n;
+      for (int y0 = ylo; y0 < yhi; y0 += 64 * kShellChunks) {
+        C<R> v[kShellChunks][3];
+#pragma unroll
+        for (int c = 0; c < kShellChunks; c++) {
+          const int y = y0 + 64 * c + lane;
+          const unsigned off = y < yhi ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob;
+          v[c][0] = s0.load(off);
+          v[c][1] = s1.load(off);
+          v[c][2] = s2.load(off);
+        }
+#pragma unroll
+        for (int c = 0; c < kShellChunks; c++) {
+          if (y0 + 64 * c >= yhi) break;  // wave-uniform
+          const int y = y0 + 64 * c + lane, ky = kwave(y, n);
+          const bool on = y < yhi && in_mask(x, ky, kz, y, z, n);
+          const double rk2 = (double)x * x + (double)ky * ky + (double)kz * kz;
+          int key = on ? (int)(sqrt(rk2) + 1.5) : -1;
+          double e = 0.0;
+          if (on) {
+            double sum = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) sum += (double)v[c][i].x * v[c][i].x + (double)v[c][i].y * v[c][i].y;
+            e = 0.5 * tf * sum;
+          }
+          double d = 2.0 * viscos * rk2 * e;
+          // segmented inclusive scan: runs of equal keys are contiguous
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const int ku = __shfl_up(key, o);
+            const double eu = __shfl_up(e, o), du = __shfl_up(d, o);
+            if (lane >= o && ku == key) {
+              e += eu;
+              d += du;
+            }
+          }
+          const int kn = __shfl_down(key, 1);
+          if (key > 0 && (lane == 63 || kn != key)) {
+            he[key - 1] += e;
+            hd[key - 1] += d;
+          }
+        }
+      }
+    }
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(tot_out, te);
-    atomicAdd(tot_out + 1, td);
+  __syncthreads();
+  for (int s = threadIdx.x; s < nh; s += blockDim.x) {
+    double e = 0.0, d = 0.0;
+#pragma unroll
+    for (int i = 0; i < kShellWaves; i++) {
+      e += hist[(size_t)i * 2 * nh + s];
+      d += hist[(size_t)i * 2 * nh + nh + s];
+    }
+    part[(size_t)b * 2 * nh + s] = e;
+    part[(size_t)b * 2 * nh + nh + s] = d;
+  }
+}
+
+// out[0..nshell) = E, out[nshell..2 nshell) = D, out[2 nshell] = TKE,
+// out[2 nshell + 1] = epsilon: the partial rows summed in block order, the
+// totals over all nh shells in shell order.  One work-group.
+__global__ void __launch_bounds__(256)
+    k_shell_fold(const double* __restrict__ part, int nb, int nh, int nshell, double* __restrict__ out) {
+  extern __shared__ double fold[];  // [2][nh]
+  for (int s = threadIdx.x; s < nh; s += blockDim.x) {
+    double e = 0.0, d = 0.0;
+    for (int b = 0; b < nb; b++) {
+      e += part[(size_t)b * 2 * nh + s];
+      d += part[(size_t)b * 2 * nh + nh + s];
+    }
+    fold[s] = e;
+    fold[nh + s] = d;
+    if (s < nshell) {
+      out[s] = e;
+      out[nshell + s] = d;
+    }
+  }
+  for (int s = nh + threadIdx.x; s < nshell; s += blockDim.x) {
+    out[s] = 0.0;
+    out[nshell + s] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0, e = 0.0;
+    for (int s = 0; s < nh; s++) {
+      t += fold[s];
+      e += fold[nh + s];
+    }
+    out[2 * nshell] = t;
+    out[2 * nshell + 1] = e;
   }
 }
 
@@ -1963,18 +2131,22 @@ struct LaunchP {
                          (CR*)c.u, (CR*)c.un, c.geo, norm, c.sx[c.kstep], c.sy[c.kstep], c.sz[c.kstep], c.xfac,
                          c.yfac, c.zfac);
   }
-  static void ek_shell(const Ctx& c, int kfor, double* d_ek) {
-    hipLaunchKernelGGL((k_ek_shell<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (const CR*)c.un,
-                       c.geo, c.force_list, c.nforce, kfor, d_ek);
+  static void ek_shell(const Ctx& c, int kfor, double* d_ek, double* terms) {
+    if (c.nforce > 0)
+      hipLaunchKernelGGL((k_ek_terms<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (const CR*)c.un,
+                         c.geo, c.force_list, c.nforce, kfor, terms);
+    hipLaunchKernelGGL(k_ek_fold, dim3((kfor + 63) / 64), dim3(64), 0, c.stream, terms, c.nforce, kfor, d_ek);
   }
   static void ek_scale(const Ctx& c, int kf_shell, const double* d_ratio) {
     hipLaunchKernelGGL((k_ek_scale<R>), dim3(grid1d(c.nforce, 256)), dim3(256), 0, c.stream, (CR*)c.u,
                        (CR*)c.un, c.geo, c.force_list, c.nforce, kf_shell, d_ratio);
   }
-  static void spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot) {
-    size_t tot = (size_t)c.geo.na * c.geo.n;
-    hipLaunchKernelGGL((k_spectrum<R>), dim3(grid1d(tot, 256 * 16)), dim3(256), 0, c.stream, (const CR*)c.un,
-                       c.geo, viscos, nshell, d_ek, d_dk, d_tot);
+  static void shell_stats(const Ctx& c, double viscos, int nshell, double* part, double* out) {
+    const int nh = shell_rows(c.geo.n);
+    hipLaunchKernelGGL((k_shell_partial<R>), dim3(kShellBlocks), dim3(64 * kShellWaves),
+                       sizeof(double) * kShellWaves * 2 * nh, c.stream, (const CR*)c.un, c.geo, viscos, nh, part);
+    hipLaunchKernelGGL(k_shell_fold, dim3(1), dim3(256), sizeof(double) * 2 * nh, c.stream, part, kShellBlocks, nh,
+                       nshell, out);
   }
   static void field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0) {
     size_t tot = (size_t)c.geo.na * c.geo.n;
@@ -2096,17 +2268,17 @@ void launch_wavespace_update(const Ctx& c, double norm, int stage) {
   if (c.prec == 8) LaunchP<double>::wavespace_update(c, norm, stage);
   else LaunchP<float>::wavespace_update(c, norm, stage);
 }
-void launch_ek_shell(const Ctx& c, int kfor, double* d_ek) {
-  if (c.prec == 8) LaunchP<double>::ek_shell(c, kfor, d_ek);
-  else LaunchP<float>::ek_shell(c, kfor, d_ek);
+void launch_ek_shell(const Ctx& c, int kfor, double* d_ek, double* terms) {
+  if (c.prec == 8) LaunchP<double>::ek_shell(c, kfor, d_ek, terms);
+  else LaunchP<float>::ek_shell(c, kfor, d_ek, terms);
 }
 void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio) {
   if (c.prec == 8) LaunchP<double>::ek_scale(c, kf_shell, d_ratio);
   else LaunchP<float>::ek_scale(c, kf_shell, d_ratio);
 }
-void launch_spectrum(const Ctx& c, double viscos, int nshell, double* d_ek, double* d_dk, double* d_tot) {
-  if (c.prec == 8) LaunchP<double>::spectrum(c, viscos, nshell, d_ek, d_dk, d_tot);
-  else LaunchP<float>::spectrum(c, viscos, nshell, d_ek, d_dk, d_tot);
+void launch_shell_stats(const Ctx& c, double viscos, int nshell, double* part, double* out) {
+  if (c.prec == 8) LaunchP<double>::shell_stats(c, viscos, nshell, part, out);
+  else LaunchP<float>::shell_stats(c, viscos, nshell, part, out);
 }
 void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0) {
   if (c.prec == 8) LaunchP<double>::field(c, seed, kpeak, d_colz0);
@@ -2198,6 +2370,32 @@ void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr
   }
   if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, dif, u1);
   else sc_update_r<float>(c, stage, u, un, tr, dt2, dif, u1);
+}
+template <typename R>
+static void epfor_r(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
+                    double* terms) {
+  const dim3 grid(grid1d((size_t)nlist, 256)), block(256);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_epfor<R, 1>), grid, block, 0, c.stream, (C<R>*)c.u, (C<R>*)c.un, c.geo, list, nlist,
+                       forc, fstride, terms);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_epfor<R, 2>), grid, block, 0, c.stream, (C<R>*)c.u, (C<R>*)c.un, c.geo, list, nlist,
+                       forc, fstride, terms);
+  else
+    hipLaunchKernelGGL((k_epfor<R, 3>), grid, block, 0, c.stream, (C<R>*)c.u, (C<R>*)c.un, c.geo, list, nlist,
+                       forc, fstride, terms);
+}
+void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
+                  double* terms, double* efkz) {
+  if (nlist > 0) {
+    if (c.prec == 8) epfor_r<double>(c, mode, list, nlist, forc, fstride, terms);
+    else epfor_r<float>(c, mode, list, nlist, forc, fstride, terms);
+  }
+  if (mode == 1) {
+    const int nt = 3 * c.geo.nxh;
+    hipLaunchKernelGGL(k_epfor_fold, dim3((nt + 255) / 256), dim3(256), 0, c.stream, list, nlist, terms, c.geo.nxh,
+                       efkz);
+  }
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);

@@ -112,6 +112,28 @@ module psdns_gpu
       real(c_double), intent(in) :: xfac(*), yfac(*), zfac(*)
     end function psdns_rk4_update
 
+    ! EPFOR forcing (force.f increments for(kfor, k2fo, k2fo, 3), complex(8)):
+    ! box and radius, the efkz(nxh, 3) tally of the last proc4a / advanc_1,
+    ! and the increments of the RK4 substeps of a step
+    integer(c_int) function psdns_set_epfor(ctx, kfor, kforce) bind(C, name="psdns_set_epfor")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kfor
+      real(c_double), value :: kforce
+    end function psdns_set_epfor
+
+    integer(c_int) function psdns_epfor_tally(ctx, efkz) bind(C, name="psdns_epfor_tally")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: efkz(*)
+    end function psdns_epfor_tally
+
+    integer(c_int) function psdns_rk4_forcing(ctx, forcing) bind(C, name="psdns_rk4_forcing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: forcing
+    end function psdns_rk4_forcing
+
     ! pencil checkpoint of un (homogeneous/outpen_write.f:59-76, inpen_read.f);
     ! path is a NUL-terminated string: trim(name)//c_null_char
     integer(c_int) function psdns_outpen(ctx, path, file_prec) bind(C, name="psdns_outpen")
@@ -140,6 +162,15 @@ module psdns_gpu
       integer(c_int), value :: enable
       real(c_double), intent(out) :: times(6)
     end function psdns_stage_times
+
+    ! per-substep pass times (timestep_module.f tt(1..10) at pass granularity):
+    ! times(p + 1, st + 1), p = 0..10 passes, st = 0..3 substeps
+    integer(c_int) function psdns_pass_times(ctx, enable, times, ntimes) bind(C, name="psdns_pass_times")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable, ntimes
+      real(c_double), intent(out) :: times(*)
+    end function psdns_pass_times
 
     type(c_ptr) function psdns_last_error_c() bind(C, name="psdns_last_error")
       import :: c_ptr

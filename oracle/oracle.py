@@ -61,6 +61,7 @@ def lib():
         L.orc_proc4b.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_kxtran.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
         L.orc_xktran.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_epfor.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp]
         L.orc_ek_shell.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.orc_ek_force.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_spectrum.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -168,6 +169,18 @@ class Grid:
         uy = self.zeros_cyl(nv)
         self.L.orc_xktran(self.h, _p(np.ascontiguousarray(ux[:nv])), _p(uy), nv)
         return uy
+
+    def epfor(self, uy, uny, kfor, kforce, forc, mode):
+        """EPFOR forcing increments forc (3, k2fo, k2fo, kfor) complex ==
+        for(kfor, k2fo, k2fo, 3), in place on uy[:3] (and uny in mode 2):
+        mode 1 proc4a.f:588-640 (returns efkz (3, n/2) == efkz(nxh, 3)),
+        mode 2 proc4b.f:118-163, mode 3 advanc_2.f/advanc_3.f."""
+        assert uy.flags["C_CONTIGUOUS"] and uny.flags["C_CONTIGUOUS"]
+        f = np.ascontiguousarray(forc, np.complex128)
+        assert f.shape == (3, 2 * kfor - 1, 2 * kfor - 1, kfor)
+        efkz = np.zeros((3, self.n // 2))
+        self.L.orc_epfor(self.h, _p(uy), _p(uny), kfor, kforce, _p(f), mode, _p(efkz))
+        return efkz
 
     def ek_shell(self, uny, kfor):
         ek = np.zeros(kfor)
@@ -351,6 +364,50 @@ def advanc(g, k, uy, uny, u1, factors=None):
         uny[:, m] = a[:, m]
 
 
+def epfor_increments(kfor, kforce, rng, amp=1.0):
+    """Forcing increments for(kfor, k2fo, k2fo, 3) as force.f:70-160 forms
+    them from two complex random amplitudes per node (alpha, beta: the
+    reference's Uhlenbeck-Ornstein velinc, here Gaussian draws scaled by
+    amp): zero outside |k| <= kforce and at k = 0; (alpha, beta, 0) on the
+    kz axis (k12 = 0); otherwise alpha normal to k and z, beta normal to
+    alpha and k (solenoidal); conjugate symmetry on kx = 0 (force.f:140-156).
+    Returned as a (3, k2fo, k2fo, kfor) complex array (C order of the
+    Fortran array)."""
+    k2fo = 2 * kfor - 1
+    f = np.zeros((3, k2fo, k2fo, kfor), np.complex128)
+    vel = amp * (rng.standard_normal((2, k2fo, k2fo, kfor)) + 1j * rng.standard_normal((2, k2fo, k2fo, kfor)))
+
+    def wave(i):  # 1-based index -> wavenumber (force.f ifwave with ky/kz of waveno.f)
+        return i - 1 if i <= kfor else i - 2 * kfor
+
+    for z in range(1, k2fo + 1):
+        k3 = float(wave(z))
+        for y in range(1, k2fo + 1):
+            k2 = float(wave(y))
+            for x in range(1, kfor + 1):
+                al, be = vel[0, z - 1, y - 1, x - 1], vel[1, z - 1, y - 1, x - 1]
+                k1 = float(x - 1)
+                k = np.sqrt(k1 * k1 + k2 * k2 + k3 * k3)
+                k12 = np.sqrt(k1 * k1 + k2 * k2)
+                if k > kforce or k == 0.0:
+                    continue
+                if k12 == 0.0:
+                    f[0, z - 1, y - 1, x - 1] = al
+                    f[1, z - 1, y - 1, x - 1] = be
+                else:
+                    f[0, z - 1, y - 1, x - 1] = (al * k * k2 + be * k1 * k3) / (k * k12)
+                    f[1, z - 1, y - 1, x - 1] = (be * k2 * k3 - al * k * k1) / (k * k12)
+                    f[2, z - 1, y - 1, x - 1] = -be * k12 / k
+    for i in range(3):
+        for y in range(kfor + 1, k2fo + 1):
+            f[i, 0, y - 1, 0] = np.conj(f[i, 0, k2fo + 1 - y, 0])
+        for z in range(kfor + 1, k2fo + 1):
+            f[i, z - 1, 0, 0] = np.conj(f[i, k2fo + 1 - z, 0, 0])
+            for y in range(2, k2fo + 1):
+                f[i, z - 1, y - 1, 0] = np.conj(f[i, k2fo + 1 - z, k2fo + 1 - y, 0])
+    return f
+
+
 def fft_c2c(x, sign):
     a = np.ascontiguousarray(x, np.complex128).copy()
     lib().orc_fft_c2c(_p(a), a.shape[-1], a.size // a.shape[-1], sign)
@@ -379,8 +436,13 @@ class Run:
     then FEK forcing (ek_force) after the corrector."""
 
     def __init__(self, grid, uny, viscos, cfl, kshift=4, kf_shell=0, shift_seed=7, ekinf=None, dt_fixed=None,
-                 phi=None, pr=None, grad=None):
+                 phi=None, pr=None, grad=None, epfor=None):
+        """epfor = (kfor, kforce, forcing) with forcing(istep) the increments
+        for(kfor, k2fo, k2fo, 3) of step istep (0-based), (3, k2fo, k2fo,
+        kfor) complex: EPFOR forcing after proc4a/proc4b (advanc_1..3)."""
         self.g = grid
+        self.epfor = epfor
+        self.efkz = None
         self.uny = np.ascontiguousarray(uny, np.complex128).copy()
         self.uy = grid.zeros_cyl(5)
         self.uy[:3] = self.uny
@@ -433,6 +495,11 @@ class Run:
                 g.proc4a(self.uy, self.uny, *self.factors)
             else:
                 g.proc4b(self.uy, self.uny)
+            if self.epfor is not None:
+                kf, kfo, fn = self.epfor
+                e = g.epfor(self.uy, self.uny, kf, kfo, fn(self.istep), k)
+                if k == 1:
+                    self.efkz = e
             if self.phi_n is not None:
                 scalar_update(g, self.phi_u, self.phi_n, terms, k, self.dt, gsh_now, self.factors, self.pr)
         if self.kfor:
@@ -470,6 +537,13 @@ class Run:
             g.wavespace_vel(nl, ks, self.dt / 2.0)
             self.uy = nl
             advanc(g, k, self.uy, self.uny, self.u1, self.factors)
+            if self.epfor is not None and k < 4:
+                # advanc_1 (tally) .. advanc_3 add to u; advanc_4's add is
+                # overwritten by u1 on the mask (advanc_4.f:96-170)
+                kf, kfo, fn = self.epfor
+                e = g.epfor(self.uy, self.uny, kf, kfo, fn(self.istep), 1 if k == 1 else 3)
+                if k == 1:
+                    self.efkz = e
             if self.phi_n is not None:
                 scalar_advanc(g, k, self.phi_u, self.phi_n, self.phi_1, terms, ks, self.dt, gsh_now, self.factors,
                               self.pr)

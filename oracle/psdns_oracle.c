@@ -529,6 +529,64 @@ void orc_proc4b(const orc_grid *g, double *uy_, double *uny_) {
   for (int v = 0; v < 3; v++) { uy[v * vs] = 0; uny[v * vs] = 0; }
 }
 
+/* EPFOR forcing increments (the -DEPFOR build, PSDNS/makefile:69).
+ * forc = for(kfor, k2fo, k2fo, 3) complex, k2fo = 2 kfor - 1, as force.f
+ * fills it: index x = kx + 1 <= kfor; iy = ky + 1 for 0 <= ky < kfor and
+ * iy = ky + 2 kfor for -kfor < ky < 0 (proc4a.f:605-626: y <= kfor -> y,
+ * y >= ny + 2 - kfor -> y - ny + k2fo), iz from kz the same way.  Entries
+ * of the cylinder on the mask whose (kx, ky, kz) fall in that box take
+ * for(x, iy, iz, i), added after the projection:
+ *  mode 1  proc4a.f:588-640 (= advanc_1.f:165-210): only where
+ *          rks = |k|^2 <= kforce^2; efkz(k, i) += 2 Re(u conj(for)) tfact(x)
+ *          (u before the add, k = int(sqrt(rks) + 1.5), ssq = 1 for a
+ *          cube), efkz zeroed first (proc4a.f:66); u <- u + for
+ *  mode 2  proc4b.f:118-163: u <- u + for in the whole box, then (as
+ *          proc4b.f:176-205 goes on) un <- u and zero mean
+ *  mode 3  advanc_2.f:97-141 / advanc_3.f:98-142: u <- u + for in the box
+ * efkz as the reference's efkz(nxh, 3): efkz[i * nxh + k - 1]. */
+void orc_epfor(const orc_grid *g, double *uy_, double *uny_, int kfor, double kforce,
+               const double *forc_, int mode, double *efkz) {
+  const int n = g->n, na = g->num_al, nxh = n / 2, k2fo = 2 * kfor - 1;
+  cplx *uy = (cplx *)uy_, *uny = (cplx *)uny_;
+  const cplx *forc = (const cplx *)forc_;
+  const size_t vs = (size_t)na * n;
+  if (mode == 1)
+    for (int e = 0; e < 3 * nxh; e++) efkz[e] = 0.0;
+  for (int a = 0; a < na; a++) {
+    const int x = g->col_x[a], z = g->col_z[a];  /* 0-based: reference x - 1, z - 1 */
+    int iz = 0;
+    if (z + 1 <= kfor) iz = z + 1;
+    else if (z + 1 >= n + 2 - kfor) iz = z + 1 - n + k2fo;
+    if (iz == 0 || x + 1 > kfor) continue;
+    for (int i = 0; i < 3; i++) {
+      for (int y = 0; y < n; y++) {
+        if (!orc_mask(g, y, a)) continue;
+        int iy = 0;
+        if (y + 1 <= kfor) iy = y + 1;
+        else if (y + 1 >= n + 2 - kfor) iy = y + 1 - n + k2fo;
+        if (iy == 0) continue;
+        const cplx f = forc[(((size_t)i * k2fo + (iz - 1)) * k2fo + (iy - 1)) * kfor + x];
+        const size_t j = (size_t)i * vs + (size_t)a * n + y;
+        if (mode == 1) {
+          const double ky = kwave(y, n), kz = kwave(z, n);
+          const double rks = (double)x * x + ky * ky + kz * kz;
+          if (!(rks <= kforce * kforce)) continue;
+          const int k = (int)(sqrt(rks) + 1.5);
+          const double term = creal(uy[j] * conj(f));
+          if (k <= nxh) efkz[i * nxh + k - 1] += 2.0 * term * (x == 0 ? 1.0 : 2.0);
+        }
+        uy[j] += f;
+      }
+    }
+  }
+  if (mode == 2) {
+    for (int i = 0; i < 3; i++)
+      for (size_t j = 0; j < vs; j++)
+        if (orc_mask(g, (int)(j % n), (int)(j / n))) uny[i * vs + j] = uy[i * vs + j];
+    for (int i = 0; i < 3; i++) { uy[i * vs] = 0; uny[i * vs] = 0; }
+  }
+}
+
 /* ek_shell.f:131-168: shell energies (ik = int(|k| + 1.5), 1-based shells,
  * returned as ek[ik-1]) for ik <= kfor, z != nzhp, on the mask,
  * tfact = 1 at kx = 0 else 2 (comsp_set.f:85-87) */

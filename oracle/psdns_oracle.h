@@ -54,6 +54,11 @@ void orc_proc4b(const orc_grid *g, double *uy, double *uny);
 void orc_kxtran(const orc_grid *g, const double *uy, double *ux, int nv);
 void orc_xktran(const orc_grid *g, const double *ux, double *uy, int nv);
 
+/* EPFOR forcing increments for(kfor, 2 kfor - 1, 2 kfor - 1, 3) (complex)
+ * after proc4a (mode 1, efkz tally), proc4b (mode 2) or advanc_2/3 (mode 3) */
+void orc_epfor(const orc_grid *g, double *uy, double *uny, int kfor, double kforce,
+               const double *forc, int mode, double *efkz);
+
 /* FEK forcing (ek_shell.f / ek_force.f) and statistics (sptr.f) */
 void orc_ek_shell(const orc_grid *g, const double *uny, int kfor, double *ek);
 void orc_ek_force(const orc_grid *g, double *uy, double *uny, int kf_shell,

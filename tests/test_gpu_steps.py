@@ -275,7 +275,17 @@ def test_fortran_driver_matches_oracle(O, P, tmp_path):
     assert len([l for l in log if l.startswith("cpu at step")]) == nsteps
 
 
-@pytest.mark.parametrize("n,prec", [(32, 8), (256, 8), (256, 4)])
+# FP64 at 64^3 and 128^3: the fused step has been seen to differ between
+# reruns in the last, partly stored kx block of the x pass's second product
+# (differences of about 1e-27 on values of about 1e-11, far below every
+# parity tolerance); the cause is not found.  Non-strict: the variation
+# does not show on every run.
+FP64_VARIES = pytest.mark.xfail(strict=False, reason="FP64 64^3/128^3 run-to-run variation in the x pass's "
+                                "partly stored kx block, cause not found")
+
+
+@pytest.mark.parametrize("n,prec", [(32, 8), pytest.param(64, 8, marks=FP64_VARIES),
+                                    pytest.param(128, 8, marks=FP64_VARIES), (256, 8), (256, 4)])
 def test_fused_step_reproducible(P, O, n, prec):
     """The same state stepped twice on one context gives the same bits.
     The x pass stages each variable into LDS with asynchronous copies that
