@@ -162,6 +162,36 @@ int psdns_set_epfor(psdns_ctx *c, int kfor, double kforce);
 int psdns_epfor_tally(psdns_ctx *c, double *efkz);
 int psdns_rk4_forcing(psdns_ctx *c, const void *forcing);
 
+/* Lagrangian particles, RK2 (hooks/LAG_LOCAL: lagpart.f, partic.f,
+ * partsp.f:217-299).  pp(np, 9) is the reference's particle array, FP64,
+ * column-major (pp[(j - 1) np + i]): columns 1-3 positions in grid units
+ * (the node i of the stage-1 grid is at gx(i) = 1 + (i - 1) + gsh(1,1),
+ * partsp.f:44-53), 4-6 velocities, 7-9 the half-step positions rk_advpos
+ * keeps.  The reference's order, from realspace at kstep 1 (before step, so
+ * with the previous step's dt): rk_advpos(1, 1/2, 1/2); the spline of the
+ * state on the stage-1 grid of the step and the interpolation at the
+ * predicted positions; rk_advpos(2, 0, 1/2) (the first step only
+ * interpolates, partic(4)).
+ * lag_put: np particles, the first ncols (3, 6 or 9) columns given, the
+ *   rest zero; lag_get: all 9 columns.
+ * lag_spline (spxyz_m.f with splx/splz/sply, solsp1.f): the tricubic
+ *   spline coefficients of the velocity of un on the phase-shifted grid of
+ *   stage 1 of the last set_shifts: the periodic (1,4,1) systems along x,
+ *   y, z on 216 x the grid values (splz.f:22-28), solved exactly in Fourier
+ *   space (eigenvalue 4 + 2 cos(2 pi k / n) per direction) and transformed
+ *   to the grid; kept in ux(1:3) (allocated on first use).  Any call that
+ *   writes the work arrays afterwards discards them.
+ * lag_interp (spcal_local.f:119-364): pp(:,4:6) = the spline velocity at
+ *   pp(:,1:3), B-spline weights of nodes ib-1 .. ib+2 (periodic), FP64 sums.
+ * lag_advpos (rk_advpos.f:15-52), fact = cof dt n / (2 pi): pstep 1
+ *   pp(:,7:9) = x + fact1 u, x = pp(:,7:9) + fact2 u; pstep 2 x = pp(:,7:9)
+ *   + fact2 u. */
+int psdns_lag_put(psdns_ctx *c, const double *pp, int np, int ncols);
+int psdns_lag_get(psdns_ctx *c, double *pp, int np);
+int psdns_lag_spline(psdns_ctx *c);
+int psdns_lag_interp(psdns_ctx *c);
+int psdns_lag_advpos(psdns_ctx *c, int pstep, double dt, double cof1, double cof2);
+
 /* FEK forcing (ek_force.f:1-76 with ek_shell.f:81-176): shell energies of
  * un for ik = int(|k| + 1.5) <= kfor, then u and un scaled by
  * sqrt(ekinf(ik)/ek(ik)) for ik <= kf_shell (ratio 0 for ik = 1).

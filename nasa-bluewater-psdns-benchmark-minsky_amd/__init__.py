@@ -31,7 +31,8 @@ EXPORTS = (
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
     "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_rk4_update",
-    "psdns_set_epfor", "psdns_epfor_tally", "psdns_rk4_forcing", "psdns_pass_times", "psdns_last_error",
+    "psdns_set_epfor", "psdns_epfor_tally", "psdns_rk4_forcing", "psdns_pass_times", "psdns_lag_put",
+    "psdns_lag_get", "psdns_lag_spline", "psdns_lag_interp", "psdns_lag_advpos", "psdns_last_error",
 )
 
 
@@ -105,6 +106,11 @@ def load_library():
         "psdns_epfor_tally": [_vp, _dp],
         "psdns_rk4_forcing": [_vp, _vp],
         "psdns_pass_times": [_vp, i, _dp, i],
+        "psdns_lag_put": [_vp, _dp, i, i],
+        "psdns_lag_get": [_vp, _dp, i],
+        "psdns_lag_spline": [_vp],
+        "psdns_lag_interp": [_vp],
+        "psdns_lag_advpos": [_vp, i, d, d, d],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -301,6 +307,40 @@ class Solver:
         self._chk(self.L.psdns_pass_times(self.h, int(enable), _dptr(t), 44))
         return t.reshape(4, 11)
 
+    # Lagrangian particles: pp is the reference's pp(np, 9) as a (9, np)
+    # float64 array (positions in grid units, velocities, saved positions)
+    def lag_put(self, pp):
+        a = np.ascontiguousarray(pp, np.float64)
+        self._chk(self.L.psdns_lag_put(self.h, _dptr(a), a.shape[1], a.shape[0]))
+        self.lag_np = a.shape[1]
+
+    def lag_get(self):
+        a = np.zeros((9, self.lag_np))
+        self._chk(self.L.psdns_lag_get(self.h, _dptr(a), self.lag_np))
+        return a
+
+    def lag_spline(self):
+        self._chk(self.L.psdns_lag_spline(self.h))
+
+    def lag_interp(self):
+        self._chk(self.L.psdns_lag_interp(self.h))
+
+    def lag_advpos(self, pstep, dt, cof1, cof2):
+        self._chk(self.L.psdns_lag_advpos(self.h, pstep, dt, cof1, cof2))
+
+    def lag_step(self, dt_prev):
+        """partsp.f's RK2 particle step after set_shifts of the new step:
+        interpolation only on the first call (dt_prev None), else
+        rk_advpos(1, 1/2, 1/2), spline + interpolation, rk_advpos(2, 0, 1/2)"""
+        if dt_prev is None:
+            self.lag_spline()
+            self.lag_interp()
+            return
+        self.lag_advpos(1, dt_prev, 0.5, 0.5)
+        self.lag_spline()
+        self.lag_interp()
+        self.lag_advpos(2, dt_prev, 0.0, 0.5)
+
     # per-routine entry points
     def phase1(self, kstep):
         self._chk(self.L.psdns_phase1(self.h, kstep))
@@ -376,7 +416,7 @@ class Run:
     stage_nonlinear -> (stage 1: dt = cfl/velmax, integrating factors) ->
     stage_update, then FEK forcing.  Shift offsets come from the caller."""
 
-    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None, epfor=None):
+    def __init__(self, solver, viscos, cfl, kf_shell=0, ekinf=None, epfor=None, particles=None):
         self.s, self.viscos, self.cfl = solver, viscos, cfl
         self.kf_shell = kf_shell
         self.kfor = kf_shell + 1 if kf_shell > 0 else 0
@@ -387,10 +427,17 @@ class Run:
         self.epfor = epfor
         if epfor is not None:
             solver.set_epfor(epfor[0], epfor[1])
+        # Lagrangian particles pp (9, np): stepped at the start of each step
+        # (lagpart from realspace at kstep 1, with the previous dt)
+        self.particles = particles is not None
+        if self.particles:
+            solver.lag_put(particles)
 
     def step(self, gsh):
         s = self.s
         s.set_shifts(gsh)
+        if self.particles:
+            s.lag_step(self.dts[-1] if self.dts else None)
         vm = s.stage_nonlinear(1)
         self.velmax = vm
         self.dt = self.cfl / vm

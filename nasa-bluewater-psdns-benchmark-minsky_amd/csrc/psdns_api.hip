@@ -75,6 +75,13 @@ struct psdns_ctx {
   int force_kfor = -1, nforce = 0;
   double* d_shell_part = nullptr;   // partial rows of psdns_shell_stats, allocated on first use
   int dev = 0;                      // device of the context's stream
+  double gsh[6] = {};               // the shift offsets of the last set_shifts
+  // Lagrangian particles: pp(np, 9) column-major (FP64) and whether ux[0..2]
+  // hold the spline coefficients of lag_spline (and its stage-1 offsets)
+  double* d_pp = nullptr;
+  int lag_np = 0;
+  bool spline_ok = false;
+  double spline_gsh[3] = {};
   // EPFOR forcing increments (psdns_set_epfor): the cylinder entries of the
   // forcing box, the caller's for(kfor, k2fo, k2fo, 3) on the device, the
   // per-entry tally terms and efkz(nxh, 3) of the last proc4a / advanc_1
@@ -314,6 +321,7 @@ int ready(psdns_ctx* c) {
 int ready_clobber(psdns_ctx* c) {
   if (int rc = ready(c)) return rc;
   c->last_nonlinear = 0;
+  c->spline_ok = false;
   return 0;
 }
 
@@ -513,7 +521,7 @@ int psdns_destroy(psdns_ctx* c) {
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
                   c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc, c->d_u1, c->d_epf, c->d_epf_for,
-                  c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part};
+                  c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part, c->d_pp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_fac) (void)hipHostFree(c->h_fac);
@@ -544,6 +552,7 @@ int psdns_columns(const psdns_ctx* c, int* col_x, int* col_z, int* cut_z) {
 int psdns_set_shifts(psdns_ctx* c, const double gsh[6]) {
   if (int rc = ready_clobber(c)) return rc;
   if (!gsh) return fail(PSDNS_EINVAL, "null gsh");
+  std::memcpy(c->gsh, gsh, sizeof c->gsh);
   // shifts.f:146-178: exp(i 2 pi k gsh(dir, stage) / n), formed on the
   // device in FP64 (and rounded to the state precision for the passes)
   psdns::launch_shifts(c->c, c->d_shift, c->d_shiftr, gsh);
@@ -1046,6 +1055,84 @@ int psdns_xktran(psdns_ctx* c, int nv) {
   if ((rc = pass(c, Pass::XR2C, nv))) return rc;
   if ((rc = pass(c, Pass::ZFwdNorm, nv))) return rc;
   if ((rc = pass(c, Pass::YFwd, nv))) return rc;
+  return sync(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Lagrangian particles (hooks/LAG_LOCAL, RK2: lagpart.f -> partic.f ->
+// partsp.f:217-299).  pp(np, 9) as the reference's pp(:, 1:9), FP64,
+// column-major: positions in grid units, velocities, rk_advpos's saved
+// positions.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psdns_lag_put(psdns_ctx* c, const double* pp, int np, int ncols) {
+  if (int rc = ready(c)) return rc;
+  if (!pp || np < 1) return fail(PSDNS_EINVAL, "need particles (np = %d)", np);
+  if (ncols != 3 && ncols != 6 && ncols != 9) return fail(PSDNS_EINVAL, "ncols = %d: 3, 6 or 9", ncols);
+  if (np != c->lag_np) {
+    if (c->d_pp) HIPCHK(hipFree(c->d_pp));
+    c->d_pp = nullptr;
+    c->lag_np = 0;
+    const size_t bytes = sizeof(double) * 9 * (size_t)np;
+    if (hipMalloc(&c->d_pp, bytes) != hipSuccess)
+      return fail(PSDNS_ENOMEM, "hipMalloc of the particle array (%zu bytes) failed", bytes);
+    c->lag_np = np;
+  }
+  const size_t col = sizeof(double) * (size_t)np;
+  HIPCHK(hipMemsetAsync(c->d_pp, 0, 9 * col, c->c.stream));
+  HIPCHK(hipMemcpyAsync(c->d_pp, pp, ncols * col, hipMemcpyHostToDevice, c->c.stream));
+  return sync(c);
+}
+
+int psdns_lag_get(psdns_ctx* c, double* pp, int np) {
+  if (int rc = ready(c)) return rc;
+  if (!pp) return fail(PSDNS_EINVAL, "null particle array");
+  if (np != c->lag_np || !c->d_pp) return fail(PSDNS_EINVAL, "np = %d: the context holds %d particles", np, c->lag_np);
+  HIPCHK(hipMemcpyAsync(pp, c->d_pp, sizeof(double) * 9 * (size_t)np, hipMemcpyDeviceToHost, c->c.stream));
+  return sync(c);
+}
+
+int psdns_lag_spline(psdns_ctx* c) {
+  if (int rc = ready_clobber(c)) return rc;
+  int rc;
+  if ((rc = ensure_ux(c))) return rc;
+  // the spline spectrum of un on the stage-1 shifted grid, in the X work
+  // array as 3 cylinder slots, then the inverse passes of kxtran into ux
+  psdns::launch_spline_prep(c->c, c->d_xs);
+  HIPCHK(hipGetLastError());
+  Ctx k = c->c;
+  k.u = c->d_xs;
+  if (psdns::launch_pass(k, Pass::YInvPlain, 3, 0) || psdns::launch_pass(k, Pass::ZInv, 3, 0) ||
+      psdns::launch_pass(k, Pass::XC2R, 3, 0))
+    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  HIPCHK(hipGetLastError());
+  if ((rc = sync(c))) return rc;
+  std::memcpy(c->spline_gsh, c->gsh, sizeof c->spline_gsh);
+  c->spline_ok = true;
+  return PSDNS_OK;
+}
+
+int psdns_lag_interp(psdns_ctx* c) {
+  if (int rc = ready(c)) return rc;
+  if (!c->d_pp) return fail(PSDNS_ESTATE, "no particles (psdns_lag_put)");
+  if (!c->spline_ok)
+    return fail(PSDNS_ESTATE, "no spline coefficients: psdns_lag_spline first (and no work-array call after it)");
+  psdns::launch_lag_interp(c->c, c->d_pp, c->lag_np, c->spline_gsh);
+  HIPCHK(hipGetLastError());
+  return sync(c);
+}
+
+int psdns_lag_advpos(psdns_ctx* c, int pstep, double dt, double cof1, double cof2) {
+  if (int rc = ready(c)) return rc;
+  if (!c->d_pp) return fail(PSDNS_ESTATE, "no particles (psdns_lag_put)");
+  if (pstep != 1 && pstep != 2) return fail(PSDNS_EINVAL, "pstep = %d: 1 or 2 (RK2)", pstep);
+  // rk_advpos.f:31-32: fact = cof dt / (2 pi) xyzl ps, xyzl = n, ps = 1
+  const double f = dt / (2.0 * M_PI) * c->n;
+  psdns::launch_lag_advpos(c->c, c->d_pp, c->lag_np, pstep, cof1 * f, cof2 * f);
+  HIPCHK(hipGetLastError());
   return sync(c);
 }
 

@@ -129,5 +129,11 @@ void launch_vg_deriv(const Ctx& c, void* d);
 void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
                   double* terms, double* efkz);
 void launch_moments(const Ctx& c, double* d_out);
+// Lagrangian particles: the spline spectrum of un (stage-1 shifts, 3
+// cylinder slots at out), the interpolation of the coefficients in ux[0..2]
+// at pp(np, 1:3) into pp(np, 4:6), the RK2 position update
+void launch_spline_prep(const Ctx& c, void* out);
+void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3]);
+void launch_lag_advpos(const Ctx& c, double* pp, int np, int pstep, double fact1, double fact2);
 
 }  // namespace psdns

@@ -1866,6 +1866,126 @@ __global__ void __launch_bounds__(256) k_moments(const R* __restrict__ f, size_t
 }
 
 // ---------------------------------------------------------------------------
+// Lagrangian particles (hooks/LAG_LOCAL).
+// Spline coefficients (spxyz_m.f: periodic (1,4,1) solves along x, z and y,
+// the z solve on 216 times the data, splz.f:22-28) of the velocity on the
+// stage-1 phase-shifted grid: the periodic system is diagonal in Fourier
+// space, eigenvalue 4 + 2 cos(2 pi k / n) per direction, so the coefficients
+// are the inverse transform of  u(k) sx sy sz (6 / (4 + 2 cos))^3-product.
+// k_spline_prep forms that spectrum on the mask (0 elsewhere, as the
+// state), the inverse passes of kxtran put the coefficients in ux[0..2].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double spline_eig(int k, int n) { return 6.0 / (4.0 + 2.0 * cospi(2.0 * k / n)); }
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_spline_prep(const C<R>* __restrict__ un, C<R>* __restrict__ out, Geo g, const double2* __restrict__ sx,
+                  const double2* __restrict__ sy, const double2* __restrict__ sz) {
+  const size_t tot = (size_t)g.na * g.n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const Pt p = point(i, g);
+    C<R> o[3] = {{0, 0}, {0, 0}, {0, 0}};
+    if (p.m) {
+      const double e = spline_eig(p.x, g.n) * spline_eig(p.ky, g.n) * spline_eig(p.kz, g.n);
+      const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        const C<R> a = un[v * tot + i];
+        const double2 w = zmul(make_double2((double)a.x, (double)a.y), s);
+        o[v] = {(R)(w.x * e), (R)(w.y * e)};
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) out[v * tot + i] = o[v];
+  }
+}
+
+// spcal_local.f:119-364 (iod = 0, sf = 1), one thread per particle: the
+// position pp(m, j) in grid units relative to the shifted grid (gfn = gsh +
+// 1), the B-spline weights of its cell and the 4 x 4 x 4 coefficients of
+// nodes ib - 1 .. ib + 2 (periodic) of the three fields c[v][y][z][x],
+// summed in the reference's order in FP64; pp column-major pp(np, 9).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_lag_interp(const R* __restrict__ c, double* __restrict__ pp, int np, int n, double g0, double g1, double g2) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= np) return;
+  const size_t n3 = (size_t)n * n * n;
+  const double gs[3] = {g0, g1, g2};
+  const double d6 = 1.0 / 6.0, tt = 2.0 / 3.0;
+  double bff[4][3];
+  int ib[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const double px = pp[(size_t)j * np + m] - (gs[j] + 1.0) - (double)n;
+    const double fl = fabs(px) < 1e15 ? floor(px) : 0.0;
+    long long jpx = (long long)fl;
+    if (jpx < 0) jpx += 1000LL * n;
+    long long jloc = jpx % n;
+    if (jloc < 0) jloc += n;
+    double bfp = (double)jloc + px - fl;
+    // the reference stops on a position it cannot place (spcal_local.f:
+    // 186-190); here such a lane (non-finite or beyond 1000 periods) takes
+    // node 0 so that every index stays inside the coefficient arrays
+    if (!(bfp >= 0.0 && bfp < (double)n)) bfp = 0.0;
+    ib[j] = (int)bfp;
+    const double pos = bfp - ib[j], ym = 1.0 - pos, yms = ym * ym;
+    bff[0][j] = ym * yms * d6;
+    bff[1][j] = tt + 0.5 * pos * (yms - 1.0);
+    bff[3][j] = pos * pos * pos * d6;
+    bff[2][j] = 1.0 - bff[0][j] - bff[1][j] - bff[3][j];
+  }
+  int xs[4], ys[4], zs[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    xs[i] = (ib[0] + i - 1 + n) % n;
+    ys[i] = (ib[1] + i - 1 + n) % n;
+    zs[i] = (ib[2] + i - 1 + n) % n;
+  }
+  double func[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double sw[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const double cst = bff[i][0];
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        double bt[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) bt[jj] = (double)c[v * n3 + ((size_t)ys[jj] * n + zs[k]) * n + xs[i]];
+        const double term1 = bt[0] * bff[0][1] + bt[2] * bff[2][1];
+        const double term2 = bt[1] * bff[1][1] + bt[3] * bff[3][1];
+        sw[v] += (term1 + term2) * cst;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) func[v] += sw[v] * bff[k][2];
+  }
+#pragma unroll
+  for (int v = 0; v < 3; v++) pp[(size_t)(3 + v) * np + m] = func[v];
+}
+
+// rk_advpos.f:15-52: pstep 1 h = x + fact1 u, x = h + fact2 u; pstep 2 x = h + fact2 u
+__global__ void __launch_bounds__(256)
+    k_lag_advpos(double* __restrict__ pp, int np, int pstep, double fact1, double fact2) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= np) return;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double* x = pp + (size_t)k * np;
+    const double u = pp[(size_t)(3 + k) * np + m];
+    double* h = pp + (size_t)(6 + k) * np;
+    if (pstep == 1) {
+      h[m] = x[m] + fact1 * u;
+      x[m] = h[m] + fact2 * u;
+    } else {
+      x[m] = h[m] + fact2 * u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Initial field (homogeneous/field.f:172-257 construction, counter-based
 // generator keyed by (seed, x, y, z)); consym on kx = 0 in a second kernel.
 // ---------------------------------------------------------------------------
@@ -2396,6 +2516,27 @@ void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const dou
     hipLaunchKernelGGL(k_epfor_fold, dim3((nt + 255) / 256), dim3(256), 0, c.stream, list, nlist, terms, c.geo.nxh,
                        efkz);
   }
+}
+void launch_spline_prep(const Ctx& c, void* out) {
+  const size_t tot = (size_t)c.geo.na * c.geo.n;
+  if (c.prec == 8)
+    hipLaunchKernelGGL((k_spline_prep<double>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                       (const C<double>*)c.un, (C<double>*)out, c.geo, c.sx[0], c.sy[0], c.sz[0]);
+  else
+    hipLaunchKernelGGL((k_spline_prep<float>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
+                       (const C<float>*)c.un, (C<float>*)out, c.geo, c.sx[0], c.sy[0], c.sz[0]);
+}
+void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3]) {
+  const dim3 grid((np + 255) / 256), block(256);
+  if (c.prec == 8)
+    hipLaunchKernelGGL((k_lag_interp<double>), grid, block, 0, c.stream, (const double*)c.ux, pp, np, c.geo.n,
+                       gsh1[0], gsh1[1], gsh1[2]);
+  else
+    hipLaunchKernelGGL((k_lag_interp<float>), grid, block, 0, c.stream, (const float*)c.ux, pp, np, c.geo.n,
+                       gsh1[0], gsh1[1], gsh1[2]);
+}
+void launch_lag_advpos(const Ctx& c, double* pp, int np, int pstep, double fact1, double fact2) {
+  hipLaunchKernelGGL(k_lag_advpos, dim3((np + 255) / 256), dim3(256), 0, c.stream, pp, np, pstep, fact1, fact2);
 }
 void launch_vg_deriv(const Ctx& c, void* d) {
   if (c.prec == 8) LaunchP<double>::vg_deriv(c, d);

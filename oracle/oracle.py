@@ -67,6 +67,10 @@ def lib():
         L.orc_spectrum.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_vg_moments.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]
+        L.orc_spline3.argtypes = [ctypes.c_int, _dp]
+        L.orc_lag_interp.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_long, _dp]
+        L.orc_lag_advpos.argtypes = [ctypes.c_int, ctypes.c_long, _dp, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -184,7 +188,7 @@ class Grid:
 
     def ek_shell(self, uny, kfor):
         ek = np.zeros(kfor)
-        self.L.orc_ek_shell(self.h, _p(np.ascontiguousarray(uny[:3])), kfor, _p(ek))
+        self.L.orc_ek_shell(self.h, _p(np.ascontiguousarray(uny[:3], np.complex128)), kfor, _p(ek))
         return ek
 
     def ek_force(self, uy, uny, kf_shell, kfor, ekinf):
@@ -195,14 +199,14 @@ class Grid:
         """Moments m = 1..8 of du/dx and dw/dz over the grid / n^3
         (PSDNS/vgmomt.f:68-143), from the state on the mask."""
         du, dw = np.zeros(8), np.zeros(8)
-        self.L.orc_vg_moments(self.h, _p(np.ascontiguousarray(uny[:3])), _p(du), _p(dw))
+        self.L.orc_vg_moments(self.h, _p(np.ascontiguousarray(uny[:3], np.complex128)), _p(du), _p(dw))
         return du, dw
 
     def spectrum(self, uny, viscos, nshell=None):
         nshell = nshell or self.n
         ek, dk = np.zeros(nshell), np.zeros(nshell)
         t, e = np.zeros(1), np.zeros(1)
-        self.L.orc_spectrum(self.h, _p(np.ascontiguousarray(uny[:3])), viscos, nshell, _p(ek), _p(dk), _p(t), _p(e))
+        self.L.orc_spectrum(self.h, _p(np.ascontiguousarray(uny[:3], np.complex128)), viscos, nshell, _p(ek), _p(dk), _p(t), _p(e))
         return ek, dk, float(t[0]), float(e[0])
 
 
@@ -408,6 +412,53 @@ def epfor_increments(kfor, kforce, rng, amp=1.0):
     return f
 
 
+# ---------------------------------------------------------------------------
+# Lagrangian particles (hooks/LAG_LOCAL): pp is the reference's pp(np, 9),
+# column-major, as a (9, np) float64 array: rows 0-2 positions in grid units
+# (gx(i) = 1 + (i - 1) + gsh(1,1)), 3-5 velocities, 6-8 rk_advpos's saved
+# half-step positions
+# ---------------------------------------------------------------------------
+def spline3(f):
+    """spxyz_m.f: tricubic-spline coefficients of periodic fields f[..., y, z, x]"""
+    out = np.ascontiguousarray(f, np.float64).copy()
+    n = out.shape[-1]
+    for v in out.reshape(-1, n, n, n):
+        lib().orc_spline3(n, _p(v))
+    return out
+
+
+def velocity_spline(g, uny, gsh):
+    """rk_part.f:28-41: spline coefficients (3, n, n, n) of the velocity of
+    uny on the stage-1 phase-shifted grid (the physical u of itransform_vel
+    at kstep 1)"""
+    uy = g.zeros_cyl(5)
+    uy[:3] = uny[:3]
+    g.set_shifts(gsh)
+    g.phshift_inplace(uy, 3, 1)
+    return spline3(g.kxtran(uy, 3))
+
+
+def lag_interp(c, gsh1, pp):
+    """spcal_local.f: pp[3:6] = spline velocity at pp[0:3] (in place)"""
+    assert pp.flags["C_CONTIGUOUS"] and pp.shape[0] == 9
+    c = np.ascontiguousarray(c, np.float64)
+    g3 = np.ascontiguousarray(np.asarray(gsh1, np.float64)[:3])
+    lib().orc_lag_interp(c.shape[-1], _p(c), c.shape[0], _p(g3), pp.shape[1], _p(pp))
+
+
+def lag_advpos(n, pp, pstep, dt, cof1, cof2):
+    """rk_advpos.f (in place)"""
+    assert pp.flags["C_CONTIGUOUS"] and pp.shape[0] == 9
+    lib().orc_lag_advpos(n, pp.shape[1], _p(pp), pstep, dt, cof1, cof2)
+
+
+def particles_uniform(n, np_, seed):
+    """np_ particles uniform in the box: positions 1 + U n in grid units"""
+    pp = np.zeros((9, np_))
+    pp[:3] = 1.0 + np.random.default_rng(seed).random((3, np_)) * n
+    return pp
+
+
 def fft_c2c(x, sign):
     a = np.ascontiguousarray(x, np.complex128).copy()
     lib().orc_fft_c2c(_p(a), a.shape[-1], a.size // a.shape[-1], sign)
@@ -436,12 +487,15 @@ class Run:
     then FEK forcing (ek_force) after the corrector."""
 
     def __init__(self, grid, uny, viscos, cfl, kshift=4, kf_shell=0, shift_seed=7, ekinf=None, dt_fixed=None,
-                 phi=None, pr=None, grad=None, epfor=None):
+                 phi=None, pr=None, grad=None, epfor=None, particles=None):
         """epfor = (kfor, kforce, forcing) with forcing(istep) the increments
         for(kfor, k2fo, k2fo, 3) of step istep (0-based), (3, k2fo, k2fo,
         kfor) complex: EPFOR forcing after proc4a/proc4b (advanc_1..3)."""
         self.g = grid
         self.epfor = epfor
+        # Lagrangian particles pp (9, np), advanced at the start of each step
+        # (lagpart from realspace at kstep 1, before step: the previous dt)
+        self.pp = None if particles is None else np.ascontiguousarray(particles, np.float64).copy()
         self.efkz = None
         self.uny = np.ascontiguousarray(uny, np.complex128).copy()
         self.uy = grid.zeros_cyl(5)
@@ -473,10 +527,29 @@ class Run:
         lib().orc_shift_rule(self.kshift, _p(self.gsh), _p(r), 0)
         return self.gsh.copy()
 
+    def particles_step(self, gsh_now):
+        """partic(4) on the first step (interpolation only), else partsp's
+        RK2: rk_advpos(1, 1/2, 1/2), the spline of the state on this step's
+        stage-1 grid and the interpolation at the predicted positions,
+        rk_advpos(2, 0, 1/2), all with the previous step's dt
+        (partsp.f:217-299)"""
+        g, pp = self.g, self.pp
+        c = velocity_spline(g, self.uny, gsh_now)
+        g.set_shifts(gsh_now)
+        if not self.dts:
+            lag_interp(c, gsh_now[:3], pp)
+            return
+        dt = self.dts[-1]
+        lag_advpos(g.n, pp, 1, dt, 0.5, 0.5)
+        lag_interp(c, gsh_now[:3], pp)
+        lag_advpos(g.n, pp, 2, dt, 0.0, 0.5)
+
     def step(self, gsh=None):
         g = self.g
         g.set_shifts(self.next_shifts() if gsh is None else gsh)
         gsh_now = self.gsh if gsh is None else np.asarray(gsh)
+        if self.pp is not None:
+            self.particles_step(np.asarray(gsh_now, np.float64).reshape(6))
         for k in (1, 2):
             g.phshift_inplace(self.uy, 3, k)
             ux = g.itransform_vel(self.uy)

@@ -791,3 +791,140 @@ void orc_field(const orc_grid *g, double *uny_, unsigned long long seed, double 
   double s = sqrt(tke / t);
   for (size_t i = 0; i < 3 * vs; i++) uny[i] *= s;
 }
+
+/* ------------------------------------------------------------------------
+ * Lagrangian particles (hooks/LAG_LOCAL), one task.
+ * ---------------------------------------------------------------------- */
+
+/* solsp1.f:9-43: factors of the periodic tridiagonal system with 4 on the
+ * diagonal and 1 off it (Ahlberg et al. 1967), p already inverted */
+typedef struct { int n; double *p, *q, *t, d; } spl_fac;
+
+static spl_fac solsp1(int n) {
+  spl_fac f = {n, malloc(sizeof(double) * (n + 1)), malloc(sizeof(double) * (n + 1)),
+               malloc(sizeof(double) * (n + 1)), 0.0};
+  double *s = malloc(sizeof(double) * (n + 1));
+  f.q[0] = 0.0;
+  for (int i = 1; i <= n; i++) {
+    f.p[i] = f.q[i - 1] + 4.0;
+    f.q[i] = -1.0 / f.p[i];
+  }
+  s[0] = 1.0;
+  for (int i = 1; i <= n; i++) s[i] = -s[i - 1] / f.p[i];
+  f.t[n] = 1.0;
+  for (int i = n - 1; i >= 1; i--) f.t[i] = f.q[i] * f.t[i + 1] + s[i];
+  f.d = 1.0 / (f.t[1] + f.t[n - 1] + 4.0);
+  for (int i = 1; i <= n; i++) f.p[i] = 1.0 / f.p[i];
+  free(s);
+  return f;
+}
+
+static void spl_free(spl_fac *f) { free(f->p); free(f->q); free(f->t); }
+
+/* one line as splx.f:24-50 (sply.f, splz.f with the data scaled by 216):
+ * data fv[(i-1) * st], i = 1..n (node i-1) -> coefficients of nodes 0..n-1
+ * in place (the reference's bs(i+1), with bs(1), bs(n+2), bs(n+3) their
+ * periodic copies) */
+static void spl_line(const spl_fac *f, double *fv, long st, double scale, double *u, double *v, double *bs) {
+  const int n = f->n;
+  u[0] = 0.0;
+  for (int x = 1; x <= n; x++) u[x] = (fv[(x - 1) * st] * scale - u[x - 1]) * f->p[x];
+  v[n] = 0.0;
+  for (int x = n - 1; x >= 1; x--) v[x] = f->q[x] * v[x + 1] + u[x];
+  bs[n + 1] = f->d * (fv[(n - 1) * st] * scale - v[1] - v[n - 1]);
+  for (int x = n - 1; x >= 1; x--) bs[x + 1] = f->t[x] * bs[n + 1] + v[x];
+  for (int x = 1; x <= n; x++) fv[(x - 1) * st] = bs[x + 1];
+}
+
+/* spxyz_m.f: the tricubic-spline coefficients of a periodic field
+ * f[y][z][x] (n^3, in place): x lines (splx.f), z lines with the data
+ * scaled by 216 (splz.f:22-28), y lines (sply.f) */
+void orc_spline3(int n, double *f) {
+  spl_fac fac = solsp1(n);
+#pragma omp parallel
+  {
+    double *u = malloc(sizeof(double) * (n + 1)), *v = malloc(sizeof(double) * (n + 1)),
+           *bs = malloc(sizeof(double) * (n + 4));
+#pragma omp for schedule(static)
+    for (long yz = 0; yz < (long)n * n; yz++) spl_line(&fac, f + yz * n, 1, 1.0, u, v, bs);
+#pragma omp for schedule(static)
+    for (long yx = 0; yx < (long)n * n; yx++) {
+      const long y = yx / n, x = yx % n;
+      spl_line(&fac, f + y * n * n + x, n, 216.0, u, v, bs);
+    }
+#pragma omp for schedule(static)
+    for (long zx = 0; zx < (long)n * n; zx++) spl_line(&fac, f + zx, (long)n * n, 1.0, u, v, bs);
+    free(u); free(v); free(bs);
+  }
+  spl_free(&fac);
+}
+
+/* spcal_local.f:119-364 (iod = 0, sf = 1): for particle m at pp(m, 1:3)
+ * (grid units, gx(i) = 1 + (i - 1) + gsh), the value of nv spline fields
+ * c[v][y][z][x] into pp(m, 4:3+nv); pp is the reference's column-major
+ * pp(np, 9): pp[(j - 1) np + m].  Same basis functions, stencil (nodes
+ * ibx - 1 .. ibx + 2, periodic) and summation order as the reference. */
+void orc_lag_interp(int n, const double *c, int nv, const double gsh[3], long np, double *pp) {
+  const double d6 = 1.0 / 6.0, tt = 2.0 / 3.0;
+  const size_t n3 = (size_t)n * n * n;
+#pragma omp parallel for schedule(static)
+  for (long m = 0; m < np; m++) {
+    double bff[4][3];
+    int ib[3];
+    for (int j = 0; j < 3; j++) {
+      const double px1 = pp[j * np + m] - (gsh[j] + 1.0);
+      const double px = px1 - (double)n;
+      const long ipx = (long)floor(px);
+      long jpx = ipx;
+      if (ipx < 0) jpx = ipx + 1000L * n;
+      const long jloc = jpx % n;
+      const double bfp = (double)jloc + px - (double)ipx;
+      ib[j] = (int)bfp;
+      const double pos = bfp - ib[j], ym = 1.0 - pos, yms = ym * ym;
+      bff[0][j] = ym * yms * d6;
+      bff[1][j] = tt + 0.5 * pos * (yms - 1.0);
+      bff[3][j] = pos * pos * pos * d6;
+      bff[2][j] = 1.0 - bff[0][j] - bff[1][j] - bff[3][j];
+    }
+    double func[3] = {0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+      const int z = ((ib[2] + k - 1) % n + n) % n;
+      double sw[3] = {0, 0, 0};
+      for (int i = 0; i < 4; i++) {
+        const int x = ((ib[0] + i - 1) % n + n) % n;
+        const double cst = bff[i][0];
+        for (int iv = 0; iv < nv; iv++) {
+          double bt[4];
+          for (int jj = 0; jj < 4; jj++) {
+            const int y = ((ib[1] + jj - 1) % n + n) % n;
+            bt[jj] = c[iv * n3 + ((size_t)y * n + z) * n + x];
+          }
+          const double term1 = bt[0] * bff[0][1] + bt[2] * bff[2][1];
+          const double term2 = bt[1] * bff[1][1] + bt[3] * bff[3][1];
+          sw[iv] += (term1 + term2) * cst;
+        }
+      }
+      for (int iv = 0; iv < nv; iv++) func[iv] += sw[iv] * bff[k][2];
+    }
+    for (int iv = 0; iv < nv; iv++) pp[(3 + iv) * np + m] = func[iv];
+  }
+}
+
+/* rk_advpos.f:15-52 (RK2, b = 1): fact = cof dt / (2 pi) n = cof dt / dx;
+ * pstep 1: pp(:,7:9) = x + fact1 u, x = pp(:,7:9) + fact2 u;
+ * pstep 2: x = pp(:,7:9) + fact2 u */
+void orc_lag_advpos(int n, long np, double *pp, int pstep, double dt, double cof1, double cof2) {
+  const double pi = 3.14159265358979323846;
+  for (int k = 0; k < 3; k++) {
+    const double fact1 = cof1 * dt / 2.0 / pi * n, fact2 = cof2 * dt / 2.0 / pi * n;
+    double *x = pp + k * np, *u = pp + (3 + k) * np, *h = pp + (6 + k) * np;
+    for (long i = 0; i < np; i++) {
+      if (pstep == 1) {
+        h[i] = x[i] + fact1 * u[i];
+        x[i] = h[i] + fact2 * u[i];
+      } else {
+        x[i] = h[i] + fact2 * u[i];
+      }
+    }
+  }
+}

@@ -68,6 +68,15 @@ void orc_spectrum(const orc_grid *g, const double *uny, double viscos,
 /* moments 1..8 of du/dx and dw/dz over the grid / n^3 (vgmomt.f:68-143) */
 void orc_vg_moments(const orc_grid *g, const double *uny, double *dudx, double *dwdz);
 
+/* Lagrangian particles: tricubic-spline coefficients of a periodic field
+ * f[y][z][x] in place (spxyz_m.f: splx, splz x 216, sply with the solsp1
+ * factors), interpolation of nv fields at pp(np, 1:3) into pp(np, 4:3+nv)
+ * (spcal_local.f), the RK2 position update (rk_advpos.f); pp(np, 9) is
+ * column-major as the reference's */
+void orc_spline3(int n, double *f);
+void orc_lag_interp(int n, const double *c, int nv, const double gsh[3], long np, double *pp);
+void orc_lag_advpos(int n, long np, double *pp, int pstep, double dt, double cof1, double cof2);
+
 /* initial field: random-phase solenoidal construction of field.f with a
  * counter-based generator, conjugate symmetry on kx=0 (consym.f) */
 void orc_field(const orc_grid *g, double *uny, unsigned long long seed,
