@@ -34,6 +34,15 @@ EXPORTS = (
     "psdns_set_epfor", "psdns_epfor_tally", "psdns_rk4_forcing", "psdns_pass_times", "psdns_lag_put",
     "psdns_lag_get", "psdns_lag_spline", "psdns_lag_interp", "psdns_lag_advpos", "psdns_last_error",
 )
+# learned SGS closure (include/psdns_sgs.h)
+SGS_EXPORTS = (
+    "psdns_sgs_param_count", "psdns_sgs_create", "psdns_sgs_destroy", "psdns_sgs_set_params",
+    "psdns_sgs_get_params", "psdns_sgs_filter", "psdns_sgs_features", "psdns_sgs_put_fields",
+    "psdns_sgs_get_field", "psdns_sgs_add_snapshot", "psdns_sgs_snapshots", "psdns_sgs_clear_snapshots",
+    "psdns_sgs_target_scale", "psdns_sgs_gradient", "psdns_sgs_train", "psdns_sgs_predict", "psdns_sgs_eval",
+    "psdns_sgs_add_stress",
+)
+SGS_LAMBDA, SGS_BASIS, SGS_TARGET, SGS_PREDICTED, SGS_GRADIENT = range(5)
 
 
 class PsdnsError(RuntimeError):
@@ -50,6 +59,7 @@ class Params(ctypes.Structure):
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
@@ -111,6 +121,24 @@ def load_library():
         "psdns_lag_spline": [_vp],
         "psdns_lag_interp": [_vp],
         "psdns_lag_advpos": [_vp, i, d, d, d],
+        "psdns_sgs_param_count": [],
+        "psdns_sgs_create": [i, i, ctypes.POINTER(_vp)],
+        "psdns_sgs_destroy": [_vp],
+        "psdns_sgs_set_params": [_vp, _fp, i],
+        "psdns_sgs_get_params": [_vp, _fp, i],
+        "psdns_sgs_filter": [_vp, _vp, _vp],
+        "psdns_sgs_features": [_vp, _vp],
+        "psdns_sgs_put_fields": [_vp, _fp, _fp, _fp],
+        "psdns_sgs_get_field": [_vp, i, _fp],
+        "psdns_sgs_add_snapshot": [_vp],
+        "psdns_sgs_snapshots": [_vp, _ip],
+        "psdns_sgs_clear_snapshots": [_vp],
+        "psdns_sgs_target_scale": [_vp, _dp],
+        "psdns_sgs_gradient": [_vp, i, ctypes.c_ulonglong, ctypes.c_longlong, d, _fp, _dp],
+        "psdns_sgs_train": [_vp, i, i, d, ctypes.c_ulonglong, d, _dp],
+        "psdns_sgs_predict": [_vp, i],
+        "psdns_sgs_eval": [_vp, d, _dp],
+        "psdns_sgs_add_stress": [_vp, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -446,6 +474,160 @@ class Run:
         s.stage_update(1, self.dt, xf, yf, zf, forcing=f)
         s.stage_nonlinear(2)
         s.stage_update(2, self.dt, xf, yf, zf, forcing=f)
+        if self.kfor:
+            s.ek_force(self.kf_shell, self.kfor, self.ekinf)
+        self.dts.append(self.dt)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+class Closure:
+    """Learned SGS closure for an LES grid n^3 (include/psdns_sgs.h): the
+    network parameters and Adam state, the features/target/prediction of
+    the current field and a store of training snapshots, all on the GPU."""
+
+    def __init__(self, n, max_snapshots=0):
+        self.L = load_library()
+        self.n = n
+        self.P = n ** 3
+        self.nparam = self.L.psdns_sgs_param_count()
+        h = _vp()
+        self._chk(self.L.psdns_sgs_create(n, max_snapshots, ctypes.byref(h)))
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PsdnsError(f"psdns error {rc}: {self.L.psdns_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.psdns_sgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, theta):
+        a = np.ascontiguousarray(theta, np.float32)
+        self._chk(self.L.psdns_sgs_set_params(self.h, _fptr(a), a.size))
+
+    def get_params(self):
+        a = np.zeros(self.nparam, np.float32)
+        self._chk(self.L.psdns_sgs_get_params(self.h, _fptr(a), a.size))
+        return a
+
+    def filter(self, fine, coarse):
+        """box filter of fine's ux slots 0..2 into coarse's ux slots 0..2 + the target"""
+        self._chk(self.L.psdns_sgs_filter(self.h, fine.h, coarse.h))
+
+    def features(self, solver):
+        self._chk(self.L.psdns_sgs_features(self.h, solver.h))
+
+    def put_fields(self, lam, basis, tau=None):
+        la = np.ascontiguousarray(lam, np.float32)
+        ba = np.ascontiguousarray(basis, np.float32)
+        ta = None if tau is None else np.ascontiguousarray(tau, np.float32)
+        self._chk(self.L.psdns_sgs_put_fields(self.h, _fptr(la), _fptr(ba), None if ta is None else _fptr(ta)))
+
+    _SHAPES = {SGS_LAMBDA: (5,), SGS_BASIS: (4, 6), SGS_TARGET: (6,), SGS_PREDICTED: (6,), SGS_GRADIENT: (3, 3)}
+
+    def field(self, which):
+        n = self.n
+        a = np.zeros(self._SHAPES[which] + (n, n, n), np.float32)
+        self._chk(self.L.psdns_sgs_get_field(self.h, which, _fptr(a)))
+        return a
+
+    def add_snapshot(self):
+        self._chk(self.L.psdns_sgs_add_snapshot(self.h))
+
+    def snapshots(self):
+        c = ctypes.c_int()
+        self._chk(self.L.psdns_sgs_snapshots(self.h, ctypes.byref(c)))
+        return c.value
+
+    def clear_snapshots(self):
+        self._chk(self.L.psdns_sgs_clear_snapshots(self.h))
+
+    def target_scale(self):
+        s = np.zeros(1)
+        self._chk(self.L.psdns_sgs_target_scale(self.h, _dptr(s)))
+        return float(s[0])
+
+    def gradient(self, batch, seed, step, sigma2):
+        g = np.zeros(self.nparam, np.float32)
+        loss = np.zeros(1)
+        self._chk(self.L.psdns_sgs_gradient(self.h, batch, seed, step, sigma2, _fptr(g), _dptr(loss)))
+        return g, float(loss[0])
+
+    def train(self, nsteps, batch, lr, seed, sigma2):
+        losses = np.zeros(max(nsteps, 1))
+        self._chk(self.L.psdns_sgs_train(self.h, nsteps, batch, lr, seed, sigma2, _dptr(losses)))
+        return losses[:nsteps]
+
+    def predict(self, bf16=False):
+        self._chk(self.L.psdns_sgs_predict(self.h, int(bool(bf16))))
+
+    def eval(self, sigma2):
+        loss = np.zeros(1)
+        self._chk(self.L.psdns_sgs_eval(self.h, sigma2, _dptr(loss)))
+        return float(loss[0])
+
+    def add_stress(self, solver):
+        self._chk(self.L.psdns_sgs_add_stress(self.h, solver.h))
+
+
+def training_snapshot(closure, fine, coarse):
+    """One training snapshot from the fine DNS state: its physical velocity
+    (kxtran), box-filtered onto the coarse grid (velocity + target stress),
+    the coarse spectral state (xktran) and its features."""
+    fine.kxtran(3)
+    closure.filter(fine, coarse)
+    coarse.xktran(3)
+    closure.features(coarse)
+    closure.add_snapshot()
+
+
+class Les:
+    """RK2 time loop of a coarse-grid LES: the per-routine step
+    (phase1, itransform_vel, realspace_vel, transform, wavespace_vel,
+    proc4a/proc4b per stage, then FEK forcing) with the closure's stress
+    added to the products.  closure None runs the same grid without a model."""
+
+    def __init__(self, solver, viscos, cfl, closure=None, bf16=True, kf_shell=0, ekinf=None):
+        self.s, self.viscos, self.cfl = solver, viscos, cfl
+        self.closure, self.bf16 = closure, bf16
+        self.kf_shell = kf_shell
+        self.kfor = kf_shell + 1 if kf_shell > 0 else 0
+        self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
+        self.dts = []
+
+    def step(self, gsh):
+        s, m = self.s, self.closure
+        s.set_shifts(gsh)
+        for k in (1, 2):
+            s.phase1(k)
+            if m is not None:
+                m.features(s)
+                m.predict(self.bf16)
+            s.itransform_vel()
+            vm = s.realspace_vel(k)
+            if m is not None:
+                m.add_stress(s)
+            s.transform()
+            if k == 1:
+                self.velmax = vm
+                self.dt = self.cfl / vm
+                xf, yf, zf = s.step_factors(self.dt)
+            s.wavespace_vel(k, self.dt / 2.0)
+            if k == 1:
+                s.proc4a(xf, yf, zf)
+            else:
+                s.proc4b()
         if self.kfor:
             s.ek_force(self.kf_shell, self.kfor, self.ekinf)
         self.dts.append(self.dt)

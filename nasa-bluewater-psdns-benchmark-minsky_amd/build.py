@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",
 ]
-HIP_SOURCES = ["psdns_kernels.hip", "psdns_api.hip"]
+HIP_SOURCES = ["psdns_kernels.hip", "psdns_api.hip", "sgs_kernels.hip", "sgs_api.hip"]
 
 
 def _run(cmd, verbose):

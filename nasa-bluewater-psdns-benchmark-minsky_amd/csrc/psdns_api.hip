@@ -17,6 +17,7 @@
 
 #include "psdns_gpu.h"
 #include "psdns_internal.h"
+#include "sgs_internal.h"
 
 using psdns::Ctx;
 using psdns::Pass;
@@ -1365,3 +1366,31 @@ int psdns_inpen(psdns_ctx* c, const char* path) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Access for the library's other C-ABI files (the SGS closure)
+// ---------------------------------------------------------------------------
+namespace psdns {
+
+int ctx_access(psdns_ctx* c, bool clobber, bool need_ux, Ctx* out, int* n) {
+  if (int rc = clobber ? ready_clobber(c) : ready(c)) return rc;
+  if (need_ux)
+    if (int rc = ensure_ux(c)) return rc;
+  *out = c->c;
+  *n = c->n;
+  return 0;
+}
+
+int ctx_sync(psdns_ctx* c) { return sync(c); }
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+}  // namespace psdns

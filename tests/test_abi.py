@@ -1,6 +1,7 @@
 """C-ABI checks that need no GPU: the library loads, exports every function
-declared in include/psdns_gpu.h, and rejects bad parameters before it
-touches the device; the Fortran interface module and driver are built."""
+declared in include/psdns_gpu.h and include/psdns_sgs.h, and rejects bad
+parameters before it touches the device; the Fortran interface module and
+driver are built."""
 import ctypes
 import pathlib
 import re
@@ -11,10 +12,11 @@ import pytest
 from conftest import REPO, load_package
 
 HEADER = REPO / "include" / "psdns_gpu.h"
+SGS_HEADER = REPO / "include" / "psdns_sgs.h"
 
 
-def declared_functions():
-    text = HEADER.read_text()
+def declared_functions(header=HEADER):
+    text = header.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(psdns_[a-z0-9_]+)\s*\(", text)))
 
@@ -37,11 +39,21 @@ def test_library_exports_every_declared_symbol(P, lib):
     out = subprocess.run(["nm", "-D", "--defined-only", str(P.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
     exported = set(re.findall(r"\bT (psdns_\w+)", out))
-    missing = [f for f in declared_functions() if f not in exported]
+    declared = declared_functions() + declared_functions(SGS_HEADER)
+    missing = [f for f in declared if f not in exported]
     assert not missing, missing
-    for f in declared_functions():
+    for f in declared:
         assert getattr(lib, f) is not None
     assert set(P.EXPORTS) == set(declared_functions())
+    assert set(P.SGS_EXPORTS) == set(declared_functions(SGS_HEADER))
+
+
+def test_sgs_rejects_bad_grids_without_a_device(P, lib):
+    h = ctypes.c_void_p()
+    assert lib.psdns_sgs_create(48, 0, ctypes.byref(h)) == -1
+    assert b"power of two" in lib.psdns_last_error()
+    assert lib.psdns_sgs_create(64, -1, ctypes.byref(h)) == -1
+    assert lib.psdns_sgs_param_count() == 12164
 
 
 @pytest.mark.parametrize("bad,msg", [
