@@ -204,6 +204,16 @@ int psdns_ek_force(psdns_ctx *c, int kf_shell, int kfor, const double *ekinf, do
  * nshell shells; tke and eps over all modes. */
 int psdns_shell_stats(psdns_ctx *c, double *ek, double *dk, int nshell, double *tke, double *eps);
 
+/* psdns_shell_stats without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results (the same kernels, bit for bit); the partial
+ * rows live in a fixed device array of the library sized for n <= 1024
+ * (7.9 MB per GPU, held until the process ends): a larger grid is refused
+ * with PSDNS_EINVAL.  Calls that overlap on one GPU share that array and
+ * race. */
+int psdns_shell_stats_noalloc(psdns_ctx *c, double *ek, double *dk, int nshell, double *tke, double *eps);
+
 /* Velocity-gradient moments (vgmomt.f:68-143): du/dx and dw/dz of the state
  * un (on the mask, unshifted grid) in physical space, dudx[m-1] and
  * dwdz[m-1] = sum of the m-th power over the grid / (nx ny nz), m = 1..8,

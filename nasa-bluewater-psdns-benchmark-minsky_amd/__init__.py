@@ -27,7 +27,8 @@ FIELD_U, FIELD_UN, FIELD_UX = 0, 1, 2
 EXPORTS = (
     "psdns_create", "psdns_destroy", "psdns_geometry", "psdns_columns", "psdns_put_state",
     "psdns_get_state", "psdns_set_shifts", "psdns_stage_nonlinear", "psdns_stage_update",
-    "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_init_field", "psdns_put_field",
+    "psdns_ek_shell", "psdns_ek_force", "psdns_shell_stats", "psdns_shell_stats_noalloc",
+    "psdns_init_field", "psdns_put_field",
     "psdns_get_field", "psdns_phase1", "psdns_itransform_vel", "psdns_realspace_vel", "psdns_transform",
     "psdns_wavespace_vel", "psdns_proc4a", "psdns_proc4b", "psdns_kxtran", "psdns_xktran",
     "psdns_stage_times", "psdns_vg_moments", "psdns_outpen", "psdns_inpen", "psdns_rk4_update",
@@ -95,6 +96,7 @@ def load_library():
         "psdns_ek_shell": [_vp, i, _dp],
         "psdns_ek_force": [_vp, i, i, _dp, _dp],
         "psdns_shell_stats": [_vp, _dp, _dp, i, _dp, _dp],
+        "psdns_shell_stats_noalloc": [_vp, _dp, _dp, i, _dp, _dp],
         "psdns_vg_moments": [_vp, _dp, _dp],
         "psdns_init_field": [_vp, ctypes.c_ulonglong, d, d],
         "psdns_put_field": [_vp, i, i, _vp],
@@ -308,11 +310,14 @@ class Solver:
         self._chk(self.L.psdns_ek_force(self.h, kf_shell, kfor, _dptr(e), _dptr(out)))
         return out
 
-    def shell_stats(self, nshell=None):
+    def shell_stats(self, nshell=None, noalloc=False):
+        """FP64 E(k), D(k), TKE, epsilon; noalloc selects the entry point
+        that allocates nothing (psdns_shell_stats_noalloc)"""
         nshell = self.n if nshell is None else nshell
         ek, dk = np.zeros(max(nshell, 1)), np.zeros(max(nshell, 1))
         t, e = np.zeros(1), np.zeros(1)
-        self._chk(self.L.psdns_shell_stats(self.h, _dptr(ek), _dptr(dk), nshell, _dptr(t), _dptr(e)))
+        f = self.L.psdns_shell_stats_noalloc if noalloc else self.L.psdns_shell_stats
+        self._chk(f(self.h, _dptr(ek), _dptr(dk), nshell, _dptr(t), _dptr(e)))
         return ek[:nshell], dk[:nshell], float(t[0]), float(e[0])
 
     def vg_moments(self):

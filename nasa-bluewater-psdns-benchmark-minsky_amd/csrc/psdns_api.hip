@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,38 @@ int fail(int code, const char* fmt, ...) {
 
 inline int kwave(int i, int n) { return i <= n / 2 ? i : i - n; }
 
+// Partial rows of the FP64 shell statistics for psdns_shell_stats_noalloc,
+// in a file-scope device array instead of a buffer allocated on first use:
+// sized for n <= kScratchMaxN (7.9 MB; every GPU the library runs on holds
+// its own copy until the process ends).  Host code never passes `scratch`
+// itself to a kernel (on the host it names a stand-in, not device memory):
+// it uses the device address hipGetSymbolAddress returns for the current
+// device, looked up once per device by psdns_create.  Every call on one GPU
+// shares the array, and k_shell_partial writes each row it folds, so a call
+// needs no clearing; calls that overlap on one GPU race.
+constexpr int kScratchMaxN = 1024;
+constexpr size_t kScratchDoubles = psdns::shell_part_doubles(kScratchMaxN);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on device dev (the current device)
+int scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(PSDNS_EHIP, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PSDNS_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 struct psdns_ctx {
@@ -75,6 +108,7 @@ struct psdns_ctx {
   double* d_force_terms = nullptr;  // 2 doubles per force-list entry (fixed-order shell sums)
   int force_kfor = -1, nforce = 0;
   double* d_shell_part = nullptr;   // partial rows of psdns_shell_stats, allocated on first use
+  double* d_scratch = nullptr;      // device address of `scratch` on dev (psdns_shell_stats_noalloc)
   int dev = 0;                      // device of the context's stream
   double gsh[6] = {};               // the shift offsets of the last set_shifts
   // Lagrangian particles: pp(np, 9) column-major (FP64) and whether ux[0..2]
@@ -420,6 +454,7 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   } while (0)
 
   CHK(hipGetDevice(&c->dev));
+  if (int rc = scratch_address(c->dev, &c->d_scratch)) return bail(rc);
   CHK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
   const size_t cyl = c->cyl_elems();
   ALLOC(c->d_colx, sizeof(int) * c->na);
@@ -885,6 +920,16 @@ int psdns_shell_stats(psdns_ctx* c, double* ek, double* dk, int nshell, double* 
       return fail(PSDNS_ENOMEM, "hipMalloc of the shell partial rows (%zu bytes) failed", bytes);
   }
   return shell_stats_into(c, c->d_shell_part, c->d_red, ek, dk, nshell, tke, eps);
+}
+
+int psdns_shell_stats_noalloc(psdns_ctx* c, double* ek, double* dk, int nshell, double* tke, double* eps) {
+  if (int rc = ready(c)) return rc;
+  if (int rc = shell_args(c, ek, dk, nshell)) return rc;
+  if (c->n > kScratchMaxN || psdns::shell_part_doubles(c->n) > kScratchDoubles)
+    return fail(PSDNS_EINVAL, "n = %d: psdns_shell_stats_noalloc holds partial rows for n <= %d only", c->n,
+                kScratchMaxN);
+  if (!c->d_scratch) return fail(PSDNS_ESTATE, "no scratch address for device %d", c->dev);
+  return shell_stats_into(c, c->d_scratch, c->d_red, ek, dk, nshell, tke, eps);
 }
 
 int psdns_vg_moments(psdns_ctx* c, double* dudx, double* dwdz) {

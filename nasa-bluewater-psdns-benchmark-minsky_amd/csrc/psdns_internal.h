@@ -95,13 +95,13 @@ void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
 // out[nshell..2 nshell) = D, out[2 nshell] = TKE, out[2 nshell + 1] = eps
 constexpr int kShellBlocks = 1024;
 // shells a mask mode can reach: ik = int(|k| + 1.5) with 9 |k|^2 <= 2 n^2
-inline int shell_rows(int n) {
+constexpr int shell_rows(int n) {
   int k2max = 2 * n * n / 9;
   int r = 0;
   while ((long long)(r + 1) * (r + 1) <= k2max) r++;  // floor(sqrt(k2max))
   return r + 2;
 }
-inline size_t shell_part_doubles(int n) { return (size_t)kShellBlocks * 2 * shell_rows(n); }
+constexpr size_t shell_part_doubles(int n) { return (size_t)kShellBlocks * 2 * shell_rows(n); }
 void launch_shell_stats(const Ctx& c, double viscos, int nshell, double* part, double* out);
 void launch_field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0);
 void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
