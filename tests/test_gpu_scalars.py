@@ -29,7 +29,10 @@ def scalar_ic(g, nc, seed):
     return phi
 
 
-@pytest.mark.parametrize("n,prec,nsteps", [(32, 8, 1), (64, 8, 3), (64, 4, 3)])
+# 128^3 and 256^3: work-groups of several waves in the scalar x pass and
+# the gradient y pass (at 32^3 and 64^3 each is one partial wave); 256^3
+# FP32 over 10 steps is the two-scalar configuration's parity case
+@pytest.mark.parametrize("n,prec,nsteps", [(32, 8, 1), (64, 8, 3), (64, 4, 3), (128, 8, 2), (256, 4, 10)])
 def test_scalar_steps_vs_oracle(O, P, n, prec, nsteps):
     banner("scalars nc=2", n, prec, nsteps)
     g = O.Grid(n)
@@ -76,5 +79,40 @@ def test_scalar_pure_diffusion_on_gpu(O, P):
         kx, ky, kz = kgrid(n, g.col_x, g.col_z)
         ref = phi0[0] * np.exp(-nu / pr * (kx * kx + ky * ky + kz * kz) * 3 * dt)
         assert rel(s.get_state()[3], ref, g.mask()) < 1e-13
+    finally:
+        s.close()
+
+
+def test_scalars_1024_fp32_properties(P):
+    """One RK2 step at the full 1024^3 FP32 size with two scalars (Sc = 1,
+    mean gradient (1, 0, 0), zero initial fluctuation): the scalars keep
+    zero mean and a Hermitian kx = kz = 0 column, and to first order in dt
+    each equals -dt (u_x^n + u_x^(n+1)) / 2 (the mean-gradient source
+    integrated by the RK2 step; advection of the scalar and diffusion are
+    O(dt |grad u|) corrections)."""
+    banner("1024^3 fp32 two scalars, one step")
+    n = 1024
+    s = P.Solver(n, 4, viscos=3.2e-5, nc=2, pr=[1.0, 1.0], grad=[(1.0, 0.0, 0.0), (1.0, 0.0, 0.0)])
+    try:
+        s.init_field()
+        ux0 = s.get_state()[0].copy()
+        rg = P.Run(s, 3.2e-5, np.pi)
+        rg.step(np.random.default_rng(1).random(6))
+        st = s.get_state()
+        cx, cz, _ = s.columns()
+        a0 = int(np.nonzero((cx == 0) & (cz == 0))[0][0])
+        dt = rg.dts[-1]
+        pred = -0.5 * dt * (ux0 + st[0])
+        del ux0
+        for i in (3, 4):
+            phi = st[i]
+            assert np.abs(phi[0, 0]) == 0.0
+            col = phi[a0, :]
+            mirror = np.conj(col[(n - np.arange(n)) % n])
+            assert np.abs(col - mirror).max() < 1e-5 * np.abs(col).max()
+            err = np.linalg.norm(phi - pred) / np.linalg.norm(pred)
+            print(f"scalar {i - 2}: |phi + dt (u_x^n + u_x^n+1)/2| / |.| = {err:.3e} (dt {dt:.3e})", flush=True)
+            assert err < 0.1
+        assert np.array_equal(st[3], st[4])  # same Sc and gradient: same bits
     finally:
         s.close()

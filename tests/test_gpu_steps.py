@@ -82,7 +82,7 @@ def skewness(g, uy, cx, cz):
     return float(np.mean(out))
 
 
-@pytest.mark.parametrize("n,prec,nsteps", [(256, 4, 10), (256, 8, 10), (512, 4, 10)])
+@pytest.mark.parametrize("n,prec,nsteps", [(256, 4, 10), (256, 8, 10), (512, 4, 10), (512, 8, 10)])
 def test_forced_steps_vs_fp64_oracle(O, P, n, prec, nsteps):
     """Forced isotropic turbulence at 256^3 and 512^3, viscosity from
     kmax eta = 1.5, FEK forcing of shells k <= 2, RK2 steps with phase
