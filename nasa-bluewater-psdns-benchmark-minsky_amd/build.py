@@ -50,20 +50,24 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build_library(verbose=False):
-    obj = PKG / "_build"
+def build_library(verbose=False, defines=(), suffix=""):
+    """lib/libpsdns_gpu.so, or with compile-time defines (tuning variants,
+    e.g. PSDNS_XMFMA=1) lib/libpsdns_gpu_<suffix>.so for PSDNS_GPU_LIB"""
+    obj = PKG / ("_build" + ("_" + suffix if suffix else ""))
     obj.mkdir(exist_ok=True)
     LIB.mkdir(exist_ok=True)
+    flags = HIP_FLAGS + [f"-D{d}" for d in defines]
     objs = [obj / (s.rsplit(".", 1)[0] + ".o") for s in HIP_SOURCES]
     with cf.ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
-        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-c", CSRC / s, "-o", o], verbose) for s, o in zip(HIP_SOURCES, objs)]
+        futs = [ex.submit(_run, [HIPCC, *flags, "-c", CSRC / s, "-o", o], verbose) for s, o in zip(HIP_SOURCES, objs)]
         for f in futs:
             f.result()
-    tmp = LIB / "libpsdns_gpu.so.tmp"
+    name = "libpsdns_gpu" + ("_" + suffix if suffix else "") + ".so"
+    tmp = LIB / (name + ".tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     _check_defined(tmp)
-    os.replace(tmp, LIB / "libpsdns_gpu.so")
-    return LIB / "libpsdns_gpu.so"
+    os.replace(tmp, LIB / name)
+    return LIB / name
 
 
 def _check_defined(so):
@@ -98,7 +102,14 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    so, exe = build(verbose="-v" in sys.argv)
-    print(so)
-    if exe:
-        print(exe)
+    args = sys.argv[1:]
+    if "--define" in args:
+        # a tuning variant of the library only: --define NAME=VALUE [--define ...] --suffix S
+        defs = [args[i + 1] for i, a in enumerate(args) if a == "--define"]
+        suffix = args[args.index("--suffix") + 1]
+        print(build_library("-v" in args, defs, suffix))
+    else:
+        so, exe = build(verbose="-v" in args)
+        print(so)
+        if exe:
+            print(exe)

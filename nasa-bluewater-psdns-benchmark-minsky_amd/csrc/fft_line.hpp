@@ -164,6 +164,72 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl<0, N>(f);
 }
 
+// Radix-8 butterflies of one wavefront on the matrix cores: the real 16 x
+// 16 matrix of the complex DFT-8 (rows: output (re, im) pairs, columns:
+// input (re, im) pairs) times the 16 reals of each of the wave's 64
+// butterflies, in four blocks of 16 butterflies.  In each block, lane l
+// (column c = l % 16, group g = l / 16) supplies the reals 4 g .. 4 g + 3 of
+// butterfly c (inputs 2 g and 2 g + 1) and receives the same reals of its
+// outputs (the operand and accumulator layouts of the 16 x 16 MFMA forms).
+// MX = 1: v_mfma_f32_16x16x4_f32, four K-steps of one real each (exact
+// FP32 products); MX = 2: split bf16 x 3 (x = hi + lo in bf16, D likewise;
+// Dh xh + Dh xl + Dl xh with v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+typedef float mx_f32x4 __attribute__((ext_vector_type(4)));
+typedef short mx_bf16x4 __attribute__((ext_vector_type(4)));
+
+template <int SIGN>
+__device__ __forceinline__ float dft8_entry(int row, int col) {
+  // row = 2 k + (re|im), col = 2 r + (re|im): w = exp(SIGN 2 pi i k r / 8)
+  const int k = row >> 1, r = col >> 1, e = (2 * k * r) & 15;
+  const float c = (float)kCos16[e], sn = (float)(SIGN * kSin16[e]);
+  const int pr = row & 1, pc = col & 1;
+  return pr == 0 ? (pc == 0 ? c : -sn) : (pc == 0 ? sn : c);
+}
+
+__device__ __forceinline__ short bf16_bits(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+__device__ __forceinline__ float bf16_value(short b) { return (float)__builtin_bit_cast(__bf16, b); }
+
+template <int SIGN, int MX>
+struct MfmaDft8 {
+  // per-lane operand of the DFT matrix
+  float a[4];
+  mx_bf16x4 ah, al;
+  __device__ __forceinline__ explicit MfmaDft8(int lane) {
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 4; j++) a[j] = dft8_entry<SIGN>(c, 4 * g + j);
+    if constexpr (MX == 2) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        ah[j] = bf16_bits(a[j]);
+        al[j] = bf16_bits(a[j] - bf16_value(ah[j]));
+      }
+    }
+  }
+  // x0, x1: inputs 2 g, 2 g + 1 of this lane's butterfly; returns outputs 2 g, 2 g + 1
+  __device__ __forceinline__ mx_f32x4 run(cpx<float> x0, cpx<float> x1) const {
+    mx_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (MX == 1) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x0.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, acc, 0, 0, 0);
+    } else {
+      const float xv[4] = {x0.x, x0.y, x1.x, x1.y};
+      mx_bf16x4 xh, xl;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        xh[j] = bf16_bits(xv[j]);
+        xl[j] = bf16_bits(xv[j] - bf16_value(xh[j]));
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, xh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, xl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, xh, acc, 0, 0, 0);
+    }
+    return acc;
+  }
+};
+
 // B lines of length L, E points per thread, LDS planes re [B][S], im [B][S].
 // Padding of the buffer written by the stage of span NSW (product of the
 // earlier radices), which writes chunks of NSW consecutive points with chunk
@@ -171,7 +237,9 @@ __device__ __forceinline__ void static_for(F&& f) {
 // 32; NSW < 32: NSW pad words every NSW*RAD points, so that the chunks of a
 // wave start on different banks; NSW >= 32: none.  SX extra words per line
 // stride shift the banks of neighbouring lines for the transposed mapping.
-template <typename R, int L, int E, int B, int SIGN, int SX = 0>
+// MX: the radix-8 stages that read and write LDS on the matrix cores
+// (MfmaDft8) when a line is one wavefront (FP32, E = 8, T = 64); 0 = VALU.
+template <typename R, int L, int E, int B, int SIGN, int SX = 0, int MX = 0>
 struct TeamFft {
   static_assert(L > E, "at least two Stockham stages");
   static constexpr int T = L / E;
@@ -324,8 +392,50 @@ struct TeamFft {
   // CROSS_IN: the buffer read was written in another mapping than the
   // natural one (the first middle stage after a transposed first stage, or
   // a last stage reading in the transposed mapping)
+  static constexpr bool kMfma = MX != 0 && std::is_same<R, float>::value && E == 8 && T == 64;
+
+  // the radix-8 stage of span NS on the matrix cores (LDS in and out): the
+  // wave's line, butterflies j = 16 nb + c in four blocks
+  template <int NS, bool CROSS_IN>
+  static __device__ __forceinline__ void stage_mfma(Slot<R> s, const cpx<R>* stw) {
+    constexpr int RAD = 8;
+    const int lane = s.q, g = lane >> 4, c = lane & 15;
+    const MfmaDft8<SIGN, MX> dft(lane);
+    cpx<R> w0[4], w1[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const int jm = (16 * nb + c) & (NS - 1);
+      w0[nb] = g == 0 ? cpx<R>{1, 0} : tw_at<NS, RAD>(stw, jm, g == 0 ? 1 : 2 * g);
+      w1[nb] = tw_at<NS, RAD>(stw, jm, 2 * g + 1);
+    }
+    sync<CROSS_IN>();
+    mx_f32x4 y[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const int j = 16 * nb + c;
+      const int p0 = pos<NS / E>(j + 2 * g * (L / RAD)), p1 = pos<NS / E>(j + (2 * g + 1) * (L / RAD));
+      const cpx<R> x0 = cmul(cpx<R>{s.re[p0], s.im[p0]}, w0[nb]);
+      const cpx<R> x1 = cmul(cpx<R>{s.re[p1], s.im[p1]}, w1[nb]);
+      y[nb] = dft.run(x0, x1);
+    }
+    sync<false>();  // this stage's reads of the line are done
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const int j = 16 * nb + c, jm = j & (NS - 1);
+      const int pb = pos<NS>((int)((unsigned)j / NS) * NS * RAD + jm);
+      s.re[pb + 2 * g * NS] = y[nb][0];
+      s.im[pb + 2 * g * NS] = y[nb][1];
+      s.re[pb + (2 * g + 1) * NS] = y[nb][2];
+      s.im[pb + (2 * g + 1) * NS] = y[nb][3];
+    }
+  }
+
   template <int RAD, int NS, int OUT, bool CROSS_IN>
   static __device__ __forceinline__ void stage(cpx<R>* v, Slot<R> s, const cpx<R>* stw) {
+    if constexpr (kMfma && RAD == 8 && OUT == 0) {
+      stage_mfma<NS, CROSS_IN>(s, stw);
+      return;
+    }
     constexpr int NB = E / RAD;  // butterflies per thread
     cpx<R> w[E];
     // the twiddle table is constant: its reads are issued ahead of the

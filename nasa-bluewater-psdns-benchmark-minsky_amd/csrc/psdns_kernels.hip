@@ -446,8 +446,14 @@ struct XPass {
   // LDS of 16 lines leaves one work-group per CU)
   static constexpr int B = K;
   static constexpr int NT = B * T;
-  using FI = TeamFft<R, M, E, B, +1, LineShift<E, B>::IN>;
-  using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN>;
+#ifndef PSDNS_XMFMA
+#define PSDNS_XMFMA 0
+#endif
+  // radix-8 stages on the matrix cores: 0 VALU (default), 1 FP32 MFMA,
+  // 2 split bf16 x 3 (a tuning build: build.py --define PSDNS_XMFMA=1)
+  static constexpr int MX = PSDNS_XMFMA;
+  using FI = TeamFft<R, M, E, B, +1, LineShift<E, B>::IN, MX>;
+  using FF = TeamFft<R, M, E, B, -1, LineShift<E, B>::IN, MX>;
   static constexpr int kLdsReals = FI::kLdsReals;
   static constexpr int kTw = FI::kTwEntries;  // shared by FI and FF
 #ifndef PSDNS_XFRESH
