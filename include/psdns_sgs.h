@@ -72,6 +72,15 @@ int psdns_sgs_filter(psdns_sgs *m, psdns_ctx *fine, psdns_ctx *coarse);
 /* Features of the context's cylinder u slots 0..2 as they are: unshifted
  * after psdns_xktran, or on the stage's shifted grid after psdns_phase1 */
 int psdns_sgs_features(psdns_sgs *m, psdns_ctx *c);
+/* Features of the context's state u on stage kstep's shifted grid (1|2:
+ * the phase shift of psdns_set_shifts applied on the fly, as the fused
+ * step's phase1): for an LES stage through the fused entry points */
+int psdns_sgs_features_stage(psdns_sgs *m, psdns_ctx *c, int kstep);
+/* LES stage through the fused entry points: the predicted stress is added
+ * to the products by the next psdns_stage_nonlinear(les, kstep) (in its x
+ * pass; used once, dropped by any call that overwrites the work arrays or
+ * the shifts).  The closure must stay alive until then. */
+int psdns_sgs_attach_stress(psdns_sgs *m, psdns_ctx *les, int kstep);
 /* host data as the current features (5 + 24 fields) and target (6; nullable) */
 int psdns_sgs_put_fields(psdns_sgs *m, const float *lambda, const float *basis, const float *tau);
 int psdns_sgs_get_field(psdns_sgs *m, int which, float *out);
@@ -93,8 +102,9 @@ int psdns_sgs_gradient(psdns_sgs *m, int batch, unsigned long long seed, long lo
  * minibatch of (seed, t); losses[nsteps] (nullable) the minibatch losses */
 int psdns_sgs_train(psdns_sgs *m, int nsteps, int batch, double lr, unsigned long long seed, double sigma2,
                     double *losses);
-/* predicted stress of the current features: FP32 (bf16 = 0) or bf16
- * operands on the matrix cores with FP32 accumulation (bf16 = 1) */
+/* predicted stress of the current features: the convolution in FP32, then
+ * the dense layers in FP32 (bf16 = 0) or with bf16 operands on the matrix
+ * cores and FP32 accumulation (bf16 = 1) */
 int psdns_sgs_predict(psdns_sgs *m, int bf16);
 /* loss of the prediction against the current target over all points */
 int psdns_sgs_eval(psdns_sgs *m, double sigma2, double *loss);

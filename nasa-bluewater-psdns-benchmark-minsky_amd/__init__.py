@@ -41,7 +41,7 @@ SGS_EXPORTS = (
     "psdns_sgs_get_params", "psdns_sgs_filter", "psdns_sgs_features", "psdns_sgs_put_fields",
     "psdns_sgs_get_field", "psdns_sgs_add_snapshot", "psdns_sgs_snapshots", "psdns_sgs_clear_snapshots",
     "psdns_sgs_target_scale", "psdns_sgs_gradient", "psdns_sgs_train", "psdns_sgs_predict", "psdns_sgs_eval",
-    "psdns_sgs_add_stress",
+    "psdns_sgs_add_stress", "psdns_sgs_features_stage", "psdns_sgs_attach_stress",
 )
 SGS_LAMBDA, SGS_BASIS, SGS_TARGET, SGS_PREDICTED, SGS_GRADIENT = range(5)
 
@@ -141,6 +141,8 @@ def load_library():
         "psdns_sgs_predict": [_vp, i],
         "psdns_sgs_eval": [_vp, d, _dp],
         "psdns_sgs_add_stress": [_vp, _vp],
+        "psdns_sgs_features_stage": [_vp, _vp, i],
+        "psdns_sgs_attach_stress": [_vp, _vp, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -530,8 +532,17 @@ class Closure:
         """box filter of fine's ux slots 0..2 into coarse's ux slots 0..2 + the target"""
         self._chk(self.L.psdns_sgs_filter(self.h, fine.h, coarse.h))
 
-    def features(self, solver):
-        self._chk(self.L.psdns_sgs_features(self.h, solver.h))
+    def features(self, solver, kstep=0):
+        """features of solver's state u: as it is (kstep 0) or on stage
+        kstep's shifted grid (1|2)"""
+        if kstep:
+            self._chk(self.L.psdns_sgs_features_stage(self.h, solver.h, kstep))
+        else:
+            self._chk(self.L.psdns_sgs_features(self.h, solver.h))
+
+    def attach(self, solver, kstep):
+        """the predicted stress goes into the next stage_nonlinear(kstep)"""
+        self._chk(self.L.psdns_sgs_attach_stress(self.h, solver.h, kstep))
 
     def put_fields(self, lam, basis, tau=None):
         la = np.ascontiguousarray(lam, np.float32)
@@ -598,20 +609,25 @@ def training_snapshot(closure, fine, coarse):
 
 
 class Les:
-    """RK2 time loop of a coarse-grid LES: the per-routine step
-    (phase1, itransform_vel, realspace_vel, transform, wavespace_vel,
-    proc4a/proc4b per stage, then FEK forcing) with the closure's stress
-    added to the products.  closure None runs the same grid without a model."""
+    """RK2 time loop of a coarse-grid LES with the closure's stress added to
+    the products; closure None runs the same grid without a model.
+    fused: the fused entry points (stage_nonlinear / stage_update, the
+    stress added in the x pass, features on the stage's shifted grid);
+    otherwise the per-routine step (phase1, itransform_vel, realspace_vel,
+    transform, wavespace_vel, proc4a/proc4b per stage).  FEK forcing after
+    the step."""
 
-    def __init__(self, solver, viscos, cfl, closure=None, bf16=True, kf_shell=0, ekinf=None):
+    def __init__(self, solver, viscos, cfl, closure=None, bf16=True, kf_shell=0, ekinf=None, fused=False):
         self.s, self.viscos, self.cfl = solver, viscos, cfl
-        self.closure, self.bf16 = closure, bf16
+        self.closure, self.bf16, self.fused = closure, bf16, fused
         self.kf_shell = kf_shell
         self.kfor = kf_shell + 1 if kf_shell > 0 else 0
         self.ekinf = ekinf if ekinf is not None or not self.kfor else solver.ek_shell(self.kfor)
         self.dts = []
 
     def step(self, gsh):
+        if self.fused:
+            return self._step_fused(gsh)
         s, m = self.s, self.closure
         s.set_shifts(gsh)
         for k in (1, 2):
@@ -633,6 +649,24 @@ class Les:
                 s.proc4a(xf, yf, zf)
             else:
                 s.proc4b()
+        if self.kfor:
+            s.ek_force(self.kf_shell, self.kfor, self.ekinf)
+        self.dts.append(self.dt)
+
+    def _step_fused(self, gsh):
+        s, m = self.s, self.closure
+        s.set_shifts(gsh)
+        for k in (1, 2):
+            if m is not None:
+                m.features(s, k)
+                m.predict(self.bf16)
+                m.attach(s, k)
+            vm = s.stage_nonlinear(k)
+            if k == 1:
+                self.velmax = vm
+                self.dt = self.cfl / vm
+                xf, yf, zf = s.step_factors(self.dt)
+            s.stage_update(k, self.dt, xf, yf, zf)
         if self.kfor:
             s.ek_force(self.kf_shell, self.kfor, self.ekinf)
         self.dts.append(self.dt)

@@ -109,6 +109,10 @@ struct psdns_ctx {
   int force_kfor = -1, nforce = 0;
   double* d_shell_part = nullptr;   // partial rows of psdns_shell_stats, allocated on first use
   double* d_scratch = nullptr;      // device address of `scratch` on dev (psdns_shell_stats_noalloc)
+  // subgrid stress for the fused x pass of the next stage_nonlinear(sgs_stage)
+  // (an LES stage; registered by the SGS closure, used once)
+  const float* sgs_tau = nullptr;
+  int sgs_stage = 0;
   int dev = 0;                      // device of the context's stream
   double gsh[6] = {};               // the shift offsets of the last set_shifts
   // Lagrangian particles: pp(np, 9) column-major (FP64) and whether ux[0..2]
@@ -357,6 +361,7 @@ int ready_clobber(psdns_ctx* c) {
   if (int rc = ready(c)) return rc;
   c->last_nonlinear = 0;
   c->spline_ok = false;
+  c->sgs_tau = nullptr;
   return 0;
 }
 
@@ -639,7 +644,12 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if ((rc = mark(c, 1))) return rc;
   if ((rc = pass(c, Pass::ZInv, 3, fused ? 1 : 0))) return rc;
   if ((rc = mark(c, 2))) return rc;
-  if ((rc = pass(c, Pass::XFused, kstep == 1))) return rc;
+  // an LES stage: the stress registered for this stage goes into the products
+  c->c.tau = c->sgs_tau && c->sgs_stage == kstep ? c->sgs_tau : nullptr;
+  c->sgs_tau = nullptr;
+  rc = pass(c, Pass::XFused, kstep == 1);
+  c->c.tau = nullptr;
+  if (rc) return rc;
   if ((rc = mark(c, 3))) return rc;
   if ((rc = pass(c, Pass::ZFwdConv, 5))) return rc;
   if ((rc = mark(c, 4))) return rc;
@@ -1427,6 +1437,13 @@ int ctx_access(psdns_ctx* c, bool clobber, bool need_ux, Ctx* out, int* n) {
 }
 
 int ctx_sync(psdns_ctx* c) { return sync(c); }
+
+int ctx_set_stress(psdns_ctx* c, const float* tau, int kstep) {
+  if (int rc = check_ctx(c)) return rc;
+  c->sgs_tau = tau;
+  c->sgs_stage = kstep;
+  return 0;
+}
 
 int set_error(int code, const char* fmt, ...) {
   char buf[512];

@@ -58,6 +58,9 @@ struct Ctx {
   unsigned long long* velmax_bits;  // kVelmaxSlots slots, kVelmaxStride apart
   const int2* force_list;  // (a, y) entries inside the FEK forcing sphere
   int nforce;
+  // physical subgrid stress [6][n][n][n] (11, 12, 13, 22, 23, 33; FP32) that
+  // the fused x pass adds to its products (an LES stage), or nullptr
+  const float* tau = nullptr;
 };
 
 enum class Pass {

@@ -735,10 +735,13 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
 // Persistent over tiles (one y, B consecutive z): the staging copy of the
 // next variable is issued before each inverse transform, and that of the
 // next tile's u before the five forward transforms.
-template <typename R, int N>
+// TAU (an LES stage): the subgrid stress tau[6][n][n][n] is added to the
+// products, u_i u_j -> u_i u_j + tau_ij in the flux, i.e. slot 0 + tau11 -
+// tau22, 1 + tau12, 2 + tau33 - tau22, 3 + tau23, 4 + tau13.
+template <typename R, int N, bool TAU>
 __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
     k_x_fused(const C<R>* __restrict__ xi, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
-              unsigned long long* __restrict__ velmax, int want_velmax) {
+              unsigned long long* __restrict__ velmax, int want_velmax, const float* __restrict__ tau) {
   using X = XPass<R, N>;
   constexpr int E = X::E, B = X::B;
   __shared__ R lds[X::kLdsReals];
@@ -781,6 +784,23 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
 #pragma unroll
         for (int j = 0; j < E; j++)
           q[j] = {product(sl, pu[j].x, pv[j].x, pw[j].x), product(sl, pu[j].y, pv[j].y, pw[j].y)};
+        if constexpr (TAU) {
+          // this thread's points: x = 2 i, 2 i + 1, i = q + T j, of line z0 + line
+          constexpr size_t P3 = (size_t)N * N * N;
+          const int line = X::FI::natural_line(threadIdx.x), qq = (int)(threadIdx.x % X::T);
+          const float* base = tau + ((size_t)y * N + z0 + line) * N + 2 * qq;
+          constexpr int ca[5] = {0, 1, 5, 4, 2}, cb[5] = {3, -1, 3, -1, -1};  // + tau[ca] - tau[cb]
+#pragma unroll
+          for (int j = 0; j < E; j++) {
+            const float2 a = *reinterpret_cast<const float2*>(base + ca[sl] * P3 + 2 * X::T * j);
+            float2 t = a;
+            if (cb[sl] >= 0) {
+              const float2 b = *reinterpret_cast<const float2*>(base + cb[sl] * P3 + 2 * X::T * j);
+              t = make_float2(a.x - b.x, a.y - b.y);
+            }
+            q[j] = {q[j].x + (R)t.x, q[j].y + (R)t.y};
+          }
+        }
         X::template r2c<false>(q, xs, sl, y, z0, g, lds, stw, xtw, (R)1);
       }
       if (!more) break;
@@ -2155,10 +2175,17 @@ struct Launch {
   }
   static void x_fused(const Ctx& c, int want_velmax) {
     using X = XPass<R, N>;
-    const auto kern = k_x_fused<R, N>;
-    static const unsigned grid = persistent_grid((const void*)kern, X::NT);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
-                       (const CR*)c.tw, c.velmax_bits, want_velmax);
+    if (c.tau) {
+      const auto kern = k_x_fused<R, N, true>;
+      static const unsigned grid = persistent_grid((const void*)kern, X::NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
+                         (const CR*)c.tw, c.velmax_bits, want_velmax, c.tau);
+    } else {
+      const auto kern = k_x_fused<R, N, false>;
+      static const unsigned grid = persistent_grid((const void*)kern, X::NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
+                         (const CR*)c.tw, c.velmax_bits, want_velmax, (const float*)nullptr);
+    }
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
     constexpr int NT = ZFwd<R, N>::NT;

@@ -57,6 +57,7 @@ struct psdns_sgs {
   int A_prec = 0;     // precision of the gradient in A (that of the last features context)
   // current field: features, target, prediction
   float *lam = nullptr, *T = nullptr, *tau = nullptr, *tau_hat = nullptr;
+  float* h0 = nullptr;  // the convolution's 16 channels of the last prediction
   bool have_features = false, have_target = false, have_pred = false;
   // training store
   float *lam_s = nullptr, *T_s = nullptr, *tau_s = nullptr;
@@ -167,6 +168,7 @@ int psdns_sgs_create(int n, int max_snapshots, psdns_sgs** out) {
   const size_t P = m->P;
   if ((rc = dalloc(&m->lam, 5 * P, "the invariants")) || (rc = dalloc(&m->T, 24 * P, "the tensor basis")) ||
       (rc = dalloc(&m->tau, 6 * P, "the target stress")) || (rc = dalloc(&m->tau_hat, 6 * P, "the predicted stress")) ||
+      (rc = dalloc(&m->h0, 16 * P, "the convolution channels")) ||
       (rc = dalloc(&m->theta, S::kParams, "theta")) || (rc = dalloc(&m->m, S::kParams, "the Adam moments")) ||
       (rc = dalloc(&m->v, S::kParams, "the Adam moments")) || (rc = dalloc(&m->grad, S::kParams, "the gradient")))
     return bail(rc);
@@ -188,7 +190,7 @@ int psdns_sgs_create(int n, int max_snapshots, psdns_sgs** out) {
 int psdns_sgs_destroy(psdns_sgs* m) {
   if (!m) return PSDNS_OK;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  void* ptrs[] = {m->cyl, m->yw, m->xi, m->A, m->lam, m->T, m->tau, m->tau_hat, m->lam_s, m->T_s,
+  void* ptrs[] = {m->cyl, m->yw, m->xi, m->A, m->lam, m->T, m->tau, m->tau_hat, m->h0, m->lam_s, m->T_s,
                   m->tau_s, m->theta, m->m, m->v, m->grad, m->work, m->lpart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -233,7 +235,11 @@ int psdns_sgs_filter(psdns_sgs* m, psdns_ctx* fine, psdns_ctx* coarse) {
   return PSDNS_OK;
 }
 
-int psdns_sgs_features(psdns_sgs* m, psdns_ctx* c) {
+}  // extern "C"
+
+namespace {
+
+int features(psdns_sgs* m, psdns_ctx* c, int stage) {
   if (!m || !c) return set_error(PSDNS_EINVAL, "null argument");
   Ctx k;
   int n, rc;
@@ -256,7 +262,7 @@ int psdns_sgs_features(psdns_sgs* m, psdns_ctx* c) {
   g.w1 = m->yw;
   g.xi = m->xi;
   g.ux = m->A;
-  S::launch_grad_spec(k, k.u, m->cyl);
+  S::launch_grad_spec(k, k.u, m->cyl, stage);
   if (psdns::launch_pass(g, psdns::Pass::YInvPlain, 9, 0) || psdns::launch_pass(g, psdns::Pass::ZInv, 9, 0) ||
       psdns::launch_pass(g, psdns::Pass::XC2R, 9, 0))
     return set_error(PSDNS_EINVAL, "unsupported grid n = %d", n);
@@ -268,6 +274,28 @@ int psdns_sgs_features(psdns_sgs* m, psdns_ctx* c) {
   m->have_features = true;
   m->have_pred = false;
   return PSDNS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psdns_sgs_features(psdns_sgs* m, psdns_ctx* c) { return features(m, c, 0); }
+
+int psdns_sgs_features_stage(psdns_sgs* m, psdns_ctx* c, int kstep) {
+  if (kstep != 1 && kstep != 2) return set_error(PSDNS_EINVAL, "kstep = %d: stage 1 or 2", kstep);
+  return features(m, c, kstep);
+}
+
+int psdns_sgs_attach_stress(psdns_sgs* m, psdns_ctx* les, int kstep) {
+  if (!m || !les) return set_error(PSDNS_EINVAL, "null argument");
+  if (kstep != 1 && kstep != 2) return set_error(PSDNS_EINVAL, "kstep = %d: stage 1 or 2", kstep);
+  if (!m->have_pred) return set_error(PSDNS_ESTATE, "no predicted stress (psdns_sgs_predict)");
+  Ctx k;
+  int n, rc;
+  if ((rc = psdns::ctx_access(les, false, false, &k, &n))) return rc;
+  if (n != m->n) return set_error(PSDNS_EINVAL, "grid %d: the closure is for n = %d", n, m->n);
+  return psdns::ctx_set_stress(les, m->tau_hat, kstep);
 }
 
 int psdns_sgs_put_fields(psdns_sgs* m, const float* lam, const float* basis, const float* tau) {
@@ -391,7 +419,7 @@ int psdns_sgs_train(psdns_sgs* m, int nsteps, int batch, double lr, unsigned lon
 int psdns_sgs_predict(psdns_sgs* m, int bf16) {
   if (!m) return set_error(PSDNS_EINVAL, "null argument");
   if (!m->have_features) return set_error(PSDNS_ESTATE, "no features (psdns_sgs_features)");
-  S::launch_predict(m->stream, bf16 != 0, m->lam, m->T, m->n, m->theta, m->tau_hat);
+  S::launch_predict(m->stream, bf16 != 0, m->lam, m->T, m->n, m->theta, m->h0, m->tau_hat);
   SGSCHK(hipGetLastError());
   if (int rc = sync(m)) return rc;
   m->have_pred = true;

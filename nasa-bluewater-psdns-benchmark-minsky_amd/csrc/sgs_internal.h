@@ -18,6 +18,10 @@ namespace psdns {
 // (the physical array allocated when need_ux), and its synchronisation.
 int ctx_access(psdns_ctx* c, bool clobber, bool need_ux, Ctx* out, int* n);
 int ctx_sync(psdns_ctx* c);
+// registers tau [6][n][n][n] for the fused x pass of the next
+// stage_nonlinear(kstep) of c (used once; any call that overwrites the work
+// arrays or shifts drops it)
+int ctx_set_stress(psdns_ctx* c, const float* tau, int kstep);
 // sets the thread's last-error message and returns code
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
@@ -44,10 +48,13 @@ inline size_t work_floats(int B) {
 }
 
 void launch_box_filter(hipStream_t st, int prec, const void* u, int n, int r, void* ubar, float* tau);
-void launch_grad_spec(const Ctx& c, const void* u, void* d);
+// i k_j u_i on the mask; with a stage (1|2) the state is first phase shifted
+// by that stage's factors (phase1 on the fly), stage 0 takes u as it is
+void launch_grad_spec(const Ctx& c, const void* u, void* d, int stage);
 void launch_invariants(hipStream_t st, int prec, const void* A, size_t P, double delta2, float* lam, float* T);
+// conv (FP32) into h0[16][P], then the dense layers (FP32 or bf16 matrix cores)
 void launch_predict(hipStream_t st, bool bf16, const float* lam, const float* T, int n, const float* theta,
-                    float* tau);
+                    float* h0, float* tau);
 void launch_add_stress(hipStream_t st, int prec, void* ux, const float* tau, size_t P);
 int field_err_blocks();
 void launch_field_err(hipStream_t st, const float* tp, const float* tt, size_t P, double* part);

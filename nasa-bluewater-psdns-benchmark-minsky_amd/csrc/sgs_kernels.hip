@@ -100,7 +100,9 @@ __global__ void __launch_bounds__(256)
 // phase1 in the LES stage, unshifted for training data).
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(256) k_grad_spec(const cpx<R>* __restrict__ u, cpx<R>* __restrict__ d, Geo g) {
+__global__ void __launch_bounds__(256)
+    k_grad_spec(const cpx<R>* __restrict__ u, cpx<R>* __restrict__ d, Geo g, const double2* __restrict__ sx,
+                const double2* __restrict__ sy, const double2* __restrict__ sz) {
   const size_t tot = (size_t)g.na * g.n;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const int a = (int)(i / g.n), y = (int)(i % g.n);
@@ -108,9 +110,17 @@ __global__ void __launch_bounds__(256) k_grad_spec(const cpx<R>* __restrict__ u,
     const int ky = y <= g.n / 2 ? y : y - g.n, kz = z <= g.n / 2 ? z : z - g.n;
     const bool m = 9 * (kx * kx + ky * ky + kz * kz) <= 2 * g.n * g.n && y != g.n / 2 && z != g.n / 2;
     const R k[3] = {(R)kx, (R)ky, (R)kz};
+    // phase1 of the stage (phshift.f:71-135): u sx(kx) sz(kz) sy(ky), in FP64
+    double2 sh = make_double2(1.0, 0.0);
+    if (sx) {
+      const double2 a2 = sx[kx], b2 = sz[z], c2 = sy[y];
+      const double2 ab = make_double2(a2.x * b2.x - a2.y * b2.y, a2.x * b2.y + a2.y * b2.x);
+      sh = make_double2(ab.x * c2.x - ab.y * c2.y, ab.x * c2.y + ab.y * c2.x);
+    }
 #pragma unroll
     for (int v = 0; v < 3; v++) {
-      const cpx<R> c = m ? u[v * tot + i] : cpx<R>{0, 0};
+      cpx<R> c = m ? u[v * tot + i] : cpx<R>{0, 0};
+      if (sx) c = {(R)((double)c.x * sh.x - (double)c.y * sh.y), (R)((double)c.x * sh.y + (double)c.y * sh.x)};
 #pragma unroll
       for (int j = 0; j < 3; j++) d[(3 * v + j) * tot + i] = {-k[j] * c.y, k[j] * c.x};
     }
@@ -177,10 +187,11 @@ __global__ void __launch_bounds__(256)
 }
 
 // conv input k of point (y, z, x): lambda[k / 27] at the neighbour d = k % 27
+// (n a power of two: the periodic wrap is a mask)
 __device__ __forceinline__ float conv_in(const float* __restrict__ lam, size_t P, int n, int y, int z, int x, int k) {
   const int ch = k / kTaps, d = k % kTaps;
-  const int dy = d / 9 - 1, dz = (d / 3) % 3 - 1, dx = d % 3 - 1;
-  return lam[ch * P + pidx((y + dy + n) % n, (z + dz + n) % n, (x + dx + n) % n, n)];
+  const int dy = d / 9 - 1, dz = (d / 3) % 3 - 1, dx = d % 3 - 1, w = n - 1;
+  return lam[ch * P + pidx((y + dy) & w, (z + dz) & w, (x + dx) & w, n)];
 }
 
 // ---------------------------------------------------------------------------
@@ -318,58 +329,107 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// Inference, FP32: one thread per point, theta in LDS; tau_hat[6][P].
+// Inference.  k_conv: the 3x3x3 convolution of the invariant field (FP32,
+// one thread per point, the 135 taps and 16 output channels unrolled, the
+// weights read once per work-group from LDS), h0 = tanh(conv) into
+// h0[16][P].  The dense layers then read [lambda(p), h0(p)] of their own
+// point only: k_mlp_f32 (one thread per point, theta in LDS) or
+// k_mlp_bf16 (matrix cores).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-    k_predict_f32(const float* __restrict__ lam, const float* __restrict__ T, int n, const float* __restrict__ theta,
-                  float* __restrict__ tau) {
-  __shared__ float th[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) th[i] = theta[i];
+    k_conv(const float* __restrict__ lam, int n, const float* __restrict__ theta, float* __restrict__ h0) {
+  // the weights tap-major in LDS: wt[k] = W0[0..15][k] as four float4
+  __shared__ float4 wt[kIn0][kConv / 4];
+  __shared__ float bias[kConv];
+  for (int i = threadIdx.x; i < kIn0 * kConv; i += blockDim.x) {
+    const int k = i / kConv, c = i % kConv;
+    reinterpret_cast<float*>(&wt[k][0])[c] = theta[oW0 + c * kIn0 + k];
+  }
+  if (threadIdx.x < kConv) bias[threadIdx.x] = theta[ob0 + threadIdx.x];
   __syncthreads();
+  // one point per thread; the (ch, dy, dz) rows run as a loop (unrolled,
+  // the 2160 weight reads would be scheduled ahead into spilled registers)
   const size_t P = (size_t)n * n * n;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
-    const int y = (int)(p / ((size_t)n * n)), z = (int)((p / n) % n), x = (int)(p % n);
-    float x1[kIn1];
-    float h0[kConv];
+  const int w = n - 1;
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p < P) {
+    const int y = (int)(p / ((size_t)n * n)), z = (int)((p / n) & w), x = (int)(p & w);
+    float h[kConv];
 #pragma unroll
-    for (int c = 0; c < kConv; c++) h0[c] = th[ob0 + c];
-    for (int k = 0; k < kIn0; k++) {
-      const float v = conv_in(lam, P, n, y, z, x, k);
+    for (int c = 0; c < kConv; c++) h[c] = bias[c];
+#pragma unroll 1
+    for (int row = 0; row < kLam * 9; row++) {
+      const int ch = row / 9, dy = (row / 3) % 3 - 1, dz = row % 3 - 1;
+      const float* src = lam + ch * P + (((size_t)((y + dy) & w) * n + ((z + dz) & w)) * n);
+      const float v[3] = {src[(x - 1) & w], src[x], src[(x + 1) & w]};
 #pragma unroll
-      for (int c = 0; c < kConv; c++) h0[c] = fmaf(th[oW0 + c * kIn0 + k], v, h0[c]);
+      for (int dx = 0; dx < 3; dx++) {
+        const int k = row * 3 + dx;  // = ch * 27 + ((dy + 1) * 3 + dz + 1) * 3 + dx
+#pragma unroll
+        for (int q = 0; q < kConv / 4; q++) {
+          const float4 wv = wt[k][q];
+          h[4 * q + 0] = fmaf(wv.x, v[dx], h[4 * q + 0]);
+          h[4 * q + 1] = fmaf(wv.y, v[dx], h[4 * q + 1]);
+          h[4 * q + 2] = fmaf(wv.z, v[dx], h[4 * q + 2]);
+          h[4 * q + 3] = fmaf(wv.w, v[dx], h[4 * q + 3]);
+        }
+      }
     }
+#pragma unroll
+    for (int c = 0; c < kConv; c++) h0[c * P + p] = tanhf(h[c]);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_f32(const float* __restrict__ lam, const float* __restrict__ h0, const float* __restrict__ T, int n,
+              const float* __restrict__ theta, float* __restrict__ tau) {
+  __shared__ float th[kParams - ob0 - kConv];  // W1 onwards
+  for (int i = threadIdx.x; i < kParams - oW1; i += blockDim.x) th[i] = theta[oW1 + i];
+  __syncthreads();
+  const float* W1 = th;
+  const float* b1 = th + (ob1 - oW1);
+  const float* W2 = th + (oW2 - oW1);
+  const float* b2 = th + (ob2 - oW1);
+  const float* W3 = th + (oW3 - oW1);
+  const float* b3 = th + (ob3 - oW1);
+  const float* W4 = th + (oW4 - oW1);
+  const float* b4 = th + (ob4 - oW1);
+  const size_t P = (size_t)n * n * n;
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // one point per thread, as k_conv
+  if (p < P) {
+    float x1[kIn1];
 #pragma unroll
     for (int c = 0; c < kLam; c++) x1[c] = lam[c * P + p];
 #pragma unroll
-    for (int c = 0; c < kConv; c++) x1[kLam + c] = tanhf(h0[c]);
+    for (int c = 0; c < kConv; c++) x1[kLam + c] = h0[c * P + p];
     float ha[kHid], hb[kHid];
 #pragma unroll
     for (int o = 0; o < kHid; o++) {
-      float a = th[ob1 + o];
+      float a = b1[o];
 #pragma unroll
-      for (int i = 0; i < kIn1; i++) a = fmaf(x1[i], th[oW1 + o * kIn1 + i], a);
+      for (int i = 0; i < kIn1; i++) a = fmaf(x1[i], W1[o * kIn1 + i], a);
       ha[o] = tanhf(a);
     }
 #pragma unroll
     for (int o = 0; o < kHid; o++) {
-      float a = th[ob2 + o];
+      float a = b2[o];
 #pragma unroll
-      for (int i = 0; i < kHid; i++) a = fmaf(ha[i], th[oW2 + o * kHid + i], a);
+      for (int i = 0; i < kHid; i++) a = fmaf(ha[i], W2[o * kHid + i], a);
       hb[o] = tanhf(a);
     }
 #pragma unroll
     for (int o = 0; o < kHid; o++) {
-      float a = th[ob3 + o];
+      float a = b3[o];
 #pragma unroll
-      for (int i = 0; i < kHid; i++) a = fmaf(hb[i], th[oW3 + o * kHid + i], a);
+      for (int i = 0; i < kHid; i++) a = fmaf(hb[i], W3[o * kHid + i], a);
       ha[o] = tanhf(a);
     }
     float g[kBasis];
 #pragma unroll
     for (int o = 0; o < kBasis; o++) {
-      float a = th[ob4 + o];
+      float a = b4[o];
 #pragma unroll
-      for (int i = 0; i < kHid; i++) a = fmaf(ha[i], th[oW4 + o * kHid + i], a);
+      for (int i = 0; i < kHid; i++) a = fmaf(ha[i], W4[o * kHid + i], a);
       g[o] = a;
     }
 #pragma unroll
@@ -383,7 +443,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// Inference on the matrix cores: v_mfma_f32_16x16x32_bf16 (bf16 operands,
+// Dense layers on the matrix cores: v_mfma_f32_16x16x32_bf16 (bf16 operands,
 // FP32 accumulation).  A wave takes 16 points per tile; each layer is
 // Y[16][O] = X[16][I] W^T: operand A (lane l) holds X[l % 16][k], k = 32 s +
 // 8 (l / 16) + j (j < 8), operand B holds W[16 nb + l % 16][k] for the same
@@ -424,22 +484,26 @@ __device__ __forceinline__ bf16x8 a_from_tile(const float* H, int s, int lane) {
   return r;
 }
 
+// tanh(x) = 1 - 2 / (exp(2 x) + 1) with the hardware exponential and
+// reciprocal (relative error ~1e-6, far below the bf16 operands' 4e-3;
+// saturates to +-1 for large |x|)
+__device__ __forceinline__ float tanh_fast(float x) { return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f); }
+
 // tanh(C + bias) of N-block nb into the tile (C layout -> rows of H)
 __device__ __forceinline__ void c_to_tile(float* H, f32x4 c, float bias, int nb, int lane, int col0) {
 #pragma unroll
-  for (int j = 0; j < 4; j++) H[(4 * (lane >> 4) + j) * kTileLd + col0 + 16 * nb + (lane & 15)] = tanhf(c[j] + bias);
+  for (int j = 0; j < 4; j++)
+    H[(4 * (lane >> 4) + j) * kTileLd + col0 + 16 * nb + (lane & 15)] = tanh_fast(c[j] + bias);
 }
 
 __global__ void __launch_bounds__(64 * kWaves)
-    k_predict_bf16(const float* __restrict__ lam, const float* __restrict__ T, int n,
-                   const float* __restrict__ theta, float* __restrict__ tau) {
+    k_mlp_bf16(const float* __restrict__ lam, const float* __restrict__ h0, const float* __restrict__ T, int n,
+               const float* __restrict__ theta, float* __restrict__ tau) {
   __shared__ float tile[kWaves][16 * kTileLd];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* H = tile[wave];
   // weights as B operands (bf16), biases of this lane's output columns
-  bf16x8 w0[5], w1[4], w2[2][4], w3[2][4], w4[2];
-#pragma unroll
-  for (int s = 0; s < 5; s++) w0[s] = b_operand(theta + oW0, kConv, kIn0, s, 0, lane);
+  bf16x8 w1[4], w2[2][4], w3[2][4], w4[2];
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
     w1[nb] = b_operand(theta + oW1, kHid, kIn1, 0, nb, lane);
@@ -452,7 +516,6 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
   for (int s = 0; s < 2; s++) w4[s] = b_operand(theta + oW4, kBasis, kHid, s, 0, lane);
   const int col = lane & 15;
-  const float bb0 = theta[ob0 + col];
   float bb1[4], bb2[4], bb3[4];
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
@@ -466,37 +529,20 @@ __global__ void __launch_bounds__(64 * kWaves)
   const size_t ntiles = P / 16;  // n^3 is a multiple of 16 for n >= 4
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   for (size_t t = (size_t)blockIdx.x * kWaves + wave; t < ntiles; t += (size_t)gridDim.x * kWaves) {
-    // this lane's point of operand A and its coordinates
+    // operand A of layer 1: [lambda, h0] of the lane's point, k = 8 (lane / 16) + j
     const size_t p = 16 * t + (lane & 15);
-    const int y = (int)(p / ((size_t)n * n)), z = (int)((p / n) % n), x = (int)(p % n);
-    // conv: 5 K-steps of the gathered neighbourhood
-    f32x4 c0 = zero;
-#pragma unroll
-    for (int s = 0; s < 5; s++) {
+    f32x4 c1[4];
+    {
       bf16x8 a;
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-        const int k = 32 * s + 8 * (lane >> 4) + j;
-        a[j] = (__bf16)(k < kIn0 ? conv_in(lam, P, n, y, z, x, k) : 0.f);
+        const int k = 8 * (lane >> 4) + j;
+        const float v = k < kLam ? lam[k * P + p] : (k < kIn1 ? h0[(k - kLam) * P + p] : 0.f);
+        a[j] = (__bf16)v;
       }
-      c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w0[s], c0, 0, 0, 0);
-    }
-    // layer 1 input tile: columns 0..4 lambda of the row's point, 5..20 h0, 21..31 zero
-    if (lane < 16) {
-#pragma unroll
-      for (int c = 0; c < kLam; c++) H[lane * kTileLd + c] = lam[c * P + p];
-#pragma unroll
-      for (int c = kIn1; c < 32; c++) H[lane * kTileLd + c] = 0.f;
-    }
-    c_to_tile(H, c0, bb0, 0, lane, kLam);
-    wave_sync();
-    f32x4 c1[4];
-    {
-      const bf16x8 a = a_from_tile(H, 0, lane);
 #pragma unroll
       for (int nb = 0; nb < 4; nb++) c1[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w1[nb], zero, 0, 0, 0);
     }
-    wave_sync();
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) c_to_tile(H, c1[nb], bb1[nb], nb, lane, 0);
     wave_sync();
@@ -631,14 +677,17 @@ void launch_box_filter(hipStream_t st, int prec, const void* u, int n, int r, vo
                        tau, nc);
 }
 
-void launch_grad_spec(const Ctx& c, const void* u, void* d) {
+void launch_grad_spec(const Ctx& c, const void* u, void* d, int stage) {
   const size_t tot = (size_t)c.geo.na * c.geo.n;
+  const double2* sx = stage ? c.sx[stage - 1] : nullptr;
+  const double2* sy = stage ? c.sy[stage - 1] : nullptr;
+  const double2* sz = stage ? c.sz[stage - 1] : nullptr;
   if (c.prec == 8)
     hipLaunchKernelGGL(k_grad_spec<double>, dim3(blocks(tot)), dim3(256), 0, c.stream, (const cpx<double>*)u,
-                       (cpx<double>*)d, c.geo);
+                       (cpx<double>*)d, c.geo, sx, sy, sz);
   else
     hipLaunchKernelGGL(k_grad_spec<float>, dim3(blocks(tot)), dim3(256), 0, c.stream, (const cpx<float>*)u,
-                       (cpx<float>*)d, c.geo);
+                       (cpx<float>*)d, c.geo, sx, sy, sz);
 }
 
 void launch_invariants(hipStream_t st, int prec, const void* A, size_t P, double delta2, float* lam, float* T) {
@@ -649,13 +698,28 @@ void launch_invariants(hipStream_t st, int prec, const void* A, size_t P, double
 }
 
 void launch_predict(hipStream_t st, bool bf16, const float* lam, const float* T, int n, const float* theta,
-                    float* tau) {
+                    float* h0, float* tau) {
   const size_t P = (size_t)n * n * n;
-  if (bf16)
-    hipLaunchKernelGGL(k_predict_bf16, dim3(blocks(P / 16, kWaves, 8192)), dim3(64 * kWaves), 0, st, lam, T, n,
-                       theta, tau);
+  const unsigned one_per_thread = (unsigned)((P + 255) / 256);
+  hipLaunchKernelGGL(k_conv, dim3(one_per_thread), dim3(256), 0, st, lam, n, theta, h0);
+  if (bf16) {
+    // persistent: each wave converts the weights to operands once and then
+    // walks many 16-point tiles
+    static const unsigned resident = [] {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_mlp_bf16, 64 * kWaves, 0) != hipSuccess ||
+          per < 1)
+        per = 1;
+      return (unsigned)((cus > 0 ? cus : 256) * per);
+    }();
+    const size_t want = (P / 16 + kWaves - 1) / kWaves;
+    const unsigned grid = (unsigned)(want < resident ? want : resident);
+    hipLaunchKernelGGL(k_mlp_bf16, dim3(grid), dim3(64 * kWaves), 0, st, lam, h0, T, n, theta, tau);
+  }
   else
-    hipLaunchKernelGGL(k_predict_f32, dim3(blocks(P, 256, 8192)), dim3(256), 0, st, lam, T, n, theta, tau);
+    hipLaunchKernelGGL(k_mlp_f32, dim3(one_per_thread), dim3(256), 0, st, lam, h0, T, n, theta, tau);
 }
 
 void launch_add_stress(hipStream_t st, int prec, void* ux, const float* tau, size_t P) {

@@ -18,7 +18,8 @@ coarse grid), run on one GPU through the library's C ABI.
    collection: the DNS continues; an LES with the trained closure (bf16
    matrix-core inference) and an LES without a model start from the filtered
    field.  Spectra are averaged over the last turnover (every 0.1 T_e): the
-   filtered DNS spectrum against each LES for shells 1 .. k_c =
+   filtered DNS spectrum against each LES (both through the fused step
+   entry points, the closure's stress added in the x pass) for shells 1 .. k_c =
    floor(sqrt(2) n_les / 3) (the shells inside the LES mask).  The LES is
    accepted when its largest relative shell error is within 5%; otherwise
    the result says it falls back to the DNS.
@@ -191,8 +192,8 @@ def run(args):
     les_n = P.Solver(nl, 4, viscos=nu)
     les_m.put_state(u0)
     les_n.put_state(u0)
-    les_m_run = P.Les(les_m, nu, math.pi, closure=m, bf16=True, kf_shell=3, ekinf=None)
-    les_n_run = P.Les(les_n, nu, math.pi, closure=None, kf_shell=3, ekinf=les_m_run.ekinf)
+    les_m_run = P.Les(les_m, nu, math.pi, closure=m, bf16=True, kf_shell=3, ekinf=None, fused=True)
+    les_n_run = P.Les(les_n, nu, math.pi, closure=None, kf_shell=3, ekinf=les_m_run.ekinf, fused=True)
     t0 = t
     t_les = {"model": t0, "none": t0}
     wall = {"model": 0.0, "none": 0.0}

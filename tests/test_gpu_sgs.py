@@ -305,3 +305,33 @@ def test_les_with_closure_adds_the_stress_divergence(O, P):
     err = rel(got, uny, g.mask()[None])
     print("LES step with a fixed stress vs oracle:", err, flush=True)
     assert err < 1e-6  # the basis passes through FP32
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_les_fused_matches_per_routine(O, P, prec):
+    """The LES through the fused entry points (features on the stage's
+    shifted grid, the stress added in the x pass) steps like the
+    per-routine LES loop (features after phase1, the stress added to ux)"""
+    print("[gpu case] fused les vs per-routine les", prec, flush=True)
+    n, nu = 32, 0.02
+    g = O.Grid(n)
+    u0 = g.field(seed=9)
+    theta = _theta(13)
+    out = []
+    for fused in (False, True):
+        s = P.Solver(n, prec, viscos=nu)
+        m = P.Closure(n)
+        try:
+            s.put_state(u0)
+            m.set_params(theta)
+            les = P.Les(s, nu, np.pi, closure=m, bf16=False, kf_shell=3, fused=fused)
+            rng = np.random.default_rng(21)
+            for _ in range(3):
+                les.step(rng.random(6))
+            out.append(s.get_state())
+        finally:
+            m.close()
+            s.close()
+    err = rel(out[1], out[0], g.mask()[None])
+    print(f"fused vs per-routine LES after 3 steps: rel L2 {err:.2e}", flush=True)
+    assert err < (1e-10 if prec == 8 else 2e-5)
