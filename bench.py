@@ -52,7 +52,10 @@ def pass_bytes(n, s, num_al, max_al_x):
       z-inverse   3 vars: read cylinder columns, write full lines   3 * n * max_al_x * (nkept + n) * 2s
       x fused     3 spectral in, 5 spectral out (kx < max_al_x)     8 * n^2 * max_al_x * 2s
       z-forward   5 lines in, 4 cylinder columns out                (5 * n * max_al_x * n + 4 * num_al * n) * 2s
-      y-fwd+upd   4 vars gathered + un read, u and un written       (4 + 3 + 3 + 3) * num_al * n * 2s
+      y-fwd+upd   4 vars gathered + un read; stage 1 writes un and
+                  stage 2's yw, stage 2 writes un only (u = un after
+                  proc4b is not stored): per launch on average
+                                                                    (4 + 3 + (6 + 3) / 2) * num_al * n * 2s
     and per passive scalar:
       gradient y  1 cylinder in, 3 transformed derivatives out      4 * num_al * n * 2s
       z-inverse   3 derivatives, as the velocity's z-inverse         3 * (num_al * n + n * max_al_x * n) * 2s
@@ -67,7 +70,7 @@ def pass_bytes(n, s, num_al, max_al_x):
         "z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
         "x_fused": 8 * n * n * max_al_x * c,
         "z_forward": (5 * n * max_al_x * n + 4 * num_al * n) * c,
-        "y_forward_update": 13 * num_al * n * c,
+        "y_forward_update": (13 + 10) * num_al * n * c // 2,
         "sc_grad_y": 4 * num_al * n * c,
         "sc_z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
         "sc_x_pass": 7 * n * n * max_al_x * c,

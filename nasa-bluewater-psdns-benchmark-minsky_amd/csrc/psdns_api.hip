@@ -113,6 +113,11 @@ struct psdns_ctx {
   // (an LES stage; registered by the SGS closure, used once)
   const float* sgs_tau = nullptr;
   int sgs_stage = 0;
+  // u slots 0..2 do not hold the state after an RK2 stage-2 update that
+  // wrote un only (u would equal un on the mask): the next stage-1 inverse
+  // y pass reads un, and an entry point that reads u as the state copies
+  // un into u first (need_u)
+  bool u_stale = false;
   int dev = 0;                      // device of the context's stream
   double gsh[6] = {};               // the shift offsets of the last set_shifts
   // Lagrangian particles: pp(np, 9) column-major (FP64) and whether ux[0..2]
@@ -270,13 +275,25 @@ enum { kTYInv, kTZInv, kTX, kTZFwd, kTYUpd, kTScGrad, kTScZInv, kTScX, kTScZFwd,
 
 // the fused forward y pass + update of a stage (timer 4, events 5 and 6);
 // inv2 also does the inverse y pass of stage 2
-int y_update(psdns_ctx* c, int kstep, double norm, bool inv2) {
+int y_update(psdns_ctx* c, int kstep, double norm, bool inv2, bool unonly = false) {
   int rc;
   c->c.kstep = kstep - 1;
   if ((rc = mark(c, 5))) return rc;
-  if (psdns::launch_y_update(c->c, kstep, norm, inv2) != 0) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  if (psdns::launch_y_update(c->c, kstep, norm, inv2, unonly) != 0)
+    return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   HIPCHK(hipGetLastError());
+  // stage 1 writes u (the predictor, or stage 2's yw), stage 2 u and un
+  // unless unonly
+  c->u_stale = unonly;
   return mark(c, 6);
+}
+
+// u slots 0..2 as the state, for an entry point that reads them so
+int need_u(psdns_ctx* c) {
+  if (!c->u_stale) return 0;
+  HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToDevice, c->c.stream));
+  c->u_stale = false;
+  return 0;
 }
 
 // runs a deferred stage-1 update on its own
@@ -607,6 +624,7 @@ int psdns_put_state(psdns_ctx* c, const void* uny, int layout) {
   const size_t vb = c->csize() * c->cyl_elems();
   HIPCHK(hipMemcpyAsync(c->d_un, uny, vb * (3 + c->nc), hipMemcpyHostToDevice, c->c.stream));
   HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, vb * 3, hipMemcpyDeviceToDevice, c->c.stream));
+  c->u_stale = false;
   if (c->nc > 0)
     HIPCHK(hipMemcpyAsync(c->d_usc, (char*)c->d_un + 3 * vb, vb * c->nc, hipMemcpyDeviceToDevice, c->c.stream));
   c->last_nonlinear = 0;
@@ -640,7 +658,14 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   c->c.kstep = (kstep - 1) % 2;
   if (kstep == 1) HIPCHK(hipMemsetAsync(c->d_velmax, 0, kVelmaxBytes, c->c.stream));
   if ((rc = mark(c, 0))) return rc;
-  if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) return rc;
+  if (!fused && kstep == 1 && c->u_stale) {
+    // the state is in un only (the last stage-2 update wrote un alone)
+    Ctx k = c->c;
+    k.u = c->d_un;
+    if (psdns::launch_pass(k, Pass::YInvPhase, 3, 0)) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  } else if (!fused && (rc = pass(c, Pass::YInvPhase, 3))) {
+    return rc;
+  }
   if ((rc = mark(c, 1))) return rc;
   if ((rc = pass(c, Pass::ZInv, 3, fused ? 1 : 0))) return rc;
   if ((rc = mark(c, 2))) return rc;
@@ -693,7 +718,8 @@ int psdns_stage_update(psdns_ctx* c, int kstep, double dt, const double* xfac, c
     return sync(c);
   }
   if (forcing && (rc = upload_forcing(c, forcing))) return rc;
-  if ((rc = y_update(c, kstep, norm, false))) return rc;
+  // stage 2 without EPFOR increments writes un only (proc4b's u = un)
+  if ((rc = y_update(c, kstep, norm, false, kstep == 2 && !forcing))) return rc;
   // EPFOR after the projection (proc4a.f:588-640 with the tally, proc4b.f:
   // 118-163 before un <- u)
   if (forcing) {
@@ -733,6 +759,7 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
   // (wavespace_vel.f:80) and advanc_kstep in one kernel, from W1
   if (psdns::launch_y_rk4(c->c, kstep, dt / 2.0 / c->n / c->n, c->d_u1) != 0)
     return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  c->u_stale = false;  // the RK4 kernels write u
   // EPFOR (advanc_1.f:165-210 with the tally, advanc_2.f, advanc_3.f): the
   // increments added to u; advanc_4's add is overwritten by u1 on the mask
   // (advanc_4.f:96-170), so substep 4 has none
@@ -882,7 +909,11 @@ int psdns_ek_force(psdns_ctx* c, int kf_shell, int kfor, const double* ekinf, do
     ratio[ik - 1] = std::sqrt(ekinf[ik - 1] / ek[ik - 1]);
   }
   HIPCHK(hipMemcpyAsync(c->d_red + 64, ratio.data(), sizeof(double) * kfor, hipMemcpyHostToDevice, c->c.stream));
-  if (c->nforce) psdns::launch_ek_scale(c->c, kf_shell, c->d_red + 64);
+  if (c->nforce) {
+    Ctx k = c->c;
+    if (c->u_stale) k.u = nullptr;  // u does not hold the state: un only
+    psdns::launch_ek_scale(k, kf_shell, c->d_red + 64);
+  }
   HIPCHK(hipGetLastError());
   if ((rc = sync(c))) return rc;
   if (ek_out) return psdns_ek_shell(c, kfor, ek_out);
@@ -984,11 +1015,14 @@ int psdns_init_field(psdns_ctx* c, unsigned long long seed, double kpeak, double
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, c->csize() * 3 * c->cyl_elems(), hipMemcpyDeviceToDevice, c->c.stream));
   c->last_nonlinear = 0;
+  c->u_stale = false;
   return sync(c);
 }
 
 int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
   if (int rc = ready_clobber(c)) return rc;
+  if (which == PSDNS_FIELD_U)
+    if (int rc = need_u(c)) return rc;
   if (!host) return fail(PSDNS_EINVAL, "null host array");
   size_t bytes;
   void* dst;
@@ -1011,6 +1045,8 @@ int psdns_put_field(psdns_ctx* c, int which, int nv, const void* host) {
 
 int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
   if (int rc = ready(c)) return rc;
+  if (which == PSDNS_FIELD_U)
+    if (int rc = need_u(c)) return rc;
   if (!host) return fail(PSDNS_EINVAL, "null host array");
   size_t bytes;
   const void* src;
@@ -1033,6 +1069,7 @@ int psdns_get_field(psdns_ctx* c, int which, int nv, void* host) {
 
 int psdns_phase1(psdns_ctx* c, int kstep) {
   if (int rc = ready_clobber(c)) return rc;
+  if (int rc = need_u(c)) return rc;
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_phshift(c->c, 3);
@@ -1041,6 +1078,7 @@ int psdns_phase1(psdns_ctx* c, int kstep) {
 
 int psdns_itransform_vel(psdns_ctx* c) {
   if (int rc = ready_clobber(c)) return rc;
+  if (int rc = need_u(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::YInvPlain, 3))) return rc;
@@ -1062,6 +1100,7 @@ int psdns_realspace_vel(psdns_ctx* c, int rkstep, double* velmax) {
 
 int psdns_transform(psdns_ctx* c) {
   if (int rc = ready_clobber(c)) return rc;
+  c->u_stale = false;  // u slots now hold this routine's terms
   int rc;
   if ((rc = ensure_ux(c))) return rc;
   if ((rc = pass(c, Pass::XR2C, 5))) return rc;
@@ -1072,6 +1111,7 @@ int psdns_transform(psdns_ctx* c) {
 
 int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
   if (int rc = ready_clobber(c)) return rc;
+  c->u_stale = false;  // u slots now hold this routine's terms
   if (kstep != 1 && kstep != 2) return fail(PSDNS_EINVAL, "kstep = %d", kstep);
   c->c.kstep = kstep - 1;
   psdns::launch_wavespace(c->c, dt2 / c->n / c->n);
@@ -1080,6 +1120,7 @@ int psdns_wavespace_vel(psdns_ctx* c, int kstep, double dt2) {
 
 int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const double* zfac) {
   if (int rc = ready_clobber(c)) return rc;
+  c->u_stale = false;  // u slots now hold this routine's terms
   int rc;
   if ((rc = upload_factors(c, xfac, yfac, zfac))) return rc;
   psdns::launch_update(c->c, 1);
@@ -1088,12 +1129,14 @@ int psdns_proc4a(psdns_ctx* c, const double* xfac, const double* yfac, const dou
 
 int psdns_proc4b(psdns_ctx* c) {
   if (int rc = ready_clobber(c)) return rc;
+  c->u_stale = false;  // u slots now hold this routine's terms
   psdns::launch_update(c->c, 2);
   return sync(c);
 }
 
 int psdns_kxtran(psdns_ctx* c, int nv) {
   if (int rc = ready_clobber(c)) return rc;
+  if (int rc = need_u(c)) return rc;
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;
@@ -1105,6 +1148,7 @@ int psdns_kxtran(psdns_ctx* c, int nv) {
 
 int psdns_xktran(psdns_ctx* c, int nv) {
   if (int rc = ready_clobber(c)) return rc;
+  if (int rc = need_u(c)) return rc;  // slots nv.. keep the state
   if (nv < 1 || nv > 5) return fail(PSDNS_EINVAL, "nv = %d", nv);
   int rc;
   if ((rc = ensure_ux(c))) return rc;
@@ -1325,6 +1369,7 @@ struct FileCloser {
 int state_to_stage(psdns_ctx* c) {
   const size_t vb = c->csize() * c->cyl_elems();
   HIPCHK(hipMemcpyAsync(c->d_u, c->d_un, vb * 3, hipMemcpyDeviceToDevice, c->c.stream));
+  c->u_stale = false;
   if (c->nc > 0)
     HIPCHK(hipMemcpyAsync(c->d_usc, (char*)c->d_un + 3 * vb, vb * c->nc, hipMemcpyDeviceToDevice, c->c.stream));
   c->last_nonlinear = 0;
@@ -1429,6 +1474,7 @@ namespace psdns {
 
 int ctx_access(psdns_ctx* c, bool clobber, bool need_ux, Ctx* out, int* n) {
   if (int rc = clobber ? ready_clobber(c) : ready(c)) return rc;
+  if (int rc = need_u(c)) return rc;
   if (need_ux)
     if (int rc = ensure_ux(c)) return rc;
   *out = c->c;

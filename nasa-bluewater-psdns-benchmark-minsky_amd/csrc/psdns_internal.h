@@ -78,8 +78,9 @@ enum class Pass {
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1);
 // fused y-forward + wavespace_vel + proc4a (stage 1) / proc4b (stage 2);
-// inv2 (stage 1): also stage 2's phase shift and inverse y pass, yw -> u
-int launch_y_update(const Ctx& c, int stage, double norm, bool inv2);
+// inv2 (stage 1): also stage 2's phase shift and inverse y pass, yw -> u;
+// unonly (stage 2): un written, u not (it would equal un on the mask)
+int launch_y_update(const Ctx& c, int stage, double norm, bool inv2, bool unonly = false);
 void launch_phshift(const Ctx& c, int nv);
 void launch_realspace(const Ctx& c, int want_velmax);
 // fold the velmax slots into slot 0
@@ -92,6 +93,7 @@ int launch_y_rk4(const Ctx& c, int k, double norm, void* u1);
 // FEK shell energies of the force list: per-entry terms (2 doubles each)
 // summed per shell in list order
 void launch_ek_shell(const Ctx& c, int kfor, double* d_ek, double* terms);
+// (c.u null: only un is scaled)
 void launch_ek_scale(const Ctx& c, int kf_shell, const double* d_ratio);
 // FP64 shell statistics in a fixed order: kShellBlocks partial rows of
 // 2 shell_rows(n) doubles in part, then out[0..nshell) = E,

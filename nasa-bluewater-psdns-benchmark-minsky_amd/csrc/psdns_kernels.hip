@@ -1257,7 +1257,9 @@ __global__ void __launch_bounds__(256)
 // Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n] of this
 // stage, sr2 those of stage 2 (INV2).
 // ---------------------------------------------------------------------------
-template <typename R, int N, int STAGE, bool INV2>
+// UNONLY (STAGE 2): un is the only output; u (= un on the mask after
+// proc4b) is not written, and the next step's inverse y pass reads un.
+template <typename R, int N, int STAGE, bool INV2, bool UNONLY = false>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
                    const C<R>* __restrict__ sr, const C<R>* __restrict__ sr2, const double* __restrict__ xfac,
@@ -1473,9 +1475,11 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
       a3[m] = n2;
     } else {
       const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
-      du0.store(ou, n0);
-      du1.store(ou, n1);
-      du2.store(ou, n2);
+      if constexpr (!UNONLY) {
+        du0.store(ou, n0);
+        du1.store(ou, n1);
+        du2.store(ou, n2);
+      }
       if constexpr (LAST) {
         dn0.store(on2, n0);
         dn1.store(on2, n1);
@@ -1558,7 +1562,7 @@ __global__ void k_ek_scale(C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, c
     const R r = (R)ratio[ik - 1];
     for (int v = 0; v < 3; v++) {
       un[v * tot + j] = cscale(un[v * tot + j], r);
-      u[v * tot + j] = cscale(u[v * tot + j], r);
+      if (u) u[v * tot + j] = cscale(u[v * tot + j], r);  // null: u does not hold the state
     }
   }
 }
@@ -2209,7 +2213,7 @@ struct Launch {
   }
   // stage 1 uses sr[0] (and sr[1] for the fused y pass of stage 2), stage 2
   // sr[1]
-  static void y_update(const Ctx& c, int stage, double norm, bool inv2) {
+  static void y_update(const Ctx& c, int stage, double norm, bool inv2, bool unonly) {
     constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
     dim3 grid(c.geo.ngrp);
     // W1 from the fused x pass carries its unnormalized r2c: x 2n
@@ -2224,6 +2228,10 @@ struct Launch {
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
                          c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
+    else if (unonly)
+      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false, true>), grid, dim3(B * T), 0, c.stream,
+                         (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac,
+                         (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
     else
       hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
                          (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
@@ -2358,14 +2366,14 @@ static int run_pass_n(const Ctx& c, Pass p, int a0, int a1) {
 }
 
 template <typename R>
-static int y_update_n(const Ctx& c, int stage, double norm, bool inv2) {
+static int y_update_n(const Ctx& c, int stage, double norm, bool inv2, bool unonly) {
   switch (c.geo.n) {
-    case 32: Launch<R, 32>::y_update(c, stage, norm, inv2); return 0;
-    case 64: Launch<R, 64>::y_update(c, stage, norm, inv2); return 0;
-    case 128: Launch<R, 128>::y_update(c, stage, norm, inv2); return 0;
-    case 256: Launch<R, 256>::y_update(c, stage, norm, inv2); return 0;
-    case 512: Launch<R, 512>::y_update(c, stage, norm, inv2); return 0;
-    case 1024: Launch<R, 1024>::y_update(c, stage, norm, inv2); return 0;
+    case 32: Launch<R, 32>::y_update(c, stage, norm, inv2, unonly); return 0;
+    case 64: Launch<R, 64>::y_update(c, stage, norm, inv2, unonly); return 0;
+    case 128: Launch<R, 128>::y_update(c, stage, norm, inv2, unonly); return 0;
+    case 256: Launch<R, 256>::y_update(c, stage, norm, inv2, unonly); return 0;
+    case 512: Launch<R, 512>::y_update(c, stage, norm, inv2, unonly); return 0;
+    case 1024: Launch<R, 1024>::y_update(c, stage, norm, inv2, unonly); return 0;
   }
   return -1;
 }
@@ -2388,9 +2396,10 @@ int launch_y_rk4(const Ctx& c, int k, double norm, void* u1) {
   return c.prec == 8 ? y_rk4_n<double>(c, k, norm, u1) : y_rk4_n<float>(c, k, norm, u1);
 }
 
-int launch_y_update(const Ctx& c, int stage, double norm, bool inv2) {
-  if (inv2 && stage != 1) return -1;
-  return c.prec == 8 ? y_update_n<double>(c, stage, norm, inv2) : y_update_n<float>(c, stage, norm, inv2);
+int launch_y_update(const Ctx& c, int stage, double norm, bool inv2, bool unonly) {
+  if ((inv2 && stage != 1) || (unonly && stage != 2)) return -1;
+  return c.prec == 8 ? y_update_n<double>(c, stage, norm, inv2, unonly)
+                     : y_update_n<float>(c, stage, norm, inv2, unonly);
 }
 
 int launch_pass(const Ctx& c, Pass p, int a0, int a1) {

@@ -304,3 +304,30 @@ def test_fused_step_reproducible(P, O, n, prec):
         out.append(s.get_state())
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[2])
     s.close()
+
+
+@pytest.mark.parametrize("n,prec", [(32, 8), (256, 8), (256, 4)])
+def test_forced_step_reproducible(P, O, n, prec):
+    """Two forced steps (FEK shells k <= 2) from the same state give the
+    same bits: the shell energies the rescale uses are fixed-order FP64
+    sums and velmax is an order-free maximum, so no reduction depends on
+    scheduling"""
+    print("[gpu case] forced reproducible", n, prec, flush=True)
+    g = O.Grid(n)
+    u0 = g.field(seed=12)
+    s = P.Solver(n, prec=prec, viscos=0.01)
+    try:
+        s.put_state(u0)
+        ekinf = s.ek_shell(4)
+        out = []
+        for _ in range(2):
+            s.put_state(u0)
+            r = P.Run(s, 0.01, np.pi, kf_shell=3, ekinf=ekinf)
+            rng = np.random.default_rng(3)
+            for _ in range(2):
+                r.step(rng.random(6))
+            out.append((s.get_state(), s.shell_stats(n)))
+        assert np.array_equal(out[0][0], out[1][0])
+        assert np.array_equal(out[0][1][0], out[1][1][0]) and out[0][1][3] == out[1][1][3]
+    finally:
+        s.close()
