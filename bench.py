@@ -14,7 +14,8 @@ scalars (Sc = 1, unit mean gradient along x, zero initial fluctuation):
 its step time, per-pass device times per RK substep and the roofline of the
 scalar path's dominant kernel go into the line's "config3_nc2" object
 (--no-config3 skips it).  Step times are also reported as min/mean/max over
-steps 2..K.
+steps 2..K.  The "les_closure" object times the learned-closure LES at 128^3
+against the 512^3 DNS it stands for (--no-les skips it).
 
 The CPU baseline times the FP64 oracle (a port of the reference algorithm)
 on a bounded sample (RK2 steps at 256^3 for about 10-20 s, all host cores)
@@ -166,6 +167,66 @@ def run_case(P, n, prec, nc, rk, steps, warmup, seed=122929298):
     }
 
 
+def les_case(P, steps=10, seed=122929298):
+    """The learned-closure LES against the DNS it replaces: a forced 512^3
+    FP32 DNS step, and a 128^3 FP32 LES step (fused entry points, closure
+    with bf16 matrix-core inference, Glorot-initialised weights: the cost
+    does not depend on their values) from the DNS field box-filtered to
+    128^3.  Returns the step times, the mean dt of each and the ratio of
+    simulated time per wall second."""
+    dns = P.Solver(512, 4, viscos=1.0)
+    les = P.Solver(128, 4, viscos=1.0)
+    m = P.Closure(128)
+    try:
+        dns.init_field(seed=seed, kpeak=3.0, tke=0.5)
+        _, _, _, e1 = dns.shell_stats(0)
+        nu = math.sqrt(e1) * (1.5 / (math.sqrt(2.0) / 3.0 * 512)) ** 2
+        dns.viscos = les.viscos = nu
+        rng = np.random.default_rng(seed)
+        run = P.Run(dns, nu, math.pi, kf_shell=3)
+        for _ in range(2):
+            run.step(rng.random(6))
+        a = time.perf_counter()
+        for _ in range(steps):
+            run.step(rng.random(6))
+        dns_ms = (time.perf_counter() - a) / steps * 1e3
+        dns_dt = float(np.mean(run.dts[-steps:]))
+        th = np.zeros(m.nparam, np.float32)
+        o = 0
+        for shape in ((16, 135), (16,), (64, 21), (64,), (64, 64), (64,), (64, 64), (64,), (4, 64), (4,)):
+            cnt = int(np.prod(shape))
+            if len(shape) == 2:
+                lim = math.sqrt(6.0 / sum(shape)) * (0.1 if shape[0] == 4 else 1.0)
+                th[o:o + cnt] = rng.uniform(-lim, lim, cnt)
+            o += cnt
+        m.set_params(th)
+        dns.kxtran(3)
+        m.filter(dns, les)
+        les.xktran(3)
+        les.put_state(les.get_field(P.FIELD_U, 3))
+        lr = P.Les(les, nu, math.pi, closure=m, bf16=True, kf_shell=3, fused=True)
+        for _ in range(2):
+            lr.step(rng.random(6))
+        a = time.perf_counter()
+        for _ in range(2 * steps):
+            lr.step(rng.random(6))
+        les_ms = (time.perf_counter() - a) / (2 * steps) * 1e3
+        les_dt = float(np.mean(lr.dts[-2 * steps:]))
+    finally:
+        m.close()
+        les.close()
+        dns.close()
+    return {
+        "workload": "forced DNS 512^3 FP32 vs a 128^3 FP32 LES of its box-filtered field with the learned "
+                    "SGS closure (bf16 matrix-core inference), both RK2 on the fused entry points",
+        "dns_ms_per_step": round(dns_ms, 3), "dns_mean_dt": dns_dt,
+        "les_ms_per_step": round(les_ms, 3), "les_mean_dt": les_dt,
+        "speedup_vs_dns": round((les_dt / les_ms) / (dns_dt / dns_ms), 2),
+        "accuracy": "spectral check of a trained closure over 2 eddy turnovers: profiles/les_config5.md "
+                    "(les_experiment.py)",
+    }
+
+
 def fft_summary(r, names):
     nb, per, passes = r["nb"], r["per"], r["passes"]
     timed = [nm for nm in names if nm in passes and passes[nm]["GBps"] is not None]
@@ -237,6 +298,8 @@ def main():
     ap.add_argument("--no-config3", dest="config3", action="store_false",
                     help="skip the two-scalar run that follows a velocity-only RK2 run")
     ap.add_argument("--config3-steps", type=int, default=5)
+    ap.add_argument("--no-les", dest="les", action="store_false",
+                    help="skip the learned-closure LES vs DNS timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -332,6 +395,9 @@ def main():
                                 "traffic": traffic_of(sc_dom, n, r3["s_bytes"], 2),
                                 "bytes_per_launch": r3["nb"][sc_dom], "ms_per_launch": round(sd_ms, 4)},
         }
+    if args.les and args.rk == 2 and n == 1024:
+        print("[bench] les vs dns", file=sys.stderr, flush=True)
+        line["les_closure"] = les_case(P)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n)
