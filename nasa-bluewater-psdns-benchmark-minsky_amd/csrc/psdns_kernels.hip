@@ -1241,6 +1241,27 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Projection of one point of the update (proc4a.f:540-571 / proc4b.f:85-106):
+// n - k (k . n) / |k|^2 for k != 0; k = 0 is left as it is, or zeroed by
+// the last stage (LAST, proc4b.f:176-205); every point off the mask is zero.
+// Branch-free: at k = 0 the divisor is replaced by 1, so psi = 0 and n is
+// unchanged (the same values as skipping the projection there).
+template <typename R, bool LAST>
+__device__ __forceinline__ void project_point(C<R>& n0, C<R>& n1, C<R>& n2, int x, int ky, int kz, bool k0,
+                                              bool on_mask) {
+  const R kxr = (R)x, kyr = (R)ky, kzr = (R)kz;
+  const R den = kxr * kxr + kyr * kyr + kzr * kzr;
+  const R rden = (R)1 / (k0 ? (R)1 : den);
+  const C<R> psi = {-(kxr * n0.x + kyr * n1.x + kzr * n2.x) * rden, -(kxr * n0.y + kyr * n1.y + kzr * n2.y) * rden};
+  n0 = {n0.x + kxr * psi.x, n0.y + kxr * psi.y};
+  n1 = {n1.x + kyr * psi.x, n1.y + kyr * psi.y};
+  n2 = {n2.x + kzr * psi.x, n2.y + kzr * psi.y};
+  const bool zero = !on_mask || (LAST && k0);
+  n0 = zero ? C<R>{0, 0} : n0;
+  n1 = zero ? C<R>{0, 0} : n1;
+  n2 = zero ? C<R>{0, 0} : n2;
+}
+
 // ---------------------------------------------------------------------------
 // Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2)
 // or the RK4 substep advanc_K (STAGE 10 + K):
@@ -1455,20 +1476,8 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   for (int m = 0; m < E; m++) {
     const int y = out.q + T * m, ky = kwave(y, N);
     const bool o = (on >> m) & 1u;
-    const bool k0 = x == 0 && z == 0 && y == 0;
     C<R> n0 = a1[m], n1 = a2[m], n2 = a3[m];
-    if (!k0) {
-      const R kxr = (R)x, kyr = (R)ky, kzr = (R)kz;
-      const R rden = (R)1 / (kxr * kxr + kyr * kyr + kzr * kzr);
-      const C<R> psi = {-(kxr * n0.x + kyr * n1.x + kzr * n2.x) * rden,
-                        -(kxr * n0.y + kyr * n1.y + kzr * n2.y) * rden};
-      n0 = {n0.x + kxr * psi.x, n0.y + kxr * psi.y};
-      n1 = {n1.x + kyr * psi.x, n1.y + kyr * psi.y};
-      n2 = {n2.x + kzr * psi.x, n2.y + kzr * psi.y};
-    } else if constexpr (LAST) {
-      n0 = n1 = n2 = {0, 0};
-    }
-    if (!o) n0 = n1 = n2 = {0, 0};
+    project_point<R, LAST>(n0, n1, n2, x, ky, kz, x == 0 && z == 0 && y == 0, o);
     if constexpr (INV2) {
       a1[m] = n0;
       a2[m] = n1;
