@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = pathlib.Path(__file__).resolve().parent
-_LIB_PATH = _HERE / "_build" / "libpsdns_oracle.so"
+# PSDNS_ORACLE_LIB selects another build of the same sources, e.g. the
+# AddressSanitizer/UBSan one of `make -C oracle sanitize`
+_LIB_PATH = pathlib.Path(os.environ.get("PSDNS_ORACLE_LIB", _HERE / "_build" / "libpsdns_oracle.so"))
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
