@@ -264,7 +264,9 @@ int psdns_stage_times(psdns_ctx *c, int enable, double *times);
  *   3 z-forward   4 y-forward + wavespace + update (RK4: + advanc_k)
  *   5 scalar gradient y pass   6 scalar z-inverse   7 scalar x pass
  *   8 scalar z-forward   9 scalar y-forward   10 scalar updates
- * summed over the scalars and over the calls since the last read (a stage 2
+ * summed over the scalars and over the calls since the last read (the x
+ * pass of a stage serves the velocity and every scalar in one kernel, so
+ * with scalars pass 2 holds their x work too and pass 7 stays 0; a stage 2
  * that follows stage_update(1) directly has its y-inverse inside substep
  * 0's pass 4).  ntimes >= 44.  Reading clears the table; enable turns the
  * timing on or off (psdns_stage_times shares the table: its times[0..4]

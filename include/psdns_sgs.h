@@ -79,7 +79,8 @@ int psdns_sgs_features_stage(psdns_sgs *m, psdns_ctx *c, int kstep);
 /* LES stage through the fused entry points: the predicted stress is added
  * to the products by the next psdns_stage_nonlinear(les, kstep) (in its x
  * pass; used once, dropped by any call that overwrites the work arrays or
- * the shifts).  The closure must stay alive until then. */
+ * the shifts).  The closure must stay alive until then.  A context with
+ * passive scalars is refused (PSDNS_EINVAL). */
 int psdns_sgs_attach_stress(psdns_sgs *m, psdns_ctx *les, int kstep);
 /* host data as the current features (5 + 24 fields) and target (6; nullable) */
 int psdns_sgs_put_fields(psdns_sgs *m, const float *lambda, const float *basis, const float *tau);

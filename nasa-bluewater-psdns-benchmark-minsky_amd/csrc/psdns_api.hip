@@ -306,32 +306,44 @@ int flush(psdns_ctx* c) {
   return add_time(c, 5, 6, kTYUpd, 0);
 }
 
-// The scalars' part of a stage's nonlinear terms, run after the velocity
-// passes, per scalar: the shifted gradient, y-transformed in one pass (X
-// serves as the y-transformed cylinder: W1 holds the velocity's nonlinear
-// terms until the update), its z inverse transforms into xi slots 3..5, the
-// fused x pass with the velocity's z-transformed xi slots 0..2
-// (advection term, r2c / n into X slot 0), then the z and y forward
-// transforms (/ (ny nz), through xi slot 3) into the scalar's term slot.
-int scalar_nonlinear(psdns_ctx* c) {
+// The scalars' part of a stage's nonlinear terms, in two halves around the
+// velocity's fused x pass.  Before it, per scalar i: the shifted gradient,
+// y-transformed in one pass (X serves as the y-transformed cylinder: W1
+// holds the velocity's nonlinear terms until the update), and its z inverse
+// transforms into xi slots 3 + 3 i .. 5 + 3 i.  The x pass then forms every
+// scalar's advection term with the velocity's c2r (r2c / n into X slot
+// 5 + i).  After the velocity's forward z pass, per scalar: the z and y
+// forward transforms (/ (ny nz), through xi slot 3) into its term slot.
+int scalar_gradients(psdns_ctx* c) {
   const Ctx& k = c->c;
   const size_t vb = c->csize() * c->cyl_elems();
   const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;  // one xi / W1 slot
-  void* xg = (char*)c->d_xi + 3 * xb;
   for (int i = 0; i < c->nc; i++) {
     Ctx kd = k;
     kd.w1 = c->d_xs;
-    kd.xi = xg;
-    Ctx kf = k;
-    kf.w1 = xg;
-    kf.u = (char*)c->d_sc + (3 + i) * vb;
+    kd.xi = (char*)c->d_xi + (3 + 3 * (size_t)i) * xb;
     int bad = 0, rc;
     if ((rc = mark_sc(c, 6 * i))) return rc;
     bad |= psdns::launch_y_grad(k, (char*)c->d_usc + i * vb, c->d_xs);
     if ((rc = mark_sc(c, 6 * i + 1))) return rc;
     bad |= psdns::launch_pass(kd, Pass::ZInv, 3, 0);
     if ((rc = mark_sc(c, 6 * i + 2))) return rc;
-    bad |= psdns::launch_x_scalar(k, c->d_xi, xg, c->grad[i]);
+    if (bad) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+int scalar_forward(psdns_ctx* c) {
+  const Ctx& k = c->c;
+  const size_t vb = c->csize() * c->cyl_elems();
+  const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;
+  const size_t xsb = c->csize() * (size_t)c->n * c->n * c->nxh;  // one X slot
+  for (int i = 0; i < c->nc; i++) {
+    Ctx kf = k;
+    kf.xs = (char*)c->d_xs + (5 + (size_t)i) * xsb;
+    kf.w1 = (char*)c->d_xi + 3 * xb;
+    kf.u = (char*)c->d_sc + (3 + i) * vb;
+    int bad = 0, rc;
     if ((rc = mark_sc(c, 6 * i + 3))) return rc;
     bad |= psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0);
     if ((rc = mark_sc(c, 6 * i + 4))) return rc;
@@ -343,11 +355,13 @@ int scalar_nonlinear(psdns_ctx* c) {
   return 0;
 }
 
-// the scalar passes' times of substep st, after a sync
+// the scalar passes' times of substep st, after a sync (the scalars' x pass
+// is inside the velocity's: its time is in kTX)
 int scalar_times(psdns_ctx* c, int st) {
   for (int i = 0; i < c->nc; i++)
     for (int p = 0; p < 5; p++)
-      if (int rc = add_time_sc(c, 6 * i + p, 6 * i + p + 1, kTScGrad + p, st)) return rc;
+      if (p != kTScX - kTScGrad)
+        if (int rc = add_time_sc(c, 6 * i + p, 6 * i + p + 1, kTScGrad + p, st)) return rc;
   return 0;
 }
 
@@ -426,7 +440,7 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   c->nc = p->nc;
   for (int i = 0; i < c->nc; i++) {
     c->pr[i] = p->pr[i];
-    for (int d = 0; d < 3; d++) c->grad[i][d] = p->grad[i][d];
+    for (int d = 0; d < 3; d++) c->grad[i][d] = c->c.scg.g[i][d] = p->grad[i][d];
   }
   // mpisetup.f:311-340
   const int ikmax = (int)(n * std::sqrt(2.0) / 3.0);
@@ -495,10 +509,11 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   }
   const size_t w1_elems = (size_t)5 * c->max_al_x * n * n, xs_elems = (size_t)5 * n * n * c->nxh;
   ALLOC(c->d_w1, c->csize() * w1_elems);
-  // xi: 3 velocity slots (5 for the per-routine path); with scalars 6 (3
-  // more for a scalar's gradient)
-  ALLOC(c->d_xi, c->csize() * (c->nc > 0 ? 6 * w1_elems / 5 : w1_elems));
-  ALLOC(c->d_xs, c->csize() * xs_elems);
+  // xi: 3 velocity slots (5 for the per-routine path); with scalars 3 + 3 nc
+  // (each scalar's gradient); X: 5 slots, with scalars 5 + nc (each
+  // scalar's advection term)
+  ALLOC(c->d_xi, c->csize() * (c->nc > 0 ? (size_t)(3 + 3 * c->nc) * w1_elems / 5 : w1_elems));
+  ALLOC(c->d_xs, c->csize() * (size_t)(5 + c->nc) * xs_elems / 5);
   ALLOC(c->d_tw, c->csize() * n);
   ALLOC(c->d_shift, sizeof(double2) * 2 * (c->nxh + 2 * n));
   ALLOC(c->d_shiftr, c->csize() * 2 * (c->nxh + 2 * n));
@@ -520,8 +535,9 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   CHK(hipMemsetAsync(c->d_u, 0, c->csize() * 5 * cyl, c->c.stream));
   CHK(hipMemsetAsync(c->d_un, 0, c->csize() * 3 * cyl, c->c.stream));
   CHK(hipMemsetAsync(c->d_w1, 0, c->csize() * w1_elems, c->c.stream));
-  CHK(hipMemsetAsync(c->d_xi, 0, c->csize() * w1_elems, c->c.stream));
-  CHK(hipMemsetAsync(c->d_xs, 0, c->csize() * xs_elems, c->c.stream));
+  CHK(hipMemsetAsync(c->d_xi, 0, c->csize() * (c->nc > 0 ? (size_t)(3 + 3 * c->nc) * w1_elems / 5 : w1_elems),
+                     c->c.stream));
+  CHK(hipMemsetAsync(c->d_xs, 0, c->csize() * (size_t)(5 + c->nc) * xs_elems / 5, c->c.stream));
   // twiddles exp(-2 pi i k / n) formed in FP64
   {
     std::vector<double> twd(2 * n);
@@ -669,22 +685,26 @@ int psdns_stage_nonlinear(psdns_ctx* c, int kstep, double* velmax) {
   if ((rc = mark(c, 1))) return rc;
   if ((rc = pass(c, Pass::ZInv, 3, fused ? 1 : 0))) return rc;
   if ((rc = mark(c, 2))) return rc;
+  // the scalars' gradients, z-transformed, for the x pass (event 7: its start)
+  if (c->nc > 0 && ((rc = scalar_gradients(c)) || (rc = mark(c, 7)))) return rc;
   // an LES stage: the stress registered for this stage goes into the products
   c->c.tau = c->sgs_tau && c->sgs_stage == kstep ? c->sgs_tau : nullptr;
   c->sgs_tau = nullptr;
+  c->c.nsc = c->nc;
   rc = pass(c, Pass::XFused, kstep == 1);
   c->c.tau = nullptr;
+  c->c.nsc = 0;
   if (rc) return rc;
   if ((rc = mark(c, 3))) return rc;
   if ((rc = pass(c, Pass::ZFwdConv, 5))) return rc;
   if ((rc = mark(c, 4))) return rc;
-  if (c->nc > 0 && (rc = scalar_nonlinear(c))) return rc;
+  if (c->nc > 0 && (rc = scalar_forward(c))) return rc;
   if (kstep == 1) {
     if ((rc = read_velmax(c, velmax))) return rc;
   }
   if ((rc = sync(c))) return rc;
   for (int i = 0; i < 4; i++)
-    if ((rc = add_time(c, i, i + 1, kTYInv + i, kstep - 1))) return rc;
+    if ((rc = add_time(c, i == 2 && c->nc > 0 ? 7 : i, i + 1, kTYInv + i, kstep - 1))) return rc;
   // the fused kernel is stage 1's update (with stage 2's y inverse)
   if (fused && (rc = add_time(c, 5, 6, kTYUpd, 0))) return rc;
   if (c->nc > 0 && (rc = scalar_times(c, kstep - 1))) return rc;
@@ -1486,6 +1506,8 @@ int ctx_sync(psdns_ctx* c) { return sync(c); }
 
 int ctx_set_stress(psdns_ctx* c, const float* tau, int kstep) {
   if (int rc = check_ctx(c)) return rc;
+  // the x pass carries either the stress or the scalars' advection terms
+  if (c->nc > 0) return set_error(PSDNS_EINVAL, "an LES stage with passive scalars (nc = %d) is not supported", c->nc);
   c->sgs_tau = tau;
   c->sgs_stage = kstep;
   return 0;

@@ -33,6 +33,13 @@ constexpr int kVelmaxStride = 32;
 inline int y_group_lines(int prec) { return prec == 8 ? 4 : 8; }
 
 // Everything a launcher needs; owned by psdns_ctx.
+// mean scalar gradients (G1, G2, G3) of up to kMaxSc scalars, passed to the
+// fused x pass by value
+constexpr int kMaxSc = 8;
+struct ScGrad {
+  double g[kMaxSc][3];
+};
+
 struct Ctx {
   int prec;  // 4 or 8
   Geo geo;
@@ -61,6 +68,11 @@ struct Ctx {
   // physical subgrid stress [6][n][n][n] (11, 12, 13, 22, 23, 33; FP32) that
   // the fused x pass adds to its products (an LES stage), or nullptr
   const float* tau = nullptr;
+  // passive scalars served by the fused x pass (0: the velocity alone): xi
+  // slots 3 + 3 s .. 5 + 3 s hold scalar s's z-transformed shifted gradient,
+  // and its advection term goes to X slot 5 + s
+  int nsc = 0;
+  ScGrad scg = {};
 };
 
 enum class Pass {
@@ -118,9 +130,6 @@ void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
 // update (stage 1 or 2) from the normalised transform tr of the advection
 // term
 int launch_y_grad(const Ctx& c, const void* phi, void* yw);
-// fused x pass of one scalar: c2r of the velocity xv[0..2] and the gradient
-// xg[0..2] (z-transformed), the advection term, r2c x 1/n into c.xs slot 0
-int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]);
 // (dif: nxh + 2 n doubles of scratch for the scalar's integrating factors;
 // stage 1 | 2 RK2, 11 .. 14 the RK4 substeps with the accumulator u1)
 void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,

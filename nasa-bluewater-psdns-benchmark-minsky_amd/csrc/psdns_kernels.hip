@@ -325,50 +325,74 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 // itransform.f:190-203): phi_s = sx sy sz phi on the mask (0 elsewhere);
 // yw slot 0 = i kx Y[phi_s], slot 1 = Y[i ky phi_s], slot 2 = i kz Y[phi_s]
 // with Y the y c2c +i of a column (kx and kz are constant along it, so two
-// transforms give the three derivatives).  One tile of B columns per
-// work-group, natural mapping in and out; yw in the cylinder layout.
+// transforms give the three derivatives).  Tiles of B columns, natural
+// mapping in and out, yw in the cylinder layout; persistent over tiles as
+// k_y_inv (the next tile's loads in flight during the current transforms;
+// off-mask entries are not read, so they enter as zeros).
+// Shift factors in the state precision: sr = sx[nxh] | sy[n] | sz[n].
 // ---------------------------------------------------------------------------
 template <typename R, int N>
 __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
-    k_y_grad(const C<R>* __restrict__ phi, C<R>* __restrict__ yw, Geo g, const double2* __restrict__ sx,
-             const double2* __restrict__ sy, const double2* __restrict__ sz, const C<R>* __restrict__ tw,
-             const int* __restrict__ colx, const int* __restrict__ colz) {
+    k_y_grad(const C<R>* __restrict__ phi, C<R>* __restrict__ yw, Geo g, const C<R>* __restrict__ sr,
+             const C<R>* __restrict__ tw, const int* __restrict__ colx, const int* __restrict__ colz) {
   constexpr int T = Team<N>::T, E = Team<N>::E, B = YInv<R>::B;
   using F = TeamFft<R, N, E, B, +1>;
   __shared__ R lds[F::kLdsReals];
   __shared__ C<R> stw[F::kTwEntries];
+  __shared__ C<R> ssy[N];
   F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  for (int k = threadIdx.x; k < N; k += F::NT) ssy[k] = sr[g.nxh + k];
   const int tid = threadIdx.x;
   const Slot<R> io = F::natural(lds, tid);
-  const int li = F::natural_line(tid), q = io.q;
-  const int a0 = xcd_block(blockIdx.x, gridDim.x) * B;
-  const bool act = a0 + li < g.na;
-  const int a = act ? a0 + li : g.na - 1;
-  const int x = colx[a], z = colz[a], kz = kwave(z, N);
-  const double2 sxz = zmul(sx[x], sz[z]);
-  const BufView<R> src(phi + (size_t)a * N, kOob);
-  C<R> b[E], b2[E];
-#pragma unroll
-  for (int j = 0; j < E; j++) {
-    const int y = q + T * j, ky = kwave(y, N);
-    const bool keep = in_mask(x, ky, kz, y, z, N);
-    const C<R> v = src.load(keep ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob);
-    b[j] = keep ? cmul(v, to_c<R>(zmul(sxz, sy[y]))) : C<R>{0, 0};
-    b2[j] = {-(R)ky * b[j].y, (R)ky * b[j].x};
-  }
-  __syncthreads();
-  F::template run<false, false, false, false>(b, lds, io, io, stw);
-  F::template run<false, false, false, false>(b2, lds, io, io, stw);
+  const int li0 = F::natural_line(tid);
+  const int nab = (g.na + B - 1) / B;  // tiles
   const size_t tot = (size_t)g.na * N;
-  const BufView<R> d0(yw + (size_t)a0 * N, kOob), d1(yw + tot + (size_t)a0 * N, kOob),
-      d2(yw + 2 * tot + (size_t)a0 * N, kOob);
+  __syncthreads();
+  // the line of a lane (wave-uniform when a line spans whole waves)
+  auto line_of = [&]() { return T >= 64 ? __builtin_amdgcn_readfirstlane(opaque(li0)) : opaque(li0); };
+  auto load = [&](Regs<R, E>& b, int t, bool valid) {
+    const int a0 = t * B;
+    const BufView<R> src(phi + (size_t)a0 * N, valid ? kOob : 0u);
+    // lines past the last column re-read the last column (never stored)
+    const int li = line_of(), q = opaque(io.q);
+    const int lc = li < g.na - a0 ? li : g.na - 1 - a0;
+    const int x = colx[a0 + lc], z = colz[a0 + lc], kz = kwave(z, N);
 #pragma unroll
-  for (int m = 0; m < E; m++) {
-    const unsigned o = act ? ((unsigned)li * N + q + T * m) * (unsigned)sizeof(C<R>) : kOob;
-    d0.store(o, C<R>{-(R)x * b[m].y, (R)x * b[m].x});
-    d1.store(o, b2[m]);
-    d2.store(o, C<R>{-(R)kz * b[m].y, (R)kz * b[m].x});
-  }
+    for (int j = 0; j < E; j++) {
+      const int y = q + T * j;
+      const bool keep = in_mask(x, kwave(y, N), kz, y, z, N);
+      b.v[j] = src.load(keep ? ((unsigned)lc * N + y) * (unsigned)sizeof(C<R>) : kOob);
+    }
+  };
+  auto finish = [&](Regs<R, E>& b, int t) {
+    const int a0 = t * B;
+    const int li = line_of(), q = opaque(io.q);
+    const int a = a0 + li < g.na ? a0 + li : g.na - 1;
+    const int x = colx[a], z = colz[a], kz = kwave(z, N);
+    const C<R> sxz = cmul(sr[x], sr[g.nxh + N + z]);
+    C<R> b2[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+      const int y = q + T * j;
+      const R ky = (R)kwave(y, N);
+      b.v[j] = cmul(b.v[j], cmul(ssy[y], sxz));
+      b2[j] = {-ky * b.v[j].y, ky * b.v[j].x};
+    }
+    F::template run<false, false, false, false>(b.v, lds, io, io, stw);
+    F::template run<false, false, false, false>(b2, lds, io, io, stw);
+    const BufView<R> d0(yw + (size_t)a0 * N, kOob), d1(yw + tot + (size_t)a0 * N, kOob),
+        d2(yw + 2 * tot + (size_t)a0 * N, kOob);
+    const bool act = a0 + li < g.na;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const unsigned o = act ? ((unsigned)li * N + q + T * m) * (unsigned)sizeof(C<R>) : kOob;
+      d0.store(o, C<R>{-(R)x * b.v[m].y, (R)x * b.v[m].x});
+      d1.store(o, b2[m]);
+      d2.store(o, C<R>{-(R)kz * b.v[m].y, (R)kz * b.v[m].x});
+    }
+  };
+  Regs<R, E> ba, bb;
+  tile_loop(nab, ba, bb, load, finish);
 }
 
 // ---------------------------------------------------------------------------
@@ -738,10 +762,23 @@ __device__ __forceinline__ R product(int s, R u, R v, R w) {
 // TAU (an LES stage): the subgrid stress tau[6][n][n][n] is added to the
 // products, u_i u_j -> u_i u_j + tau_ij in the flux, i.e. slot 0 + tau11 -
 // tau22, 1 + tau12, 2 + tau33 - tau22, 3 + tau23, 4 + tau13.
-template <typename R, int N, bool TAU>
-__global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
+// SC (nsc passive scalars, itransform.f:522-531 between the x transforms):
+// after u, v, w the pass also transforms each scalar's shifted gradient
+// phi_x, phi_y, phi_z (xi slots 3 + 3 s ..), forms the advection term
+// u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z) in registers and writes
+// its r2c x 1/n into X slot 5 + s, so the velocity's c2r serves every
+// scalar; the gradients go through the same staging buffer in order, the
+// last one followed by the next tile's u.
+#ifndef PSDNS_XSC_WAVES
+#define PSDNS_XSC_WAVES 2
+#endif
+template <typename R, int N, bool TAU, bool SC = false>
+__global__ void __launch_bounds__((XPass<R, N>::NT))
+__attribute__((amdgpu_waves_per_eu(SC ? PSDNS_XSC_WAVES : 4, SC ? PSDNS_XSC_WAVES : 4)))
     k_x_fused(const C<R>* __restrict__ xi, C<R>* __restrict__ xs, Geo g, const C<R>* __restrict__ tw,
-              unsigned long long* __restrict__ velmax, int want_velmax, const float* __restrict__ tau) {
+              unsigned long long* __restrict__ velmax, int want_velmax, const float* __restrict__ tau, int nsc,
+              ScGrad scg) {
+  static_assert(!(TAU && SC), "an LES stage carries no scalars");
   using X = XPass<R, N>;
   constexpr int E = X::E, B = X::B;
   __shared__ R lds[X::kLdsReals];
@@ -769,13 +806,36 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
       X::c2r_staged_pre(pw, S, xtw);
       const int tn = t + w.step;
       const bool more = tn < w.end;
-      if (more) X::stage_load(S, xi, 0, tn / NZB, (tn % NZB) * B, g, true);
+      if (SC) X::stage_load(S, xi, 3, y, z0, g, true);
+      else if (more) X::stage_load(S, xi, 0, tn / NZB, (tn % NZB) * B, g, true);
       X::c2r_staged_fft(pw, lds, stw);
       if (want_velmax) {
 #pragma unroll
         for (int j = 0; j < E; j++) {
           vm = fmax(vm, fabs(pu[j].x) + fabs(pv[j].x) + fabs(pw[j].x));
           vm = fmax(vm, fabs(pu[j].y) + fabs(pv[j].y) + fabs(pw[j].y));
+        }
+      }
+      if constexpr (SC) {
+        for (int s = 0; s < nsc; s++) {
+          C<R> acc[E];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            C<R> pb[E];
+            X::stage_publish();
+            X::c2r_staged_pre(pb, S, xtw);
+            if (d < 2 || s + 1 < nsc) X::stage_load(S, xi, 4 + 3 * s + d, y, z0, g, true);
+            else if (more) X::stage_load(S, xi, 0, tn / NZB, (tn % NZB) * B, g, true);
+            X::c2r_staged_fft(pb, lds, stw);
+            const R gd = (R)scg.g[s][d];
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+              const C<R> a = d == 0 ? pu[j] : (d == 1 ? pv[j] : pw[j]);
+              const C<R> tj = {a.x * (gd + pb[j].x), a.y * (gd + pb[j].y)};
+              acc[j] = d == 0 ? tj : acc[j] + tj;
+            }
+          }
+          X::template r2c<true>(acc, xs, 5 + s, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
         }
       }
 #pragma unroll
@@ -808,67 +868,6 @@ __global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves
     }
   }
   if (want_velmax) velmax_commit((double)vm * (double)N, velmax);
-}
-
-// Fused x pass of one passive scalar (itransform.f:522-531 between the x
-// transforms): c2r of u, v, w (xv, the velocity's z-transformed slots 0..2)
-// and of the shifted gradient phi_x, phi_y, phi_z (xg slots 0..2), the
-// advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z) in
-// registers, r2c x 1/n into X slot 0.  Same tiles and staging as k_x_fused
-// (one tile per work-group): the six variables pass through the LDS staging
-// buffer in the order u, phi_x, v, phi_y, w, phi_z, each copied under the
-// previous one's transform.
-template <typename R, int N>
-__global__ void __launch_bounds__((XPass<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4, 4)))
-    k_x_scalar(const C<R>* __restrict__ xv, const C<R>* __restrict__ xg, C<R>* __restrict__ xs, Geo g,
-               const C<R>* __restrict__ tw, double g0, double g1, double g2) {
-  using X = XPass<R, N>;
-  constexpr int E = X::E, B = X::B;
-  __shared__ R lds[X::kLdsReals];
-  __shared__ C<R> stw[X::kTw], xtw[X::M];
-  __shared__ __attribute__((aligned(16))) C<R> S[X::M * B];
-  X::fill(stw, xtw, tw);
-  constexpr int NZB = N / B;
-  // one tile per work-group (y, B consecutive z)
-  const int t = xcd_block(blockIdx.x, gridDim.x);
-  const int y = t / NZB, z0 = (t % NZB) * B;
-  C<R> pa[E], pb[E], acc[E];
-  // acc += a (gd + b) on both packed points
-  auto add = [&](R gd, bool first) {
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-      const C<R> tj = {pa[j].x * (gd + pb[j].x), pa[j].y * (gd + pb[j].y)};
-      acc[j] = first ? tj : acc[j] + tj;
-    }
-  };
-  X::stage_load(S, xv, 0, y, z0, g, true);
-  X::stage_publish();
-  X::c2r_staged_pre(pa, S, xtw);  // u
-  X::stage_load(S, xg, 0, y, z0, g, true);
-  X::c2r_staged_fft(pa, lds, stw);
-  X::stage_publish();
-  X::c2r_staged_pre(pb, S, xtw);  // phi_x
-  X::stage_load(S, xv, 1, y, z0, g, true);
-  X::c2r_staged_fft(pb, lds, stw);
-  add((R)g0, true);
-  X::stage_publish();
-  X::c2r_staged_pre(pa, S, xtw);  // v
-  X::stage_load(S, xg, 1, y, z0, g, true);
-  X::c2r_staged_fft(pa, lds, stw);
-  X::stage_publish();
-  X::c2r_staged_pre(pb, S, xtw);  // phi_y
-  X::stage_load(S, xv, 2, y, z0, g, true);
-  X::c2r_staged_fft(pb, lds, stw);
-  add((R)g1, false);
-  X::stage_publish();
-  X::c2r_staged_pre(pa, S, xtw);  // w
-  X::stage_load(S, xg, 2, y, z0, g, true);
-  X::c2r_staged_fft(pa, lds, stw);
-  X::stage_publish();
-  X::c2r_staged_pre(pb, S, xtw);  // phi_z
-  X::c2r_staged_fft(pb, lds, stw);
-  add((R)g2, false);
-  X::template r2c<true>(acc, xs, 0, y, z0, g, lds, stw, xtw, (R)1 / (R)N);
 }
 
 // realspace_vel alone on physical ux (slots 0..4)
@@ -1358,10 +1357,6 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     a2[m] = a2[m] + cscale(X.v[m], (R)kz);
     a3[m] = {-ky * X.v[m].y, ky * X.v[m].x};
   }
-  fft(Y);
-#pragma unroll
-  for (int m = 0; m < E; m++) a3[m] = a3[m] + Y.v[m];
-
   // wavespace_vel + proc4a / proc4b in the natural mapping.  nl replaces
   // a1..a3 in place; un is read and used one component at a time (the
   // three components at once do not fit the registers of 16 wavefronts per
@@ -1373,24 +1368,29 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   const BufView<R> dn0(un + col, kOob), dn1(un + tot + col, kOob), dn2(un + 2 * tot + col, kOob);
   const unsigned off0 = (unsigned)(((size_t)ln * N + out.q) * sizeof(C<R>));
   auto ofs = [&](int m) { return off0 + (unsigned)(T * m * sizeof(C<R>)); };
+  Buf U0;
+  fft(Y);
+#pragma unroll
+  for (int m = 0; m < E; m++) a3[m] = a3[m] + Y.v[m];
   unsigned on = 0;  // mask bits of the thread's points
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const int y = out.q + T * m;
     on |= in_mask(x, kwave(y, N), kz, y, z, N) ? 1u << m : 0u;
   }
-  Buf U0;
   {
+    // every point is scaled (no select per point, which the compiler turns
+    // into a branch): points off the mask are never stored or transformed
+    // before the projection zeroes them
     const C<R> sxz = cmul(sr[g.nxh + N + z], sr[x]);
     const R rn = (R)norm;
 #pragma unroll
     for (int m = 0; m < E; m++) {
       const C<R> sp = cmul(sxz, ssy[out.q + T * m]);
       const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
-      const bool o = (on >> m) & 1u;
-      a1[m] = o ? cmul(sxyz, a1[m]) : C<R>{0, 0};
-      a2[m] = o ? cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]) : C<R>{0, 0};
-      a3[m] = o ? cmul(sxyz, a3[m]) : C<R>{0, 0};
+      a1[m] = cmul(sxyz, a1[m]);
+      a2[m] = cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]);
+      a3[m] = cmul(sxyz, a3[m]);
     }
   }
   // one velocity component: STAGE 1 un' = f (un + N) (stored on the mask),
@@ -1799,8 +1799,9 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 // Passive scalars (per scalar, beside the velocity stage):
 //  k_y_grad     (with the y passes) phase1.f:44-64 / itransform.f:190-203:
 //               the shifted gradient, y-transformed
-//  k_x_scalar   (below, with the x passes) itransform.f:522-531: the
+//  k_x_fused<..., SC> (with the x passes) itransform.f:522-531: the
 //               advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
+//               of every scalar, beside the velocity's products
 //  k_sc_update  wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz) T with
 //               T the normalised forward transform of term; proc4a.f:506-520
 //               (STAGE 1: F = xdif ydif zdif, xdif = xfac^(1/pr), step.f:
@@ -2192,12 +2193,17 @@ struct Launch {
       const auto kern = k_x_fused<R, N, true>;
       static const unsigned grid = persistent_grid((const void*)kern, X::NT);
       hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
-                         (const CR*)c.tw, c.velmax_bits, want_velmax, c.tau);
+                         (const CR*)c.tw, c.velmax_bits, want_velmax, c.tau, 0, ScGrad{});
+    } else if (c.nsc > 0) {
+      const auto kern = k_x_fused<R, N, false, true>;
+      static const unsigned grid = persistent_grid((const void*)kern, X::NT);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
+                         (const CR*)c.tw, c.velmax_bits, want_velmax, (const float*)nullptr, c.nsc, c.scg);
     } else {
       const auto kern = k_x_fused<R, N, false>;
       static const unsigned grid = persistent_grid((const void*)kern, X::NT);
       hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const CR*)c.xi, (CR*)c.xs, c.geo,
-                         (const CR*)c.tw, c.velmax_bits, want_velmax, (const float*)nullptr);
+                         (const CR*)c.tw, c.velmax_bits, want_velmax, (const float*)nullptr, 0, ScGrad{});
     }
   }
   static void z_fwd(const Ctx& c, int mode, int nv) {
@@ -2473,10 +2479,10 @@ void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]) {
 template <typename R, int N>
 static void y_grad_n(const Ctx& c, const void* phi, void* yw) {
   constexpr int NT = YInv<R>::B * Team<N>::T;
-  const unsigned grid = (unsigned)((c.geo.na + YInv<R>::B - 1) / YInv<R>::B);
-  const int k = c.kstep;
-  hipLaunchKernelGGL((k_y_grad<R, N>), dim3(grid), dim3(NT), 0, c.stream, (const C<R>*)phi, (C<R>*)yw, c.geo,
-                     c.sx[k], c.sy[k], c.sz[k], (const C<R>*)c.tw, c.geo.colx, c.geo.colz);
+  const auto kern = k_y_grad<R, N>;
+  static const unsigned grid = persistent_grid((const void*)kern, NT);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, c.stream, (const C<R>*)phi, (C<R>*)yw, c.geo,
+                     (const C<R>*)c.sr[c.kstep], (const C<R>*)c.tw, c.geo.colx, c.geo.colz);
 }
 template <typename R>
 static int y_grad_r(const Ctx& c, const void* phi, void* yw) {
@@ -2492,29 +2498,6 @@ static int y_grad_r(const Ctx& c, const void* phi, void* yw) {
 }
 int launch_y_grad(const Ctx& c, const void* phi, void* yw) {
   return c.prec == 8 ? y_grad_r<double>(c, phi, yw) : y_grad_r<float>(c, phi, yw);
-}
-template <typename R, int N>
-static void x_scalar_n(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
-  using X = XPass<R, N>;
-  const auto kern = k_x_scalar<R, N>;
-  const unsigned grid = N * (N / X::B);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(X::NT), 0, c.stream, (const C<R>*)xv, (const C<R>*)xg, (C<R>*)c.xs,
-                     c.geo, (const C<R>*)c.tw, grad[0], grad[1], grad[2]);
-}
-template <typename R>
-static int x_scalar_r(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
-  switch (c.geo.n) {
-    case 32: x_scalar_n<R, 32>(c, xv, xg, grad); return 0;
-    case 64: x_scalar_n<R, 64>(c, xv, xg, grad); return 0;
-    case 128: x_scalar_n<R, 128>(c, xv, xg, grad); return 0;
-    case 256: x_scalar_n<R, 256>(c, xv, xg, grad); return 0;
-    case 512: x_scalar_n<R, 512>(c, xv, xg, grad); return 0;
-    case 1024: x_scalar_n<R, 1024>(c, xv, xg, grad); return 0;
-  }
-  return -1;
-}
-int launch_x_scalar(const Ctx& c, const void* xv, const void* xg, const double grad[3]) {
-  return c.prec == 8 ? x_scalar_r<double>(c, xv, xg, grad) : x_scalar_r<float>(c, xv, xg, grad);
 }
 template <typename R>
 static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif,
