@@ -1261,6 +1261,19 @@ __device__ __forceinline__ void project_point(C<R>& n0, C<R>& n1, C<R>& n2, int 
   n2 = zero ? C<R>{0, 0} : n2;
 }
 
+// Line shape of the y update: in FP32 up to 16 points per thread, so that at
+// n = 1024 a line is one wavefront (no work-group barrier inside a
+// transform) and 8 lines are 8 wavefronts with up to 256 VGPRs each; in FP64
+// at most 8 (the accumulators of 16 points do not fit the registers).
+#ifndef PSDNS_YUPD_E32
+#define PSDNS_YUPD_E32 16
+#endif
+template <typename R, int N>
+struct YUpd {
+  static constexpr int kEmax = sizeof(R) == 4 ? PSDNS_YUPD_E32 : 8;
+  static constexpr int E = Team<N, kEmax>::E, T = Team<N, kEmax>::T;
+};
+
 // ---------------------------------------------------------------------------
 // Fused forward y pass + wavespace_vel + proc4a (STAGE 1) / proc4b (STAGE 2)
 // or the RK4 substep advanc_K (STAGE 10 + K):
@@ -1280,7 +1293,7 @@ __device__ __forceinline__ void project_point(C<R>& n0, C<R>& n1, C<R>& n2, int 
 // UNONLY (STAGE 2): un is the only output; u (= un on the mask after
 // proc4b) is not written, and the next step's inverse y pass reads un.
 template <typename R, int N, int STAGE, bool INV2, bool UNONLY = false>
-__global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
+__global__ void __launch_bounds__((RunLines<R>::B * YUpd<R, N>::T))
     k_y_fwd_update(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double norm,
                    const C<R>* __restrict__ sr, const C<R>* __restrict__ sr2, const double* __restrict__ xfac,
                    const double* __restrict__ yfac, const double* __restrict__ zfac, const C<R>* __restrict__ tw,
@@ -1293,7 +1306,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
   constexpr int K = STAGE - 10;
   constexpr bool IF = STAGE == 1 || STAGE == 11;  // integrating factors
   constexpr bool LAST = STAGE == 2 || STAGE == 14;  // writes un, zero mean
-  constexpr int T = Team<N, 8>::T, E = Team<N, 8>::E, B = RunLines<R>::B;
+  constexpr int T = YUpd<R, N>::T, E = YUpd<R, N>::E, B = RunLines<R>::B;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   using Buf = Regs<R, E>;
   __shared__ R lds[F::kLdsReals];
@@ -1307,217 +1320,239 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N, 8>::T))
     if constexpr (IF) syf[k] = (R)yfac[k];
     if constexpr (INV2) ssy2[k] = sr2[g.nxh + k];
   }
-  const int tid = threadIdx.x;
-  const Slot<R> in = F::transposed(lds, tid);
-  const Slot<R> out = F::natural(lds, tid);
-  const int2 gr = grp[xcd_block(blockIdx.x, gridDim.x)];
-  const int x = colx[gr.x], z0 = colz[gr.x];
-  // gathered loads of one variable (64-byte runs along z); lanes past the
-  // group read nothing
-  const int lt = F::transposed_line(tid);
-  auto gather = [&](Buf& b, int var) {
-    const BufView<R> src(w1 + w1idx(var, x, 0, z0, g), kOob);
+  // Persistent over the column groups (XCD-contiguous ranges, tile_walk):
+  // the next group's first two gathers are issued once this group's un
+  // phase is done, so that they load during its projection, stores and
+  // (INV2) inverse transforms.  Per-thread indices are formed per group
+  // from an opaque thread id (not held in registers across the loop).
+  const TileWalk wk = tile_walk(g.ngrp);
+  if (wk.t >= wk.end) return;
+  // gathered loads of variable var of group gr (64-byte runs along z); lanes
+  // past the group read nothing, and with valid = false no lane reads
+  auto gather = [&](Buf& b, int var, int2 gr, bool valid) {
+    const int tid = opaque((int)threadIdx.x);
+    const int lt = F::transposed_line(tid), q = (int)((unsigned)tid / B);
+    const BufView<R> src(w1 + w1idx(var, colx[gr.x], 0, colz[gr.x], g), valid ? kOob : 0u);
     const bool act = lt < gr.y;
 #pragma unroll
     for (int j = 0; j < E; j++)
-      b.v[j] = src.load(act ? ((unsigned)(in.q + T * j) * N + lt) * (unsigned)sizeof(C<R>) : kOob);
+      b.v[j] = src.load(act ? ((unsigned)(q + T * j) * N + lt) * (unsigned)sizeof(C<R>) : kOob);
   };
-  auto fft = [&](Buf& b) { F::template run<false, true, true, false>(b.v, lds, in, out, stw); };
-  // shift-free sums (wavespace_vel.f:165-170 with sxyz factored out):
-  // A1 = C1 + i ky P2, A2 = kx P2 + kz P4, A3 = C3 + i ky P4;
-  // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3.  Each variable's loads are
-  // issued before the transform of the previous one.
-  // lanes past the group take the group's first column (nothing of theirs
-  // is stored), so that their table reads stay inside sz and zfac
-  const int ln = F::natural_line(tid);
-  const int z = z0 + (ln < gr.y ? ln : 0), kz = kwave(z, N);
-  C<R> a1[E], a2[E], a3[E];
   Buf X, Y;
-  gather(X, 1);  // P2
-  gather(Y, 0);  // C1
-  issue_fence();
-  fft(X);
-#pragma unroll
-  for (int m = 0; m < E; m++) {
-    const R ky = (R)kwave(out.q + T * m, N);
-    a1[m] = {-ky * X.v[m].y, ky * X.v[m].x};
-    a2[m] = cscale(X.v[m], (R)x);
-  }
-  gather(X, 3);  // P4
-  issue_fence();
-  fft(Y);
-#pragma unroll
-  for (int m = 0; m < E; m++) a1[m] = a1[m] + Y.v[m];
-  gather(Y, 2);  // C3
-  issue_fence();
-  fft(X);
-#pragma unroll
-  for (int m = 0; m < E; m++) {
-    const R ky = (R)kwave(out.q + T * m, N);
-    a2[m] = a2[m] + cscale(X.v[m], (R)kz);
-    a3[m] = {-ky * X.v[m].y, ky * X.v[m].x};
-  }
-  // wavespace_vel + proc4a / proc4b in the natural mapping.  nl replaces
-  // a1..a3 in place; un is read and used one component at a time (the
-  // three components at once do not fit the registers of 16 wavefronts per
-  // CU).  Off the mask u is zero and un is not written.
-  const bool act = ln < gr.y;
-  const size_t tot = (size_t)g.na * N;
-  const size_t col = (size_t)gr.x * N;
-  const BufView<R> du0(u + col, kOob), du1(u + tot + col, kOob), du2(u + 2 * tot + col, kOob);
-  const BufView<R> dn0(un + col, kOob), dn1(un + tot + col, kOob), dn2(un + 2 * tot + col, kOob);
-  const unsigned off0 = (unsigned)(((size_t)ln * N + out.q) * sizeof(C<R>));
-  auto ofs = [&](int m) { return off0 + (unsigned)(T * m * sizeof(C<R>)); };
-  Buf U0;
-  fft(Y);
-#pragma unroll
-  for (int m = 0; m < E; m++) a3[m] = a3[m] + Y.v[m];
-  unsigned on = 0;  // mask bits of the thread's points
-#pragma unroll
-  for (int m = 0; m < E; m++) {
-    const int y = out.q + T * m;
-    on |= in_mask(x, kwave(y, N), kz, y, z, N) ? 1u << m : 0u;
-  }
-  {
-    // every point is scaled (no select per point, which the compiler turns
-    // into a branch): points off the mask are never stored or transformed
-    // before the projection zeroes them
-    const C<R> sxz = cmul(sr[g.nxh + N + z], sr[x]);
-    const R rn = (R)norm;
+  int t = wk.t;
+  gather(X, 1, grp[t], true);  // P2
+  gather(Y, 0, grp[t], true);  // C1
+  for (;;) {
+    const int2 gr = grp[t];
+    const int t2 = t + wk.step;
+    const bool more = t2 < wk.end;
+    const int tid = opaque((int)threadIdx.x);
+    const Slot<R> in = F::transposed(lds, tid);
+    const Slot<R> out = F::natural(lds, tid);
+    const int x = colx[gr.x], z0 = colz[gr.x];
+    auto fft = [&](Buf& b) { F::template run<false, true, true, false>(b.v, lds, in, out, stw); };
+    // shift-free sums (wavespace_vel.f:165-170 with sxyz factored out):
+    // A1 = C1 + i ky P2, A2 = kx P2 + kz P4, A3 = C3 + i ky P4;
+    // N1 = sxyz A1, N2 = sxyz i A2, N3 = sxyz A3.  Each variable's loads are
+    // issued before the transform of the previous one.
+    // lanes past the group take the group's first column (nothing of theirs
+    // is stored), so that their table reads stay inside sz and zfac
+    const int ln = F::natural_line(tid);
+    const int z = z0 + (ln < gr.y ? ln : 0), kz = kwave(z, N);
+    C<R> a1[E], a2[E], a3[E];
+    issue_fence();
+    fft(X);
 #pragma unroll
     for (int m = 0; m < E; m++) {
-      const C<R> sp = cmul(sxz, ssy[out.q + T * m]);
-      const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
-      a1[m] = cmul(sxyz, a1[m]);
-      a2[m] = cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]);
-      a3[m] = cmul(sxyz, a3[m]);
+      const R ky = (R)kwave(out.q + T * m, N);
+      a1[m] = {-ky * X.v[m].y, ky * X.v[m].x};
+      a2[m] = cscale(X.v[m], (R)x);
     }
-  }
-  // one velocity component: STAGE 1 un' = f (un + N) (stored on the mask),
-  // nl <- f (un + 2 N) (proc4a.f:418-487); STAGE 2 nl <- un + N (proc4b.f:54-67)
-  // integrating factor xfac yfac zfac as (xfac zfac) yfac in the state
-  // precision
-  R fxz = 0;
-  if constexpr (IF) fxz = (R)(xfac[x] * zfac[z]);
-  auto component = [&](C<R>* nl, const Buf& b, const BufView<R>& dn) {
+    gather(X, 3, gr, true);  // P4
+    issue_fence();
+    fft(Y);
+#pragma unroll
+    for (int m = 0; m < E; m++) a1[m] = a1[m] + Y.v[m];
+    gather(Y, 2, gr, true);  // C3
+    issue_fence();
+    fft(X);
 #pragma unroll
     for (int m = 0; m < E; m++) {
-      if constexpr (STAGE == 1) {
-        const R f = fxz * syf[out.q + T * m];
-        const C<R> ut = b.v[m];
-        const C<R> un2 = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
-        nl[m] = {f * (ut.x + nl[m].x * (R)2), f * (ut.y + nl[m].y * (R)2)};
-        dn.store(act && ((on >> m) & 1u) ? ofs(m) : kOob, un2);
-      } else {
-        nl[m] = b.v[m] + nl[m];
-      }
+      const R ky = (R)kwave(out.q + T * m, N);
+      a2[m] = a2[m] + cscale(X.v[m], (R)kz);
+      a3[m] = {-ky * X.v[m].y, ky * X.v[m].x};
     }
-  };
-  if constexpr (!RK4) {
+    // wavespace_vel + proc4a / proc4b in the natural mapping.  nl replaces
+    // a1..a3 in place; un is read and used one component at a time (the
+    // three components at once do not fit the registers of 16 wavefronts per
+    // CU).  Off the mask u is zero and un is not written.
+    const bool act = ln < gr.y;
+    const size_t tot = (size_t)g.na * N;
+    const size_t col = (size_t)gr.x * N;
+    const BufView<R> du0(u + col, kOob), du1(u + tot + col, kOob), du2(u + 2 * tot + col, kOob);
+    const BufView<R> dn0(un + col, kOob), dn1(un + tot + col, kOob), dn2(un + 2 * tot + col, kOob);
+    const unsigned off0 = (unsigned)(((size_t)ln * N + out.q) * sizeof(C<R>));
+    auto ofs = [&](int m) { return off0 + (unsigned)(T * m * sizeof(C<R>)); };
+    Buf U0;
+    fft(Y);
 #pragma unroll
-    for (int m = 0; m < E; m++) U0.v[m] = dn0.load(act ? ofs(m) : kOob);
-    issue_fence();
-    component(a1, U0, dn0);
-    issue_fence();
+    for (int m = 0; m < E; m++) a3[m] = a3[m] + Y.v[m];
+    unsigned on = 0;  // mask bits of the thread's points
 #pragma unroll
-    for (int m = 0; m < E; m++) U0.v[m] = dn1.load(act ? ofs(m) : kOob);
-    issue_fence();
-    component(a2, U0, dn1);
-    issue_fence();
-#pragma unroll
-    for (int m = 0; m < E; m++) U0.v[m] = dn2.load(act ? ofs(m) : kOob);
-    issue_fence();
-    component(a3, U0, dn2);
-    issue_fence();
-  } else {
-    // advanc_K per component (advanc_1.f:88-99, advanc_2.f, advanc_3.f,
-    // advanc_4.f:46-52): K 1..3 store u1 (on the mask) and leave the
-    // pre-projection u in nl; K 4 leaves u1 + N/3 in nl for the projection
-    // (the accumulator is not needed again: substep 1 sets it anew)
-    const BufView<R> d10(u1 + col, kOob), d11(u1 + tot + col, kOob), d12(u1 + 2 * tot + col, kOob);
-    auto rk4 = [&](C<R>* nl, const BufView<R>& dn, const BufView<R>& d1) {
-      Buf A;
-      if constexpr (K != 4) {
-#pragma unroll
-        for (int m = 0; m < E; m++) U0.v[m] = dn.load(act ? ofs(m) : kOob);
-      }
-      if constexpr (K != 1) {
-#pragma unroll
-        for (int m = 0; m < E; m++) A.v[m] = d1.load(act ? ofs(m) : kOob);
-      }
-      issue_fence();
+    for (int m = 0; m < E; m++) {
+      const int y = out.q + T * m;
+      on |= in_mask(x, kwave(y, N), kz, y, z, N) ? 1u << m : 0u;
+    }
+    {
+      // every point is scaled (no select per point, which the compiler turns
+      // into a branch): points off the mask are never stored or transformed
+      // before the projection zeroes them
+      const C<R> sxz = cmul(sr[g.nxh + N + z], sr[x]);
+      const R rn = (R)norm;
 #pragma unroll
       for (int m = 0; m < E; m++) {
-        const unsigned so = act && ((on >> m) & 1u) ? ofs(m) : kOob;
-        if constexpr (K == 1) {
+        const C<R> sp = cmul(sxz, ssy[out.q + T * m]);
+        const C<R> sxyz = {-rn * sp.x, rn * sp.y};  // -norm conj(sxz sy)
+        a1[m] = cmul(sxyz, a1[m]);
+        a2[m] = cmul(C<R>{-sxyz.y, sxyz.x}, a2[m]);
+        a3[m] = cmul(sxyz, a3[m]);
+      }
+    }
+    // one velocity component: STAGE 1 un' = f (un + N) (stored on the mask),
+    // nl <- f (un + 2 N) (proc4a.f:418-487); STAGE 2 nl <- un + N (proc4b.f:54-67)
+    // integrating factor xfac yfac zfac as (xfac zfac) yfac in the state
+    // precision
+    R fxz = 0;
+    if constexpr (IF) fxz = (R)(xfac[x] * zfac[z]);
+    auto component = [&](C<R>* nl, const Buf& b, const BufView<R>& dn) {
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        if constexpr (STAGE == 1) {
           const R f = fxz * syf[out.q + T * m];
-          const C<R> ut = U0.v[m];
-          d1.store(so, C<R>{f * (ut.x + nl[m].x / (R)3), f * (ut.y + nl[m].y / (R)3)});
-          nl[m] = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
-        } else if constexpr (K == 2 || K == 3) {
-          d1.store(so, C<R>{A.v[m].x + nl[m].x * (R)2 / (R)3, A.v[m].y + nl[m].y * (R)2 / (R)3});
-          const R w = K == 2 ? (R)1 : (R)2;
-          nl[m] = {U0.v[m].x + nl[m].x * w, U0.v[m].y + nl[m].y * w};
+          const C<R> ut = b.v[m];
+          const C<R> un2 = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
+          nl[m] = {f * (ut.x + nl[m].x * (R)2), f * (ut.y + nl[m].y * (R)2)};
+          dn.store(act && ((on >> m) & 1u) ? ofs(m) : kOob, un2);
         } else {
-          nl[m] = {A.v[m].x + nl[m].x / (R)3, A.v[m].y + nl[m].y / (R)3};
+          nl[m] = b.v[m] + nl[m];
         }
       }
     };
-    rk4(a1, dn0, d10);
-    issue_fence();
-    rk4(a2, dn1, d11);
-    issue_fence();
-    rk4(a3, dn2, d12);
-    issue_fence();
-  }
-  // projection (proc4a.f:540-571 / proc4b.f:85-106, k = 0 excluded; zeroed
-  // in STAGE 2, proc4b.f:176-205); u off the mask is zero
+    if constexpr (!RK4) {
 #pragma unroll
-  for (int m = 0; m < E; m++) {
-    const int y = out.q + T * m, ky = kwave(y, N);
-    const bool o = (on >> m) & 1u;
-    C<R> n0 = a1[m], n1 = a2[m], n2 = a3[m];
-    project_point<R, LAST>(n0, n1, n2, x, ky, kz, x == 0 && z == 0 && y == 0, o);
-    if constexpr (INV2) {
-      a1[m] = n0;
-      a2[m] = n1;
-      a3[m] = n2;
+      for (int m = 0; m < E; m++) U0.v[m] = dn0.load(act ? ofs(m) : kOob);
+      issue_fence();
+      component(a1, U0, dn0);
+      issue_fence();
+#pragma unroll
+      for (int m = 0; m < E; m++) U0.v[m] = dn1.load(act ? ofs(m) : kOob);
+      issue_fence();
+      component(a2, U0, dn1);
+      issue_fence();
+#pragma unroll
+      for (int m = 0; m < E; m++) U0.v[m] = dn2.load(act ? ofs(m) : kOob);
+      issue_fence();
+      component(a3, U0, dn2);
+      issue_fence();
     } else {
-      const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
-      if constexpr (!UNONLY) {
-        du0.store(ou, n0);
-        du1.store(ou, n1);
-        du2.store(ou, n2);
-      }
-      if constexpr (LAST) {
-        dn0.store(on2, n0);
-        dn1.store(on2, n1);
-        dn2.store(on2, n2);
+      // advanc_K per component (advanc_1.f:88-99, advanc_2.f, advanc_3.f,
+      // advanc_4.f:46-52): K 1..3 store u1 (on the mask) and leave the
+      // pre-projection u in nl; K 4 leaves u1 + N/3 in nl for the projection
+      // (the accumulator is not needed again: substep 1 sets it anew)
+      const BufView<R> d10(u1 + col, kOob), d11(u1 + tot + col, kOob), d12(u1 + 2 * tot + col, kOob);
+      auto rk4 = [&](C<R>* nl, const BufView<R>& dn, const BufView<R>& d1) {
+        Buf A;
+        if constexpr (K != 4) {
+#pragma unroll
+          for (int m = 0; m < E; m++) U0.v[m] = dn.load(act ? ofs(m) : kOob);
+        }
+        if constexpr (K != 1) {
+#pragma unroll
+          for (int m = 0; m < E; m++) A.v[m] = d1.load(act ? ofs(m) : kOob);
+        }
+        issue_fence();
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const unsigned so = act && ((on >> m) & 1u) ? ofs(m) : kOob;
+          if constexpr (K == 1) {
+            const R f = fxz * syf[out.q + T * m];
+            const C<R> ut = U0.v[m];
+            d1.store(so, C<R>{f * (ut.x + nl[m].x / (R)3), f * (ut.y + nl[m].y / (R)3)});
+            nl[m] = {f * (ut.x + nl[m].x), f * (ut.y + nl[m].y)};
+          } else if constexpr (K == 2 || K == 3) {
+            d1.store(so, C<R>{A.v[m].x + nl[m].x * (R)2 / (R)3, A.v[m].y + nl[m].y * (R)2 / (R)3});
+            const R w = K == 2 ? (R)1 : (R)2;
+            nl[m] = {U0.v[m].x + nl[m].x * w, U0.v[m].y + nl[m].y * w};
+          } else {
+            nl[m] = {A.v[m].x + nl[m].x / (R)3, A.v[m].y + nl[m].y / (R)3};
+          }
+        }
+      };
+      rk4(a1, dn0, d10);
+      issue_fence();
+      rk4(a2, dn1, d11);
+      issue_fence();
+      rk4(a3, dn2, d12);
+      issue_fence();
+    }
+    // the next group's P2 and C1 (X and Y are free until its first transform)
+    {
+      const int2 gn = grp[more ? t2 : t];
+      gather(X, 1, gn, more);
+      gather(Y, 0, gn, more);
+    }
+    issue_fence();
+    // projection (proc4a.f:540-571 / proc4b.f:85-106, k = 0 excluded; zeroed
+    // in STAGE 2, proc4b.f:176-205); u off the mask is zero
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int y = out.q + T * m, ky = kwave(y, N);
+      const bool o = (on >> m) & 1u;
+      C<R> n0 = a1[m], n1 = a2[m], n2 = a3[m];
+      project_point<R, LAST>(n0, n1, n2, x, ky, kz, x == 0 && z == 0 && y == 0, o);
+      if constexpr (INV2) {
+        a1[m] = n0;
+        a2[m] = n1;
+        a3[m] = n2;
+      } else {
+        const unsigned ou = act ? ofs(m) : kOob, on2 = act && o ? ofs(m) : kOob;
+        if constexpr (!UNONLY) {
+          du0.store(ou, n0);
+          du1.store(ou, n1);
+          du2.store(ou, n2);
+        }
+        if constexpr (LAST) {
+          dn0.store(on2, n0);
+          dn1.store(on2, n1);
+          dn2.store(on2, n2);
+        }
       }
     }
-  }
-  if constexpr (INV2) {
-    // u* is zero off the mask, so the shift needs no mask of its own; the
-    // transforms run in the natural mapping in and out, on the lines the
-    // forward transforms left (each line read and written by its own threads)
-    using FI = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
-    static_assert(T % B == 0, "y blocks of a store");
-    const C<R> sxz2 = cmul(sr2[x], sr2[g.nxh + N + z]);
-    // y-blocked yw in the u slots (blocks of B y, as the inverse z pass reads)
-    const unsigned lo = (((unsigned)out.q / B * g.na + ln) * B + (unsigned)out.q % B) * (unsigned)sizeof(C<R>);
-    auto inverse = [&](C<R>* a, int v) {
+    if constexpr (INV2) {
+      // u* is zero off the mask, so the shift needs no mask of its own; the
+      // transforms run in the natural mapping in and out, on the lines the
+      // forward transforms left (each line read and written by its own threads)
+      using FI = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
+      static_assert(T % B == 0, "y blocks of a store");
+      const C<R> sxz2 = cmul(sr2[x], sr2[g.nxh + N + z]);
+      // y-blocked yw in the u slots (blocks of B y, as the inverse z pass reads)
+      const unsigned lo = (((unsigned)out.q / B * g.na + ln) * B + (unsigned)out.q % B) * (unsigned)sizeof(C<R>);
+      auto inverse = [&](C<R>* a, int v) {
 #pragma unroll
-      for (int m = 0; m < E; m++) a[m] = cmul(a[m], cmul(ssy2[out.q + T * m], sxz2));
-      FI::template run<false, true, false, false>(a, lds, out, out, stw);
+        for (int m = 0; m < E; m++) a[m] = cmul(a[m], cmul(ssy2[out.q + T * m], sxz2));
+        FI::template run<false, true, false, false>(a, lds, out, out, stw);
 #pragma unroll
-      for (int m = 0; m < E; m++) {
-        const BufView<R> d(u + ywidx<B>(v, gr.x, T * m, g), kOob);
-        d.store(act ? lo : kOob, a[m]);
-      }
-    };
-    inverse(a1, 0);
-    inverse(a2, 1);
-    inverse(a3, 2);
+        for (int m = 0; m < E; m++) {
+          const BufView<R> d(u + ywidx<B>(v, gr.x, T * m, g), kOob);
+          d.store(act ? lo : kOob, a[m]);
+        }
+      };
+      inverse(a1, 0);
+      inverse(a2, 1);
+      inverse(a3, 2);
+    }
+    if (!more) break;
+    t = t2;
   }
 }
 
@@ -2117,15 +2152,26 @@ static inline unsigned grid1d(size_t work, int per) {
   return (unsigned)(b ? b : 1);
 }
 
-// Work-groups of a persistent kernel: as many as are resident at once on
-// the device (occupancy x CUs), rounded down to a multiple of the 8 XCDs.
-static unsigned persistent_grid(const void* kernel, int threads) {
+// Work-groups of a persistent kernel: scale x as many as are resident at
+// once on the device (occupancy x CUs), rounded down to a multiple of the 8
+// XCDs.  A scale above 1 makes a larger grid than fits at once (work-groups
+// take fewer tiles each).
 #ifdef PSDNS_RESIDENT_SCALE
-  // tuning: a larger grid than fits at once (work-groups take fewer tiles)
-  constexpr int kScale = PSDNS_RESIDENT_SCALE;
+constexpr int kResidentScale = PSDNS_RESIDENT_SCALE;
 #else
-  constexpr int kScale = 16;
+constexpr int kResidentScale = 16;
 #endif
+// the y update's scale: one work-group fits per CU; 16x makes work-groups of
+// about a dozen column groups each that replace one another as they finish
+// (1x, each work-group walking a fixed 1/256 of the groups, measured slower:
+// the slowest work-group ends the kernel)
+#ifdef PSDNS_YSCALE
+constexpr int kYScale = PSDNS_YSCALE;
+#else
+constexpr int kYScale = 16;
+#endif
+static unsigned persistent_grid(const void* kernel, int threads, int scale = kResidentScale) {
+  const int kScale = scale;
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -2229,42 +2275,36 @@ struct Launch {
   // stage 1 uses sr[0] (and sr[1] for the fused y pass of stage 2), stage 2
   // sr[1]
   static void y_update(const Ctx& c, int stage, double norm, bool inv2, bool unonly) {
-    constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
-    dim3 grid(c.geo.ngrp);
+    constexpr int NT = RunLines<R>::B * YUpd<R, N>::T;
     // W1 from the fused x pass carries its unnormalized r2c: x 2n
     norm /= 2.0 * N;
     const CR* sr = (const CR*)c.sr[stage - 1];
     const CR* sr2 = (const CR*)c.sr[1];
-    if (stage == 1 && inv2)
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, true>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
-                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
-    else if (stage == 1)
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 1, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
-                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
-    else if (unonly)
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false, true>), grid, dim3(B * T), 0, c.stream,
-                         (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac,
-                         (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
-    else
-      hipLaunchKernelGGL((k_y_fwd_update<R, N, 2, false>), grid, dim3(B * T), 0, c.stream, (const CR*)c.w1,
-                         (CR*)c.u, (CR*)c.un, c.geo, norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw,
-                         c.geo.grp, c.geo.colx, c.geo.colz, (CR*)nullptr);
+    const int v = stage == 1 ? (inv2 ? 0 : 1) : (unonly ? 2 : 3);
+    const auto kern = v == 0   ? k_y_fwd_update<R, N, 1, true>
+                      : v == 1 ? k_y_fwd_update<R, N, 1, false>
+                      : v == 2 ? k_y_fwd_update<R, N, 2, false, true>
+                               : k_y_fwd_update<R, N, 2, false>;
+    static unsigned grids[4] = {0, 0, 0, 0};
+    if (!grids[v]) grids[v] = persistent_grid((const void*)kern, NT, kYScale);
+    hipLaunchKernelGGL(kern, dim3(grids[v]), dim3(NT), 0, c.stream, (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo,
+                       norm, sr, sr2, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz,
+                       (CR*)nullptr);
   }
   // RK4 substep k: y forward pass + wavespace_vel + advanc_k, shifts of
   // stage (k - 1) % 2 + 1
   static void y_rk4(const Ctx& c, int k, double norm, void* u1) {
-    constexpr int B = RunLines<R>::B, T = Team<N, 8>::T;
-    dim3 grid(c.geo.ngrp);
+    constexpr int NT = RunLines<R>::B * YUpd<R, N>::T;
     norm /= 2.0 * N;  // W1 from the fused x pass: x 2n
     const CR* sr = (const CR*)c.sr[(k - 1) % 2];
     const auto kern = k == 1   ? k_y_fwd_update<R, N, 11, false>
                       : k == 2 ? k_y_fwd_update<R, N, 12, false>
                       : k == 3 ? k_y_fwd_update<R, N, 13, false>
                                : k_y_fwd_update<R, N, 14, false>;
-    hipLaunchKernelGGL(kern, grid, dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo, norm,
-                       sr, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz,
+    static unsigned grids[4] = {0, 0, 0, 0};
+    if (!grids[k - 1]) grids[k - 1] = persistent_grid((const void*)kern, NT, kYScale);
+    hipLaunchKernelGGL(kern, dim3(grids[k - 1]), dim3(NT), 0, c.stream, (const CR*)c.w1, (CR*)c.u, (CR*)c.un, c.geo,
+                       norm, sr, sr, c.xfac, c.yfac, c.zfac, (const CR*)c.tw, c.geo.grp, c.geo.colx, c.geo.colz,
                        (CR*)u1);
   }
 };
