@@ -893,16 +893,26 @@ __global__ void __launch_bounds__(256)
 // mapping; the last stage leaves z lines in the natural mapping for
 // contiguous stores of the cylinder z (the cut is skipped).
 // ---------------------------------------------------------------------------
+// points per thread of the forward z pass in FP32 (FP64: at most 8).  With
+// 16 a line of n = 1024 is one wavefront and the work-group 8 wavefronts of
+// up to 256 VGPRs: the pass itself is 0.15 ms faster per launch at 1024^3,
+// but the y update that reads its output is 0.5 ms slower, so 8.
 #ifndef PSDNS_ZFWD_E
 #define PSDNS_ZFWD_E 8
 #endif
+
 #ifndef PSDNS_ZFRESH
 #define PSDNS_ZFRESH false
 #endif
 constexpr bool ZFRESH = PSDNS_ZFRESH;
 template <typename R, int N>
 struct ZFwd {
-  static constexpr int T = Team<N, PSDNS_ZFWD_E>::T, E = Team<N, PSDNS_ZFWD_E>::E, B = RunLines<R>::B;
+  static constexpr int kEmax = sizeof(R) == 4 ? PSDNS_ZFWD_E : 8;
+  static constexpr int T = Team<N, kEmax>::T, E = Team<N, kEmax>::E, B = RunLines<R>::B;
+  // waves per SIMD the registers of k_z_fwd are sized for: 2 when a thread
+  // holds 16 points of its three buffers (4 spills 188 bytes per lane);
+  // k_z_fwd_norm (two buffers) fits 4
+  static constexpr int kWaves = E >= 16 ? 2 : 4;
   static constexpr int NT = B * T;
   using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
   using Buf = Regs<R, E>;
@@ -942,7 +952,7 @@ struct ZFwd {
 // with three register buffers in fixed roles: A holds P5 (loaded during the
 // previous tile), B takes P1 then P2, C takes P3 then P4.
 template <typename R, int N>
-__global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((ZFwd<R, N>::NT)) __attribute__((amdgpu_waves_per_eu(ZFwd<R, N>::kWaves)))
     k_z_fwd(const C<R>* __restrict__ xs, C<R>* __restrict__ w1, Geo g, const C<R>* __restrict__ tw,
             const int* __restrict__ cutz) {
   using Z = ZFwd<R, N>;
