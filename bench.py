@@ -63,8 +63,9 @@ def pass_bytes(n, s, num_al, max_al_x, nc=0):
       z-inverse   3 derivatives, as the velocity's z-inverse         3 * (num_al * n + n * max_al_x * n) * 2s
       x pass      inside the velocity's x fused pass                0
       z-forward   1 line set in, 1 cylinder out                      (n * max_al_x * n + num_al * n) * 2s
-      y-forward   1 cylinder in, 1 out                               2 * num_al * n * 2s
-      update      term and un in, u and un out                       4 * num_al * n * 2s
+      y-forward   inside the update                                 0
+      update      the term gathered (y forward transform) and un
+                  in, u and un out                                  4 * num_al * n * 2s
     (nkept * max_al_x = num_al)."""
     c = 2 * s
     return {
@@ -77,7 +78,7 @@ def pass_bytes(n, s, num_al, max_al_x, nc=0):
         "sc_z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
         "sc_x_pass": 0,
         "sc_z_forward": (n * max_al_x * n + num_al * n) * c,
-        "sc_y_forward": 2 * num_al * n * c,
+        "sc_y_forward": 0,
         "sc_update": 4 * num_al * n * c,
     }
 
@@ -90,13 +91,14 @@ def launches_per_step(nc, rk):
     """launches of each pass per step: RK2 has two stages (the stage-2 y
     inverse of a velocity-only run is done by the stage-1 update kernel),
     RK4 four substeps; every scalar pass runs once per scalar and stage,
-    except the x pass, which the velocity's x fused pass does for all"""
+    except the x pass, which the velocity's x fused pass does for all, and
+    the y forward pass, which is part of the scalar's update"""
     st = 2 if rk == 2 else 4
     per = {nm: st for nm in VEL}
     if rk == 2 and nc == 0:
         per["y_inverse"] = 1
     for nm in SC:
-        per[nm] = st * nc if nm != "sc_x_pass" else 0
+        per[nm] = st * nc if nm not in ("sc_x_pass", "sc_y_forward") else 0
     return per
 
 

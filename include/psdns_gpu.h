@@ -266,7 +266,9 @@ int psdns_stage_times(psdns_ctx *c, int enable, double *times);
  *   8 scalar z-forward   9 scalar y-forward   10 scalar updates
  * summed over the scalars and over the calls since the last read (the x
  * pass of a stage serves the velocity and every scalar in one kernel, so
- * with scalars pass 2 holds their x work too and pass 7 stays 0; a stage 2
+ * with scalars pass 2 holds their x work too and pass 7 stays 0; each
+ * scalar's y forward pass is part of its update, so pass 9 stays 0 and
+ * pass 10 holds both; a stage 2
  * that follows stage_update(1) directly has its y-inverse inside substep
  * 0's pass 4).  ntimes >= 44.  Reading clears the table; enable turns the
  * timing on or off (psdns_stage_times shares the table: its times[0..4]

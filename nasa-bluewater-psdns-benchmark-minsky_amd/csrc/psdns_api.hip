@@ -84,11 +84,12 @@ struct psdns_ctx {
   Ctx c;
   int n = 0, nxh = 0, na = 0, max_al_x = 0, prec = 8;
   double viscos = 0.0;
-  // passive scalars: state in un slots 3.., stage values in d_usc, and
-  // d_sc = 3 derivative slots + one transformed advection term per scalar
+  // passive scalars: state in un slots 3.., stage values in d_usc; their
+  // advection terms wait in xi slots 3.. between stage_nonlinear and the
+  // update
   int nc = 0;
   double pr[8] = {}, grad[8][3] = {};
-  void *d_usc = nullptr, *d_sc = nullptr;
+  void *d_usc = nullptr;
   // RK4 accumulator u1(ny, num_al, 3), allocated on the first RK4 substep
   void* d_u1 = nullptr;
   std::vector<int> colx, colz, cutz;
@@ -312,8 +313,9 @@ int flush(psdns_ctx* c) {
 // holds the velocity's nonlinear terms until the update), and its z inverse
 // transforms into xi slots 3 + 3 i .. 5 + 3 i.  The x pass then forms every
 // scalar's advection term with the velocity's c2r (r2c / n into X slot
-// 5 + i).  After the velocity's forward z pass, per scalar: the z and y
-// forward transforms (/ (ny nz), through xi slot 3) into its term slot.
+// 5 + i).  After the velocity's forward z pass, per scalar: the z forward
+// transform (/ (ny nz)) into xi slot 3 + i, where the scalar's update reads
+// it (its y forward transform is part of the update pass).
 int scalar_gradients(psdns_ctx* c) {
   const Ctx& k = c->c;
   const size_t vb = c->csize() * c->cyl_elems();
@@ -333,22 +335,23 @@ int scalar_gradients(psdns_ctx* c) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+// scalar i's z-transformed advection term (one W1-layout slot in xi): kept
+// from stage_nonlinear until the update reads it
+void* scalar_term(psdns_ctx* c, int i) {
+  const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;  // one xi / W1 slot
+  return (char*)c->d_xi + (3 + (size_t)i) * xb;
+}
 int scalar_forward(psdns_ctx* c) {
   const Ctx& k = c->c;
-  const size_t vb = c->csize() * c->cyl_elems();
-  const size_t xb = c->csize() * (size_t)c->max_al_x * c->n * c->n;
   const size_t xsb = c->csize() * (size_t)c->n * c->n * c->nxh;  // one X slot
   for (int i = 0; i < c->nc; i++) {
     Ctx kf = k;
     kf.xs = (char*)c->d_xs + (5 + (size_t)i) * xsb;
-    kf.w1 = (char*)c->d_xi + 3 * xb;
-    kf.u = (char*)c->d_sc + (3 + i) * vb;
+    kf.w1 = scalar_term(c, i);
     int bad = 0, rc;
     if ((rc = mark_sc(c, 6 * i + 3))) return rc;
     bad |= psdns::launch_pass(kf, Pass::ZFwdNorm, 1, 0);
     if ((rc = mark_sc(c, 6 * i + 4))) return rc;
-    bad |= psdns::launch_pass(kf, Pass::YFwd, 1, 0);
-    if ((rc = mark_sc(c, 6 * i + 5))) return rc;
     if (bad) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   }
   HIPCHK(hipGetLastError());
@@ -356,24 +359,29 @@ int scalar_forward(psdns_ctx* c) {
 }
 
 // the scalar passes' times of substep st, after a sync (the scalars' x pass
-// is inside the velocity's: its time is in kTX)
+// is inside the velocity's, its time in kTX; their y forward pass is inside
+// their update, in kTScUpd)
 int scalar_times(psdns_ctx* c, int st) {
   for (int i = 0; i < c->nc; i++)
-    for (int p = 0; p < 5; p++)
+    for (int p = 0; p < 4; p++)
       if (p != kTScX - kTScGrad)
         if (int rc = add_time_sc(c, 6 * i + p, 6 * i + p + 1, kTScGrad + p, st)) return rc;
   return 0;
 }
 
-// the scalars' update of stage kstep (after the velocity's)
-int scalar_update(psdns_ctx* c, int kstep, double dt2) {
+// the scalars' update of stage kstep (after the velocity's): per scalar its
+// y forward pass and the update in one kernel (stage 1 | 2, or 10 + K for
+// RK4 substep K with the accumulator u1)
+int scalar_update(psdns_ctx* c, int kstep, double dt2, void* u1 = nullptr) {
   const size_t vb = c->csize() * c->cyl_elems();
   const int e0 = 6 * psdns_ctx::kMaxScalars;
-  int rc;
+  int rc, bad = 0;
   if ((rc = mark_sc(c, e0))) return rc;
   for (int i = 0; i < c->nc; i++)
-    psdns::launch_sc_update(c->c, kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
-                            (const char*)c->d_sc + (3 + i) * vb, dt2, 1.0 / c->pr[i], c->d_red);
+    bad |= psdns::launch_sc_update(c->c, kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
+                                   scalar_term(c, i), dt2, 1.0 / c->pr[i], c->d_red,
+                                   u1 ? (char*)u1 + (3 + i) * vb : nullptr);
+  if (bad) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   HIPCHK(hipGetLastError());
   return mark_sc(c, e0 + 1);
 }
@@ -503,7 +511,6 @@ int psdns_create(const psdns_params* p, psdns_ctx** out) {
   ALLOC(c->d_un, c->csize() * (3 + c->nc) * cyl);
   if (c->nc > 0) {
     ALLOC(c->d_usc, c->csize() * c->nc * cyl);
-    ALLOC(c->d_sc, c->csize() * (3 + c->nc) * cyl);
     CHK(hipMemsetAsync(c->d_usc, 0, c->csize() * c->nc * cyl, c->c.stream));
     CHK(hipMemsetAsync((char*)c->d_un + c->csize() * 3 * cyl, 0, c->csize() * c->nc * cyl, c->c.stream));
   }
@@ -594,7 +601,7 @@ int psdns_destroy(psdns_ctx* c) {
   if (c->c.stream) (void)hipStreamSynchronize(c->c.stream);
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
-                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_sc, c->d_u1, c->d_epf, c->d_epf_for,
+                  c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_u1, c->d_epf, c->d_epf_for,
                   c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part, c->d_pp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -789,15 +796,9 @@ int psdns_rk4_update(psdns_ctx* c, int kstep, double dt, const double* xfac, con
     HIPCHK(hipMemsetAsync(c->d_efkz, 0, sizeof(double) * 3 * c->nxh, c->c.stream));
   }
   // the scalars' advanc_kstep (the same loops over i = 4 .. 3 + nc), from
-  // their transformed advection terms of stage_nonlinear
+  // their advection terms of stage_nonlinear
   if ((rc = mark(c, 6))) return rc;
-  if ((rc = mark_sc(c, 6 * psdns_ctx::kMaxScalars))) return rc;
-  for (int i = 0; i < c->nc; i++)
-    psdns::launch_sc_update(c->c, 10 + kstep, (char*)c->d_usc + i * vb, (char*)c->d_un + (3 + i) * vb,
-                            (const char*)c->d_sc + (3 + i) * vb, dt / 2.0, 1.0 / c->pr[i], c->d_red,
-                            (char*)c->d_u1 + (3 + i) * vb);
-  HIPCHK(hipGetLastError());
-  if ((rc = mark_sc(c, 6 * psdns_ctx::kMaxScalars + 1))) return rc;
+  if ((rc = scalar_update(c, 10 + kstep, dt / 2.0, c->d_u1))) return rc;
   if ((rc = sync(c))) return rc;
   if (c->nc > 0 && (rc = add_time_sc(c, 6 * psdns_ctx::kMaxScalars, 6 * psdns_ctx::kMaxScalars + 1, kTScUpd,
                                      kstep - 1)))

@@ -127,13 +127,13 @@ void launch_scale(const Ctx& c, void* p, size_t cnt, double s);
 void launch_shifts(const Ctx& c, double2* s64, void* sr, const double gsh[6]);
 // passive scalars, stage c.kstep: the shifted gradient i k (sx sy sz phi)
 // of one scalar, y-transformed (y c2c +i), into yw slots 0..2; the scalar
-// update (stage 1 or 2) from the normalised transform tr of the advection
-// term
+// update from tr, the advection term z-transformed and normalised (one W1
+// slot): its y forward transform and the update in one pass (stage 1 | 2
+// RK2, 11 .. 14 the RK4 substeps with the accumulator u1; dif: nxh + 2 n
+// doubles of scratch for the scalar's integrating factors)
 int launch_y_grad(const Ctx& c, const void* phi, void* yw);
-// (dif: nxh + 2 n doubles of scratch for the scalar's integrating factors;
-// stage 1 | 2 RK2, 11 .. 14 the RK4 substeps with the accumulator u1)
-void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
-                      double* dif, void* u1 = nullptr);
+int launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
+                     double* dif, void* u1 = nullptr);
 // velocity-gradient moments: i kx u, i kz w of un into cylinder slots d[0..1];
 // power sums 1..8 of ux slots 0 and 1 added into d_out[16]
 void launch_vg_deriv(const Ctx& c, void* d);

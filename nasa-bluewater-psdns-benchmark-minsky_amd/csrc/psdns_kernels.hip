@@ -1695,8 +1695,9 @@ __global__ void __launch_bounds__(256)
 // fixed tree, and only the run's last lane adds it to the wave's own LDS
 // histogram (one writer per shell per instruction, in program order).  The
 // work-group then adds its wave histograms in wave order into its partial
-// row part[b][0..nh) (E) and part[b][nh..2 nh) (D); k_shell_fold sums the
-// rows in block order.  No atomics: the result depends on B only.
+// row part[b][0..nh) (E) and part[b][nh..2 nh) (D); k_shell_fold_rows and
+// k_shell_fold sum the rows in two fixed levels.  No atomics: the result
+// depends on B only.
 // ---------------------------------------------------------------------------
 constexpr int kShellWaves = 8;      // waves per work-group of k_shell_partial
 constexpr int kShellChunks = 8;     // 64-entry chunks of a column half per pass
@@ -1736,8 +1737,13 @@ __global__ void __launch_bounds__(64 * kShellWaves)
           if (y0 + 64 * c >= yhi) break;  // wave-uniform
           const int y = y0 + 64 * c + lane, ky = kwave(y, n);
           const bool on = y < yhi && in_mask(x, ky, kz, y, z, n);
-          const double rk2 = (double)x * x + (double)ky * ky + (double)kz * kz;
-          int key = on ? (int)(sqrt(rk2) + 1.5) : -1;
+          const int k2 = x * x + ky * ky + kz * kz;
+          const double rk2 = (double)k2;
+          // ik = int(|k| + 1.5) (ek_shell.f:66): |k|^2 is an integer below
+          // 2^24, and sqrt(k2) + 1.5 is never within 1e-4 of an integer for
+          // k2 <= 2 n^2 / 9 (n <= 1024), far more than FP32 rounding, so the
+          // FP32 square root gives the same index as FP64
+          int key = on ? (int)(__fsqrt_rn((float)k2) + 1.5f) : -1;
           double e = 0.0;
           if (on) {
             double sum = 0.0;
@@ -1778,17 +1784,32 @@ __global__ void __launch_bounds__(64 * kShellWaves)
   }
 }
 
+// The partial rows are folded in two levels, each in row order: work-group
+// g of k_shell_fold_rows sums rows g kShellFoldRows .. (g + 1)
+// kShellFoldRows - 1 into the first of them (each thread reads its own
+// column of those rows before it writes it), then k_shell_fold sums those
+// nb / kShellFoldRows rows.
+constexpr int kShellFoldRows = 32;
+__global__ void __launch_bounds__(256) k_shell_fold_rows(double* __restrict__ part, int nh) {
+  double* r0 = part + (size_t)blockIdx.x * kShellFoldRows * 2 * nh;
+  for (int s = threadIdx.x; s < 2 * nh; s += blockDim.x) {
+    double v = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < kShellFoldRows; b++) v += r0[(size_t)b * 2 * nh + s];
+    r0[s] = v;
+  }
+}
 // out[0..nshell) = E, out[nshell..2 nshell) = D, out[2 nshell] = TKE,
-// out[2 nshell + 1] = epsilon: the partial rows summed in block order, the
-// totals over all nh shells in shell order.  One work-group.
+// out[2 nshell + 1] = epsilon: the rows b rstride (b < nb) summed in row
+// order, the totals over all nh shells in shell order.  One work-group.
 __global__ void __launch_bounds__(256)
-    k_shell_fold(const double* __restrict__ part, int nb, int nh, int nshell, double* __restrict__ out) {
+    k_shell_fold(const double* __restrict__ part, int nb, int rstride, int nh, int nshell, double* __restrict__ out) {
   extern __shared__ double fold[];  // [2][nh]
   for (int s = threadIdx.x; s < nh; s += blockDim.x) {
     double e = 0.0, d = 0.0;
     for (int b = 0; b < nb; b++) {
-      e += part[(size_t)b * 2 * nh + s];
-      d += part[(size_t)b * 2 * nh + nh + s];
+      e += part[(size_t)b * rstride * 2 * nh + s];
+      d += part[(size_t)b * rstride * 2 * nh + nh + s];
     }
     fold[s] = e;
     fold[nh + s] = d;
@@ -1861,48 +1882,74 @@ __global__ void __launch_bounds__(256) k_sc_dif(const double* __restrict__ fac, 
   if (e < len) dif[e] = pow(fac[e], rpr);
 }
 
-template <typename R, int STAGE>
-__global__ void __launch_bounds__(256)
-    k_sc_update(C<R>* __restrict__ u, C<R>* __restrict__ un, const C<R>* __restrict__ tr, Geo g, double dt2,
-                const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
-                const double* __restrict__ xdif, const double* __restrict__ ydif, const double* __restrict__ zdif,
-                C<R>* __restrict__ u1) {
-  const size_t tot = (size_t)g.na * g.n;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const Pt p = point(i, g);
-    if (!p.m) continue;
-    const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
-    const C<R> sn = {(R)(-dt2 * s.x), (R)(dt2 * s.y)};  // -(dt/2) conj(s)
-    const C<R> nl = cmul(sn, tr[i]);
+// One scalar's forward y pass (xkcomm2.f:229) fused with its update: a
+// group of columns of the z-transformed term (W1 layout, one slot) is
+// gathered and y-transformed as k_y_fwd does, and each point on the mask is
+// updated from the transformed term T at once (the term is never stored):
+// N = -(dt/2) conj(sx sy sz) T, then STAGE 1 / 2 of RK2 or substep 10 + K
+// of RK4 (the accumulator u1).  One group per work-group.
+template <typename R, int N, int STAGE>
+__global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T))
+    k_y_fwd_sc(const C<R>* __restrict__ w1, C<R>* __restrict__ u, C<R>* __restrict__ un, Geo g, double dt2,
+               const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
+               const double* __restrict__ xdif, const double* __restrict__ ydif, const double* __restrict__ zdif,
+               const C<R>* __restrict__ tw, C<R>* __restrict__ u1) {
+  constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
+  using F = TeamFft<R, N, E, B, -1, LineShift<E, B>::IN>;
+  __shared__ R lds[F::kLdsReals];
+  __shared__ C<R> stw[F::kTwEntries];
+  F::template fill_twiddles<1>(stw, tw, threadIdx.x, F::NT);
+  const int tid = threadIdx.x;
+  const int2 grp = g.grp[xcd_block(blockIdx.x, gridDim.x)];
+  const Slot<R> in = F::transposed(lds, tid);
+  C<R> v[E];
+  y_gather<R, N, 16>(v, w1, 0, g, grp, F::transposed_line(tid), in.q);
+  const Slot<R> out = F::natural(lds, tid);
+  F::template run<false, true, true, false>(v, lds, in, out, stw);
+  const int l = F::natural_line(tid);
+  if (l >= grp.y) return;  // after the transform's last barrier
+  const int a = grp.x + l, x = g.colx[a], z = g.colz[a], kz = kwave(z, N);
+  const double2 sxz = zmul(sx[x], sz[z]);
+  const size_t col = (size_t)a * N;
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int y = out.q + T * j;
+    if (!in_mask(x, kwave(y, N), kz, y, z, N)) continue;
+    const size_t i = col + y;
+    const double2 sp = zmul(sxz, sy[y]);
+    const C<R> sn = {(R)(-dt2 * sp.x), (R)(dt2 * sp.y)};  // -(dt/2) conj(s)
+    const C<R> nl = cmul(sn, v[j]);
     if constexpr (STAGE == 1) {
-      const R f = (R)(xdif[p.x] * ydif[p.y] * zdif[p.z]);
+      // proc4a.f:506-520: F = xdif ydif zdif; un <- F (un + N), u <- F (un + 2 N)
+      const R f = (R)(xdif[x] * ydif[y] * zdif[z]);
       const C<R> ut = un[i];
       un[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
       u[i] = {f * (ut.x + nl.x * (R)2), f * (ut.y + nl.y * (R)2)};
     } else if constexpr (STAGE == 2) {
-      const bool k0 = p.a == 0 && p.y == 0 && p.x == 0 && p.z == 0;
-      const C<R> v = k0 ? C<R>{0, 0} : un[i] + nl;
-      u[i] = v;
-      un[i] = v;
+      // proc4b.f:69-74,183-190,201-205: u <- un + N, un <- u, zero mean
+      const bool k0 = a == 0 && y == 0 && x == 0 && z == 0;
+      const C<R> w = k0 ? C<R>{0, 0} : un[i] + nl;
+      u[i] = w;
+      un[i] = w;
     } else if constexpr (STAGE == 11) {
       // advanc_1.f:101-111: factor xdif ydif zdif; u1 = F (un + N/3), u = F (un + N)
-      const R f = (R)(xdif[p.x] * ydif[p.y] * zdif[p.z]);
+      const R f = (R)(xdif[x] * ydif[y] * zdif[z]);
       const C<R> ut = un[i];
       u1[i] = {f * (ut.x + nl.x / (R)3), f * (ut.y + nl.y / (R)3)};
       u[i] = {f * (ut.x + nl.x), f * (ut.y + nl.y)};
     } else if constexpr (STAGE == 12 || STAGE == 13) {
       // advanc_2.f / advanc_3.f: u1 += N 2/3; u = un + N (resp. + 2 N)
-      const C<R> a = u1[i], b = un[i];
-      u1[i] = {a.x + nl.x * (R)2 / (R)3, a.y + nl.y * (R)2 / (R)3};
+      const C<R> p1 = u1[i], b = un[i];
+      u1[i] = {p1.x + nl.x * (R)2 / (R)3, p1.y + nl.y * (R)2 / (R)3};
       const R w = STAGE == 12 ? (R)1 : (R)2;
       u[i] = {b.x + nl.x * w, b.y + nl.y * w};
     } else {
       // advanc_4.f:51,149-176: u1 += N/3; un = u = u1; zero mean
-      const bool k0 = p.a == 0 && p.y == 0 && p.x == 0 && p.z == 0;
-      const C<R> a = u1[i];
-      const C<R> v = k0 ? C<R>{0, 0} : C<R>{a.x + nl.x / (R)3, a.y + nl.y / (R)3};
-      u[i] = v;
-      un[i] = v;
+      const bool k0 = a == 0 && y == 0 && x == 0 && z == 0;
+      const C<R> p1 = u1[i];
+      const C<R> w = k0 ? C<R>{0, 0} : C<R>{p1.x + nl.x / (R)3, p1.y + nl.y / (R)3};
+      u[i] = w;
+      un[i] = w;
     }
   }
 }
@@ -2371,8 +2418,10 @@ struct LaunchP {
     const int nh = shell_rows(c.geo.n);
     hipLaunchKernelGGL((k_shell_partial<R>), dim3(kShellBlocks), dim3(64 * kShellWaves),
                        sizeof(double) * kShellWaves * 2 * nh, c.stream, (const CR*)c.un, c.geo, viscos, nh, part);
-    hipLaunchKernelGGL(k_shell_fold, dim3(1), dim3(256), sizeof(double) * 2 * nh, c.stream, part, kShellBlocks, nh,
-                       nshell, out);
+    static_assert(kShellBlocks % kShellFoldRows == 0, "whole fold chunks");
+    hipLaunchKernelGGL(k_shell_fold_rows, dim3(kShellBlocks / kShellFoldRows), dim3(256), 0, c.stream, part, nh);
+    hipLaunchKernelGGL(k_shell_fold, dim3(1), dim3(256), sizeof(double) * 2 * nh, c.stream, part,
+                       kShellBlocks / kShellFoldRows, kShellFoldRows, nh, nshell, out);
   }
   static void field(const Ctx& c, unsigned long long seed, double kpeak, const int* d_colz0) {
     size_t tot = (size_t)c.geo.na * c.geo.n;
@@ -2549,31 +2598,43 @@ static int y_grad_r(const Ctx& c, const void* phi, void* yw) {
 int launch_y_grad(const Ctx& c, const void* phi, void* yw) {
   return c.prec == 8 ? y_grad_r<double>(c, phi, yw) : y_grad_r<float>(c, phi, yw);
 }
-template <typename R>
-static void sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif,
+template <typename R, int N>
+static void sc_update_n(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif,
                         void* u1) {
-  const size_t tot = (size_t)c.geo.na * c.geo.n;
+  constexpr int NT = RunLines<R>::B * Team<N>::T;
   const int k = c.kstep;
-  const auto kern = stage == 1    ? k_sc_update<R, 1>
-                    : stage == 2  ? k_sc_update<R, 2>
-                    : stage == 11 ? k_sc_update<R, 11>
-                    : stage == 12 ? k_sc_update<R, 12>
-                    : stage == 13 ? k_sc_update<R, 13>
-                                  : k_sc_update<R, 14>;
+  const auto kern = stage == 1    ? k_y_fwd_sc<R, N, 1>
+                    : stage == 2  ? k_y_fwd_sc<R, N, 2>
+                    : stage == 11 ? k_y_fwd_sc<R, N, 11>
+                    : stage == 12 ? k_y_fwd_sc<R, N, 12>
+                    : stage == 13 ? k_y_fwd_sc<R, N, 13>
+                                  : k_y_fwd_sc<R, N, 14>;
   const int nxh = c.geo.nxh, n = c.geo.n;
-  hipLaunchKernelGGL(kern, dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream, (C<R>*)u, (C<R>*)un,
-                     (const C<R>*)tr, c.geo, dt2, c.sx[k], c.sy[k], c.sz[k], dif, dif + nxh, dif + nxh + n,
-                     (C<R>*)u1);
+  hipLaunchKernelGGL(kern, dim3(c.geo.ngrp), dim3(NT), 0, c.stream, (const C<R>*)tr, (C<R>*)u, (C<R>*)un, c.geo,
+                     dt2, c.sx[k], c.sy[k], c.sz[k], dif, dif + nxh, dif + nxh + n, (const C<R>*)c.tw, (C<R>*)u1);
 }
-void launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
-                      double* dif, void* u1) {
+template <typename R>
+static int sc_update_r(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double* dif,
+                       void* u1) {
+  switch (c.geo.n) {
+    case 32: sc_update_n<R, 32>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+    case 64: sc_update_n<R, 64>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+    case 128: sc_update_n<R, 128>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+    case 256: sc_update_n<R, 256>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+    case 512: sc_update_n<R, 512>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+    case 1024: sc_update_n<R, 1024>(c, stage, u, un, tr, dt2, dif, u1); return 0;
+  }
+  return -1;
+}
+int launch_sc_update(const Ctx& c, int stage, void* u, void* un, const void* tr, double dt2, double rpr,
+                     double* dif, void* u1) {
   if (stage == 1 || stage == 11) {
     // c.xfac | c.yfac | c.zfac are one contiguous table of nxh + 2 n entries
     const int len = c.geo.nxh + 2 * c.geo.n;
     hipLaunchKernelGGL(k_sc_dif, dim3((len + 255) / 256), dim3(256), 0, c.stream, c.xfac, len, rpr, dif);
   }
-  if (c.prec == 8) sc_update_r<double>(c, stage, u, un, tr, dt2, dif, u1);
-  else sc_update_r<float>(c, stage, u, un, tr, dt2, dif, u1);
+  return c.prec == 8 ? sc_update_r<double>(c, stage, u, un, tr, dt2, dif, u1)
+                     : sc_update_r<float>(c, stage, u, un, tr, dt2, dif, u1);
 }
 template <typename R>
 static void epfor_r(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
