@@ -212,3 +212,17 @@ def test_epfor_increments_are_solenoidal_and_hermitian(O):
     back = g.xktran(g.kxtran(f, 3), 3)
     assert rel(back, f[:3]) < 1e-14
     assert np.abs(f[:, 0, 0]).max() == 0  # k = 0 is never forced
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_fp32_shell_index_equals_fp64(n):
+    """The GPU shell statistics form ik = int(|k| + 1.5) (PSDNS/ek_shell.f:66)
+    with a correctly rounded FP32 square root of the integer |k|^2: for every
+    |k|^2 a mask mode can reach (9 |k|^2 <= 2 n^2) it equals the FP64 index,
+    because sqrt(k2) + 1.5 stays at least 2.5e-4 away from an integer."""
+    k2 = np.arange(0, 2 * n * n // 9 + 1, dtype=np.int64)
+    fp64 = np.floor(np.sqrt(k2.astype(np.float64)) + 1.5).astype(np.int64)
+    fp32 = np.floor(np.sqrt(k2.astype(np.float32)) + np.float32(1.5)).astype(np.int64)
+    assert np.array_equal(fp32, fp64)
+    t = np.sqrt(k2.astype(np.float64)) + 1.5
+    assert np.min(np.abs(t - np.round(t))) > 2.5e-4
