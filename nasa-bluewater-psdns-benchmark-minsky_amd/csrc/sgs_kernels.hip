@@ -466,6 +466,13 @@ __device__ __forceinline__ void wave_sync() {
 }
 constexpr int kTileLd = 68;     // LDS row stride of a wave's 16 x 64 activation tile (floats)
 
+// an opaque copy of the lane index: addresses formed from it are recomputed
+// where they are used
+__device__ __forceinline__ int opaque_lane(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // B operand of K-step s, N-block nb of a layer with W[O][I] (row stride I)
 __device__ __forceinline__ bf16x8 b_operand(const float* __restrict__ W, int O, int I, int s, int nb, int lane) {
   const int o = 16 * nb + (lane & 15), k0 = 32 * s + 8 * (lane >> 4);
@@ -485,9 +492,13 @@ __device__ __forceinline__ bf16x8 a_from_tile(const float* H, int s, int lane) {
 }
 
 // tanh(x) = 1 - 2 / (exp(2 x) + 1) with the hardware exponential and
-// reciprocal (relative error ~1e-6, far below the bf16 operands' 4e-3;
-// saturates to +-1 for large |x|)
-__device__ __forceinline__ float tanh_fast(float x) { return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f); }
+// reciprocal, v_exp_f32 and v_rcp_f32 (relative error ~1e-6, far below the
+// bf16 operands' 4e-3; saturates to +-1 for large |x|).  A division here
+// (even __fdividef) compiles to the full IEEE sequence, several times the
+// cost of the rest of the activation.
+__device__ __forceinline__ float tanh_fast(float x) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
+}
 
 // tanh(C + bias) of N-block nb into the tile (C layout -> rows of H)
 __device__ __forceinline__ void c_to_tile(float* H, f32x4 c, float bias, int nb, int lane, int col0) {
@@ -496,25 +507,32 @@ __device__ __forceinline__ void c_to_tile(float* H, f32x4 c, float bias, int nb,
     H[(4 * (lane >> 4) + j) * kTileLd + col0 + 16 * nb + (lane & 15)] = tanh_fast(c[j] + bias);
 }
 
+// the weights' B operands in LDS, one 64-lane block (16 bytes per lane) per
+// (layer, K-step, N-block): W1 nb 0..3, W2 (s, nb) 4..11, W3 (s, nb)
+// 12..19, W4 s 20..21; a wave reads an operand with one 16-byte load per
+// lane (consecutive lanes, consecutive addresses)
+constexpr int kOpBlocks = 22;
+__device__ __forceinline__ bf16x8 b_block(const float* __restrict__ theta, int blk, int lane) {
+  if (blk < 4) return b_operand(theta + oW1, kHid, kIn1, 0, blk, lane);
+  if (blk < 12) return b_operand(theta + oW2, kHid, kHid, (blk - 4) / 4, (blk - 4) % 4, lane);
+  if (blk < 20) return b_operand(theta + oW3, kHid, kHid, (blk - 12) / 4, (blk - 12) % 4, lane);
+  return b_operand(theta + oW4, kBasis, kHid, blk - 20, 0, lane);
+}
+
 __global__ void __launch_bounds__(64 * kWaves)
     k_mlp_bf16(const float* __restrict__ lam, const float* __restrict__ h0, const float* __restrict__ T, int n,
                const float* __restrict__ theta, float* __restrict__ tau) {
   __shared__ float tile[kWaves][16 * kTileLd];
+  __shared__ bf16x8 wop[kOpBlocks][64];
+  for (int i = threadIdx.x; i < kOpBlocks * 64; i += blockDim.x) wop[i / 64][i % 64] = b_block(theta, i / 64, i % 64);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* H = tile[wave];
-  // weights as B operands (bf16), biases of this lane's output columns
-  bf16x8 w1[4], w2[2][4], w3[2][4], w4[2];
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) {
-    w1[nb] = b_operand(theta + oW1, kHid, kIn1, 0, nb, lane);
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      w2[s][nb] = b_operand(theta + oW2, kHid, kHid, s, nb, lane);
-      w3[s][nb] = b_operand(theta + oW3, kHid, kHid, s, nb, lane);
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 2; s++) w4[s] = b_operand(theta + oW4, kBasis, kHid, s, 0, lane);
+  // B operand of block blk, read per use (the lane index passes through an
+  // empty asm statement so that the reads are not hoisted out of the tile
+  // loop into registers: with all 22 operands resident the kernel runs 2
+  // waves per SIMD)
+  auto wb = [&](int blk) { return wop[blk][opaque_lane(lane)]; };
   const int col = lane & 15;
   float bb1[4], bb2[4], bb3[4];
 #pragma unroll
@@ -528,20 +546,41 @@ __global__ void __launch_bounds__(64 * kWaves)
   const size_t P = (size_t)n * n * n;
   const size_t ntiles = P / 16;  // n^3 is a multiple of 16 for n >= 4
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-  for (size_t t = (size_t)blockIdx.x * kWaves + wave; t < ntiles; t += (size_t)gridDim.x * kWaves) {
-    // operand A of layer 1: [lambda, h0] of the lane's point, k = 8 (lane / 16) + j
+  // A tile's global reads, issued one tile ahead so that they are in flight
+  // while the previous tile runs through the layers: xin[j] = input k =
+  // 8 (lane / 16) + j of the lane's point ([lambda, h0], zero past 21), and
+  // for the stress lanes (lane < 48: point lane % 16, components
+  // 2 (lane / 16) + cc) tb[4 cc + k] = T[k][component].
+  auto load_tile = [&](float (&xin)[8], float (&tb)[8], size_t t) {
     const size_t p = 16 * t + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = 8 * (lane >> 4) + j;
+      xin[j] = k < kLam ? lam[k * P + p] : (k < kIn1 ? h0[(k - kLam) * P + p] : 0.f);
+    }
+    if (lane < 48) {
+      const int c0i = 2 * (lane >> 4);
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+        for (int k = 0; k < kBasis; k++) tb[4 * cc + k] = T[(k * 6 + c0i + cc) * P + p];
+    }
+  };
+  const size_t step = (size_t)gridDim.x * kWaves;
+  size_t t = (size_t)blockIdx.x * kWaves + wave;
+  float xin[8] = {}, tb[8] = {};
+  if (t < ntiles) load_tile(xin, tb, t);
+  for (; t < ntiles; t += step) {
+    float xnext[8] = {}, tnext[8] = {};
+    if (t + step < ntiles) load_tile(xnext, tnext, t + step);
+    // operand A of layer 1: [lambda, h0] of the lane's point, k = 8 (lane / 16) + j
     f32x4 c1[4];
     {
       bf16x8 a;
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = 8 * (lane >> 4) + j;
-        const float v = k < kLam ? lam[k * P + p] : (k < kIn1 ? h0[(k - kLam) * P + p] : 0.f);
-        a[j] = (__bf16)v;
-      }
+      for (int j = 0; j < 8; j++) a[j] = (__bf16)xin[j];
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) c1[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w1[nb], zero, 0, 0, 0);
+      for (int nb = 0; nb < 4; nb++) c1[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(nb), zero, 0, 0, 0);
     }
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) c_to_tile(H, c1[nb], bb1[nb], nb, lane, 0);
@@ -551,7 +590,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     for (int s = 0; s < 2; s++) {
       const bf16x8 a = a_from_tile(H, s, lane);
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) c2[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2[s][nb], c2[nb], 0, 0, 0);
+      for (int nb = 0; nb < 4; nb++) c2[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(4 + 4 * s + nb), c2[nb], 0, 0, 0);
     }
     wave_sync();
 #pragma unroll
@@ -562,7 +601,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     for (int s = 0; s < 2; s++) {
       const bf16x8 a = a_from_tile(H, s, lane);
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) c3[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w3[s][nb], c3[nb], 0, 0, 0);
+      for (int nb = 0; nb < 4; nb++) c3[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(12 + 4 * s + nb), c3[nb], 0, 0, 0);
     }
     wave_sync();
 #pragma unroll
@@ -570,7 +609,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     wave_sync();
     f32x4 c4 = zero;
 #pragma unroll
-    for (int s = 0; s < 2; s++) c4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_from_tile(H, s, lane), w4[s], c4, 0, 0, 0);
+    for (int s = 0; s < 2; s++) c4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_from_tile(H, s, lane), wb(20 + s), c4, 0, 0, 0);
     wave_sync();
     // basis coefficients g[point][n] (n < 4) to the tile, then the stress
     if (col < kBasis) {
@@ -586,11 +625,16 @@ __global__ void __launch_bounds__(64 * kWaves)
         const int c = c0i + cc;
         float s = 0.f;
 #pragma unroll
-        for (int k = 0; k < kBasis; k++) s = fmaf(H[i * kTileLd + k], T[(k * 6 + c) * P + q], s);
+        for (int k = 0; k < kBasis; k++) s = fmaf(H[i * kTileLd + k], tb[4 * cc + k], s);
         tau[c * P + q] = s;
       }
     }
     wave_sync();
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      xin[j] = xnext[j];
+      tb[j] = tnext[j];
+    }
   }
 }
 
