@@ -444,27 +444,26 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 // Dense layers on the matrix cores: v_mfma_f32_16x16x32_bf16 (bf16 operands,
-// FP32 accumulation).  A wave takes 16 points per tile; each layer is
-// Y[16][O] = X[16][I] W^T: operand A (lane l) holds X[l % 16][k], k = 32 s +
-// 8 (l / 16) + j (j < 8), operand B holds W[16 nb + l % 16][k] for the same
-// k, and the accumulator C holds Y[4 (l / 16) + j][16 nb + l % 16] (j < 4).
-// The weights are converted to bf16 operands in registers once per wave;
-// between layers a wave moves its activations from the C to the A layout
-// through its own LDS tile.  Bias, tanh and the basis sum stay in FP32.
+// FP32 accumulation), D[16][16] = A[16][32] B[32][16] + C with lane l
+// holding A[l % 16][slot], B[slot][l % 16] for the eight K slots 8 (l / 16)
+// + j, and D[4 (l / 16) + jj][l % 16] (jj < 4).  The K order inside a
+// product is free as long as A and B use the same one.
+// Each layer is computed transposed, Y^T[O][16 points] = W X^T: operand A
+// is a 16-row block ob of the weights, operand B the activations, and the
+// accumulator of block ob leaves lane l with outputs o = 16 ob + 4 (l / 16)
+// + jj of point l % 16.  Those 16 values per lane are exactly the operand B
+// of the next layer if its K slots are ordered as
+//   step s, slot (g, j) <-> input k = 16 (2 s + j / 4) + 4 g + j % 4,
+// so the activations stay in the lane's registers from layer to layer (no
+// LDS tile, no cross-lane exchange).  The weights are stored as operand-A
+// blocks in LDS in that K order (layer 1 in natural order: its operand B is
+// the lane's own inputs k = 8 (l / 16) + j), and the biases as the
+// accumulators' initial values.  Bias and tanh are FP32.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;       // waves per work-group
-
-// orders a wave's LDS accesses across lanes (a wave executes its LDS
-// operations in order; the fences keep the compiler from moving them)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-constexpr int kTileLd = 68;     // LDS row stride of a wave's 16 x 64 activation tile (floats)
+constexpr int kWaves = 4;  // waves per work-group
 
 // an opaque copy of the lane index: addresses formed from it are recomputed
 // where they are used
@@ -473,22 +472,43 @@ __device__ __forceinline__ int opaque_lane(int v) {
   return v;
 }
 
-// B operand of K-step s, N-block nb of a layer with W[O][I] (row stride I)
-__device__ __forceinline__ bf16x8 b_operand(const float* __restrict__ W, int O, int I, int s, int nb, int lane) {
-  const int o = 16 * nb + (lane & 15), k0 = 32 * s + 8 * (lane >> 4);
-  bf16x8 r;
-#pragma unroll
-  for (int j = 0; j < 8; j++) r[j] = (__bf16)((o < O && k0 + j < I) ? W[o * I + k0 + j] : 0.f);
-  return r;
-}
+// input k of K step s and slot (g, j) of a layer fed by a 64-wide layer
+__device__ __forceinline__ constexpr int slot_k(int s, int g, int j) { return 16 * (2 * s + j / 4) + 4 * g + j % 4; }
 
-// A operand of K-step s from a wave's activation tile H[16][kTileLd]
-__device__ __forceinline__ bf16x8 a_from_tile(const float* H, int s, int lane) {
-  const float* row = H + (lane & 15) * kTileLd + 32 * s + 8 * (lane >> 4);
-  bf16x8 r;
+// operand-A blocks in LDS, 64 lanes x 16 bytes each: W1 ob 0..3, W2 (s, ob)
+// 4..11, W3 (s, ob) 12..19, W4 s 20..21 (rows 0..3 of 16)
+constexpr int kOpBlocks = 22;
+__device__ __forceinline__ bf16x8 a_block(const float* __restrict__ theta, int blk, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  bf16x8 v;
+  if (blk < 4) {
+    const int o = 16 * blk + r;
 #pragma unroll
-  for (int j = 0; j < 8; j++) r[j] = (__bf16)row[j];
-  return r;
+    for (int j = 0; j < 8; j++) {
+      const int k = 8 * g + j;
+      v[j] = (__bf16)(k < kIn1 ? theta[oW1 + o * kIn1 + k] : 0.f);
+    }
+    return v;
+  }
+  const int base = blk < 12 ? oW2 : (blk < 20 ? oW3 : oW4);
+  const int s = blk < 20 ? ((blk - 4) % 8) / 4 : blk - 20;
+  const int o = blk < 20 ? 16 * ((blk - 4) % 4) + r : r;
+  const int rows = blk < 20 ? kHid : kBasis;
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = (__bf16)(o < rows ? theta[base + o * kHid + slot_k(s, g, j)] : 0.f);
+  return v;
+}
+// accumulator initial values: bias of outputs 16 ob + 4 g + jj (layer 0..2
+// the hidden layers, ob < 4; layer 3 the output, only g = 0 valid)
+__device__ __forceinline__ f32x4 bias_block(const float* __restrict__ theta, int layer, int ob, int g) {
+  f32x4 b;
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    const int o = 16 * ob + 4 * g + jj;
+    b[jj] = layer == 0 ? theta[ob1 + o] : layer == 1 ? theta[ob2 + o] : layer == 2 ? theta[ob3 + o]
+                                                                              : (o < kBasis ? theta[ob4 + o] : 0.f);
+  }
+  return b;
 }
 
 // tanh(x) = 1 - 2 / (exp(2 x) + 1) with the hardware exponential and
@@ -499,53 +519,38 @@ __device__ __forceinline__ bf16x8 a_from_tile(const float* H, int s, int lane) {
 __device__ __forceinline__ float tanh_fast(float x) {
   return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
 }
-
-// tanh(C + bias) of N-block nb into the tile (C layout -> rows of H)
-__device__ __forceinline__ void c_to_tile(float* H, f32x4 c, float bias, int nb, int lane, int col0) {
+// operand B of K step s of the next layer: tanh of accumulator blocks 2 s
+// and 2 s + 1 of this lane
+__device__ __forceinline__ bf16x8 act_operand(f32x4 d0, f32x4 d1) {
+  bf16x8 r;
 #pragma unroll
-  for (int j = 0; j < 4; j++)
-    H[(4 * (lane >> 4) + j) * kTileLd + col0 + 16 * nb + (lane & 15)] = tanh_fast(c[j] + bias);
-}
-
-// the weights' B operands in LDS, one 64-lane block (16 bytes per lane) per
-// (layer, K-step, N-block): W1 nb 0..3, W2 (s, nb) 4..11, W3 (s, nb)
-// 12..19, W4 s 20..21; a wave reads an operand with one 16-byte load per
-// lane (consecutive lanes, consecutive addresses)
-constexpr int kOpBlocks = 22;
-__device__ __forceinline__ bf16x8 b_block(const float* __restrict__ theta, int blk, int lane) {
-  if (blk < 4) return b_operand(theta + oW1, kHid, kIn1, 0, blk, lane);
-  if (blk < 12) return b_operand(theta + oW2, kHid, kHid, (blk - 4) / 4, (blk - 4) % 4, lane);
-  if (blk < 20) return b_operand(theta + oW3, kHid, kHid, (blk - 12) / 4, (blk - 12) % 4, lane);
-  return b_operand(theta + oW4, kBasis, kHid, blk - 20, 0, lane);
+  for (int jj = 0; jj < 4; jj++) {
+    r[jj] = (__bf16)tanh_fast(d0[jj]);
+    r[4 + jj] = (__bf16)tanh_fast(d1[jj]);
+  }
+  return r;
 }
 
 __global__ void __launch_bounds__(64 * kWaves)
     k_mlp_bf16(const float* __restrict__ lam, const float* __restrict__ h0, const float* __restrict__ T, int n,
                const float* __restrict__ theta, float* __restrict__ tau) {
-  __shared__ float tile[kWaves][16 * kTileLd];
   __shared__ bf16x8 wop[kOpBlocks][64];
-  for (int i = threadIdx.x; i < kOpBlocks * 64; i += blockDim.x) wop[i / 64][i % 64] = b_block(theta, i / 64, i % 64);
+  __shared__ f32x4 bias[13][4];  // [3 layers x 4 ob + the output layer][g]
+  for (int i = threadIdx.x; i < kOpBlocks * 64; i += blockDim.x) wop[i / 64][i % 64] = a_block(theta, i / 64, i % 64);
+  for (int i = threadIdx.x; i < 13 * 4; i += blockDim.x) {
+    const int b = i / 4, g = i % 4;
+    bias[b][g] = b < 12 ? bias_block(theta, b / 4, b % 4, g) : bias_block(theta, 3, 0, g);
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float* H = tile[wave];
-  // B operand of block blk, read per use (the lane index passes through an
-  // empty asm statement so that the reads are not hoisted out of the tile
-  // loop into registers: with all 22 operands resident the kernel runs 2
-  // waves per SIMD)
-  auto wb = [&](int blk) { return wop[blk][opaque_lane(lane)]; };
-  const int col = lane & 15;
-  float bb1[4], bb2[4], bb3[4];
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) {
-    bb1[nb] = theta[ob1 + 16 * nb + col];
-    bb2[nb] = theta[ob2 + 16 * nb + col];
-    bb3[nb] = theta[ob3 + 16 * nb + col];
-  }
-  const float bb4 = col < kBasis ? theta[ob4 + col] : 0.f;
+  // operand A of block blk and the bias of accumulator block b, read per use
+  // (through an opaque lane index, so that they are not hoisted out of the
+  // tile loop into registers)
+  auto wa = [&](int blk) { return wop[blk][opaque_lane(lane)]; };
+  auto bc = [&](int b) { return bias[b][opaque_lane(lane) >> 4]; };
 
   const size_t P = (size_t)n * n * n;
   const size_t ntiles = P / 16;  // n^3 is a multiple of 16 for n >= 4
-  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   // A tile's global reads, issued one tile ahead so that they are in flight
   // while the previous tile runs through the layers: xin[j] = input k =
   // 8 (lane / 16) + j of the lane's point ([lambda, h0], zero past 21), and
@@ -573,63 +578,47 @@ __global__ void __launch_bounds__(64 * kWaves)
   for (; t < ntiles; t += step) {
     float xnext[8] = {}, tnext[8] = {};
     if (t + step < ntiles) load_tile(xnext, tnext, t + step);
-    // operand A of layer 1: [lambda, h0] of the lane's point, k = 8 (lane / 16) + j
-    f32x4 c1[4];
-    {
-      bf16x8 a;
+    // layer 1: operand B the lane's own inputs
+    bf16x8 x;
 #pragma unroll
-      for (int j = 0; j < 8; j++) a[j] = (__bf16)xin[j];
+    for (int j = 0; j < 8; j++) x[j] = (__bf16)xin[j];
+    f32x4 d[4];
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) c1[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(nb), zero, 0, 0, 0);
-    }
+    for (int ob = 0; ob < 4; ob++) d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa(ob), x, bc(ob), 0, 0, 0);
+    // layers 2 and 3
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) c_to_tile(H, c1[nb], bb1[nb], nb, lane, 0);
-    wave_sync();
-    f32x4 c2[4] = {zero, zero, zero, zero};
+    for (int layer = 1; layer < 3; layer++) {
+      const bf16x8 b0 = act_operand(d[0], d[1]), b1 = act_operand(d[2], d[3]);
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const bf16x8 a = a_from_tile(H, s, lane);
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) c2[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(4 + 4 * s + nb), c2[nb], 0, 0, 0);
-    }
-    wave_sync();
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++) c_to_tile(H, c2[nb], bb2[nb], nb, lane, 0);
-    wave_sync();
-    f32x4 c3[4] = {zero, zero, zero, zero};
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const bf16x8 a = a_from_tile(H, s, lane);
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) c3[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb(12 + 4 * s + nb), c3[nb], 0, 0, 0);
-    }
-    wave_sync();
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++) c_to_tile(H, c3[nb], bb3[nb], nb, lane, 0);
-    wave_sync();
-    f32x4 c4 = zero;
-#pragma unroll
-    for (int s = 0; s < 2; s++) c4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_from_tile(H, s, lane), wb(20 + s), c4, 0, 0, 0);
-    wave_sync();
-    // basis coefficients g[point][n] (n < 4) to the tile, then the stress
-    if (col < kBasis) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) H[(4 * (lane >> 4) + j) * kTileLd + col] = c4[j] + bb4;
-    }
-    wave_sync();
-    if (lane < 48) {
-      const int i = lane & 15, c0i = 2 * (lane >> 4);
-      const size_t q = 16 * t + i;
-#pragma unroll
-      for (int cc = 0; cc < 2; cc++) {
-        const int c = c0i + cc;
-        float s = 0.f;
-#pragma unroll
-        for (int k = 0; k < kBasis; k++) s = fmaf(H[i * kTileLd + k], tb[4 * cc + k], s);
-        tau[c * P + q] = s;
+      for (int ob = 0; ob < 4; ob++) {
+        const int blk = 4 + 8 * (layer - 1) + ob;
+        f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa(blk), b0, bc(4 * layer + ob), 0, 0, 0);
+        d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa(blk + 4), b1, acc, 0, 0, 0);
       }
     }
-    wave_sync();
+    // output layer: rows 0..3 (lanes 0..15) are the basis coefficients g of
+    // point lane % 16
+    f32x4 g4;
+    {
+      const bf16x8 b0 = act_operand(d[0], d[1]), b1 = act_operand(d[2], d[3]);
+      g4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa(20), b0, bc(12), 0, 0, 0);
+      g4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa(21), b1, g4, 0, 0, 0);
+    }
+    // the stress: lanes < 48 take g of their point from lane lane % 16
+    float gk[kBasis];
+#pragma unroll
+    for (int k = 0; k < kBasis; k++) gk[k] = __shfl(g4[k], lane & 15);
+    if (lane < 48) {
+      const int c0i = 2 * (lane >> 4);
+      const size_t q = 16 * t + (lane & 15);
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        float sacc = 0.f;
+#pragma unroll
+        for (int k = 0; k < kBasis; k++) sacc = fmaf(gk[k], tb[4 * cc + k], sacc);
+        tau[(c0i + cc) * P + q] = sacc;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       xin[j] = xnext[j];
