@@ -1868,8 +1868,9 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 //  k_x_fused<..., SC> (with the x passes) itransform.f:522-531: the
 //               advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
 //               of every scalar, beside the velocity's products
-//  k_sc_update  wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz) T with
-//               T the normalised forward transform of term; proc4a.f:506-520
+//  k_y_fwd_sc   the term's y forward transform (xkcomm2.f:229) with the
+//               update, wavespace_module.f:347-352: N = -(dt/2) conj(sx sy sz)
+//               T with T the normalised forward transform of term; proc4a.f:506-520
 //               (STAGE 1: F = xdif ydif zdif, xdif = xfac^(1/pr), step.f:
 //               387-407; un <- F (un + N), u <- F (un + 2 N)) or proc4b.f:
 //               69-74,183-190,201-205 (STAGE 2: u <- un + N, un <- u, zero
