@@ -335,3 +335,25 @@ def test_les_fused_matches_per_routine(O, P, prec):
     err = rel(out[1], out[0], g.mask()[None])
     print(f"fused vs per-routine LES after 3 steps: rel L2 {err:.2e}", flush=True)
     assert err < (1e-10 if prec == 8 else 2e-5)
+
+
+def test_les_stage_with_scalars_is_refused(O, P):
+    """The fused x pass carries either the subgrid stress or the scalars'
+    advection terms: attaching a stress to a context with passive scalars
+    is refused with PSDNS_EINVAL and leaves the context usable."""
+    n = 32
+    print("[gpu case] attach stress with nc = 1", n, flush=True)
+    s = P.Solver(n, 4, viscos=0.01, nc=1, pr=[1.0], grad=[[1.0, 0.0, 0.0]])
+    m = P.Closure(n)
+    try:
+        s.init_field(seed=5, kpeak=3.0, tke=0.5)
+        m.set_params(_theta(7))
+        rng = np.random.default_rng(3)
+        m.put_fields(rng.standard_normal((5, n, n, n)), rng.standard_normal((24, n, n, n)))
+        m.predict(True)
+        with pytest.raises(P.PsdnsError, match="passive scalars"):
+            m.attach(s, 1)
+        s.shell_stats(0)  # the context still works
+    finally:
+        m.close()
+        s.close()
