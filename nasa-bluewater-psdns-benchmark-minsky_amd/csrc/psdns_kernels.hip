@@ -221,20 +221,29 @@ struct BufView {
       return {__hiloint2double((int)d[1], (int)d[0]), __hiloint2double((int)d[3], (int)d[2])};
     }
   }
-  // soff: a wave-uniform byte offset added to off (a scalar register); a
-  // lane with off = kOob stays out of range
+  // soff: a wave-uniform byte offset added to off; a lane with off = kOob
+  // stays out of range.  An 8-byte store takes soff in the instruction's
+  // scalar offset register.  A 16-byte store adds it to its vector offset
+  // and keeps the scalar offset an immediate 0: a buffer store of more than
+  // 8 bytes reads its data registers after issue, the compiler inserts the
+  // wait state that protects them from the next VALU write only when the
+  // scalar offset is not a register, and on gfx950 such a store with a
+  // register offset was seen to store the value a following v_mov wrote
+  // into the low dword of its data (FP64 results differing between reruns).
   __device__ __forceinline__ void store(unsigned off, C<R> v, unsigned soff = 0) const {
-    asm volatile("" : "+v"(off));
     if constexpr (sizeof(R) == 4) {
+      asm volatile("" : "+v"(off));
       using V2 = unsigned __attribute__((ext_vector_type(2)));
       V2 d = {__float_as_uint(v.x), __float_as_uint(v.y)};
       __builtin_amdgcn_raw_buffer_store_b64(d, r, off, soff, 0);
     } else {
+      off += soff;
+      asm volatile("" : "+v"(off));
       using V4 = unsigned __attribute__((ext_vector_type(4)));
       const unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
       const unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
       V4 d = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
-      __builtin_amdgcn_raw_buffer_store_b128(d, r, off, soff, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
     }
   }
 };

@@ -280,18 +280,17 @@ def test_fortran_driver_matches_oracle(O, P, tmp_path):
 # (differences of about 1e-27 on values of about 1e-11, far below every
 # parity tolerance); the cause is not found.  Non-strict: the variation
 # does not show on every run.
-FP64_VARIES = pytest.mark.xfail(strict=False, reason="FP64 64^3/128^3 run-to-run variation in the x pass's "
-                                "partly stored kx block, cause not found")
-
-
-@pytest.mark.parametrize("n,prec", [(32, 8), pytest.param(64, 8, marks=FP64_VARIES),
-                                    pytest.param(128, 8, marks=FP64_VARIES), (256, 8), (256, 4)])
+@pytest.mark.parametrize("n,prec", [(32, 8), (64, 8), (128, 8), (256, 8), (64, 4), (256, 4)])
 def test_fused_step_reproducible(P, O, n, prec):
-    """The same state stepped twice on one context gives the same bits.
+    """The same state stepped three times on one context gives the same bits.
     The x pass stages each variable into LDS with asynchronous copies that
     several waves read, so every wave waits for its own copies before the
     publishing barrier; a missing wait shows up here as run-to-run
-    differences in the highest kx modes."""
+    differences in the highest kx modes.  In FP64 the forward x transform
+    stores 16 bytes per lane: with the per-j offset in the store's scalar
+    offset register, the low dword of a stored value was sometimes the one
+    the next instruction wrote (64^3 and 128^3, last kx block of X slot 1);
+    the offset now goes into the vector offset."""
     print("[gpu case] reproducible", n, prec, flush=True)
     g = O.Grid(n)
     u0 = g.field(seed=9)
