@@ -15,7 +15,9 @@ its step time, per-pass device times per RK substep and the roofline of the
 scalar path's dominant kernel go into the line's "config3_nc2" object
 (--no-config3 skips it).  Step times are also reported as min/mean/max over
 steps 2..K.  The "les_closure" object times the learned-closure LES at 128^3
-against the 512^3 DNS it stands for (--no-les skips it).
+against the 512^3 DNS it stands for (--no-les skips it), and
+"config4_particles" a 512^3 step with 2^20 Lagrangian particles against the
+same step without them (--no-particles skips it).
 
 The CPU baseline times the FP64 oracle (a port of the reference algorithm)
 on a bounded sample (RK2 steps at 256^3 for about 10-20 s, all host cores)
@@ -231,6 +233,63 @@ def les_case(P, steps=10, seed=122929298):
     }
 
 
+def particles_case(P, steps=10, seed=122929298, npart=1 << 20):
+    """Configuration 4: a forced 512^3 FP32 RK2 step with 2^20 Lagrangian
+    particles uniform in the box (positions 1 + U n in grid units, seed
+    122929299), in partsp.f's order (rk_advpos 1, spline + interpolation at
+    the new shifts, rk_advpos 2), against the same step without particles;
+    then each particle entry point timed on its own (each returns after its
+    kernels finish).  The interpolation gathers 4 x 4 x 4 spline
+    coefficients of 3 components per particle and reads and writes 3 FP64
+    position / velocity entries: 64 * 3 * 4 + 48 bytes per particle in
+    FP32."""
+    n = 512
+    s = P.Solver(n, 4, viscos=1.0)
+    try:
+        s.init_field(seed=seed, kpeak=3.0, tke=0.5)
+        _, _, _, e1 = s.shell_stats(0)
+        nu = math.sqrt(e1) * (1.5 / (math.sqrt(2.0) / 3.0 * n)) ** 2
+        s.viscos = nu
+        rng = np.random.default_rng(seed)
+
+        def timed(run, k):
+            for _ in range(2):
+                run.step(rng.random(6))
+            a = time.perf_counter()
+            for _ in range(k):
+                run.step(rng.random(6))
+            return (time.perf_counter() - a) / k * 1e3
+
+        plain_ms = timed(P.Run(s, nu, math.pi, kf_shell=3), steps)
+        pp = np.zeros((9, npart))
+        pp[:3] = 1.0 + np.random.default_rng(122929299).random((3, npart)) * n
+        run = P.Run(s, nu, math.pi, kf_shell=3, particles=pp)
+        with_ms = timed(run, steps)
+        dt = run.dts[-1]
+        calls = {"advpos_1": lambda: s.lag_advpos(1, dt, 0.5, 0.5), "spline": s.lag_spline,
+                 "interp": s.lag_interp, "advpos_2": lambda: s.lag_advpos(2, dt, 0.0, 0.5)}
+        call_ms = {k: [] for k in calls}
+        for _ in range(steps):
+            for k, f in calls.items():
+                a = time.perf_counter()
+                f()
+                call_ms[k].append((time.perf_counter() - a) * 1e3)
+    finally:
+        s.close()
+    ms = {k: round(float(np.median(v)), 4) for k, v in call_ms.items()}
+    interp_bytes = npart * (64 * 3 * 4 + 48)
+    gbs = interp_bytes / (ms["interp"] * 1e-3) / 1e9
+    return {
+        "workload": "forced DNS 512^3 FP32 RK2 with 2^20 particles uniform in the box (LAG_LOCAL cubic-spline "
+                    "path, partsp RK2 order), against the same step without particles",
+        "particles": npart, "ms_per_step": round(with_ms, 3), "ms_per_step_no_particles": round(plain_ms, 3),
+        "particle_ms_per_step": round(with_ms - plain_ms, 3), "call_ms_median": ms,
+        "interp_roofline": {"bound": "hbm (gather)", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_call": interp_bytes,
+                            "note": "host-timed call (kernel + launch + sync)"},
+    }
+
+
 def fft_summary(r, names):
     nb, per, passes = r["nb"], r["per"], r["passes"]
     timed = [nm for nm in names if nm in passes and passes[nm]["GBps"] is not None]
@@ -304,6 +363,8 @@ def main():
     ap.add_argument("--config3-steps", type=int, default=5)
     ap.add_argument("--no-les", dest="les", action="store_false",
                     help="skip the learned-closure LES vs DNS timing")
+    ap.add_argument("--no-particles", dest="particles", action="store_false",
+                    help="skip the 512^3 step with 2^20 Lagrangian particles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -402,6 +463,9 @@ def main():
     if args.les and args.rk == 2 and n == 1024:
         print("[bench] les vs dns", file=sys.stderr, flush=True)
         line["les_closure"] = les_case(P)
+    if args.particles and args.rk == 2 and n == 1024:
+        print("[bench] particles", file=sys.stderr, flush=True)
+        line["config4_particles"] = particles_case(P)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n)
