@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "psdns_gpu.h"
 #include "psdns_internal.h"
 #include "sgs_internal.h"
@@ -125,6 +127,10 @@ struct psdns_ctx {
   // hold the spline coefficients of lag_spline (and its stage-1 offsets)
   double* d_pp = nullptr;
   int lag_np = 0;
+  // the interpolation order: bin keys and particle numbers (each twice,
+  // sort input and output) and the radix sort's temporary storage
+  void* d_lag_sort = nullptr;
+  size_t lag_tmp_bytes = 0;
   bool spline_ok = false;
   double spline_gsh[3] = {};
   // EPFOR forcing increments (psdns_set_epfor): the cylinder entries of the
@@ -602,7 +608,7 @@ int psdns_destroy(psdns_ctx* c) {
   void* ptrs[] = {c->d_colx, c->d_colz, c->d_cutz, c->d_col0,  c->d_colz0, c->d_grp, c->d_u,   c->d_un,
                   c->d_w1,   c->d_xi,   c->d_xs,   c->d_ux,    c->d_tw,     c->d_shift, c->d_shiftr, c->d_fac,
                   c->d_red,  c->d_velmax, c->d_force, c->d_usc, c->d_u1, c->d_epf, c->d_epf_for,
-                  c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part, c->d_pp};
+                  c->d_epf_terms, c->d_efkz, c->d_force_terms, c->d_shell_part, c->d_pp, c->d_lag_sort};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_fac) (void)hipHostFree(c->h_fac);
@@ -1189,17 +1195,35 @@ int psdns_xktran(psdns_ctx* c, int nv) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
+// bits per axis of the interpolation's bin keys: n / 4 = 2^lb bins of 4
+// cells (0: no sort)
+static int lag_bits(const psdns_ctx* c) {
+  int lb = 0;
+  while ((4 << lb) < c->n) lb++;
+  return lb;
+}
+
 int psdns_lag_put(psdns_ctx* c, const double* pp, int np, int ncols) {
   if (int rc = ready(c)) return rc;
   if (!pp || np < 1) return fail(PSDNS_EINVAL, "need particles (np = %d)", np);
   if (ncols != 3 && ncols != 6 && ncols != 9) return fail(PSDNS_EINVAL, "ncols = %d: 3, 6 or 9", ncols);
   if (np != c->lag_np) {
     if (c->d_pp) HIPCHK(hipFree(c->d_pp));
+    if (c->d_lag_sort) HIPCHK(hipFree(c->d_lag_sort));
     c->d_pp = nullptr;
+    c->d_lag_sort = nullptr;
     c->lag_np = 0;
     const size_t bytes = sizeof(double) * 9 * (size_t)np;
     if (hipMalloc(&c->d_pp, bytes) != hipSuccess)
       return fail(PSDNS_ENOMEM, "hipMalloc of the particle array (%zu bytes) failed", bytes);
+    size_t tmp = 0;
+    if (const int lb = lag_bits(c))
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                                (const int*)nullptr, (int*)nullptr, np, 0, 3 * lb, c->c.stream));
+    c->lag_tmp_bytes = (tmp + 255) / 256 * 256;
+    const size_t sbytes = c->lag_tmp_bytes + 4 * sizeof(int) * (size_t)np;
+    if (hipMalloc(&c->d_lag_sort, sbytes) != hipSuccess)
+      return fail(PSDNS_ENOMEM, "hipMalloc of the particle sort buffers (%zu bytes) failed", sbytes);
     c->lag_np = np;
   }
   const size_t col = sizeof(double) * (size_t)np;
@@ -1220,13 +1244,17 @@ int psdns_lag_spline(psdns_ctx* c) {
   if (int rc = ready_clobber(c)) return rc;
   int rc;
   if ((rc = ensure_ux(c))) return rc;
-  // the spline spectrum of un on the stage-1 shifted grid, in the X work
-  // array as 3 cylinder slots, then the inverse passes of kxtran into ux
-  psdns::launch_spline_prep(c->c, c->d_xs);
+  // the spline spectrum of un on the stage-1 shifted grid, u sx sy sz times
+  // the spline factors, formed on load by the inverse y pass (its shift
+  // tables, in the X work array, carry the factors), then the inverse
+  // passes of kxtran into ux
+  psdns::launch_spline_shifts(c->c, c->d_xs);
   HIPCHK(hipGetLastError());
   Ctx k = c->c;
-  k.u = c->d_xs;
-  if (psdns::launch_pass(k, Pass::YInvPlain, 3, 0) || psdns::launch_pass(k, Pass::ZInv, 3, 0) ||
+  k.u = c->d_un;
+  k.kstep = 0;
+  k.sr[0] = c->d_xs;
+  if (psdns::launch_pass(k, Pass::YInvPhase, 3, 0) || psdns::launch_pass(k, Pass::ZInv, 3, 0) ||
       psdns::launch_pass(k, Pass::XC2R, 3, 0))
     return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   HIPCHK(hipGetLastError());
@@ -1241,7 +1269,24 @@ int psdns_lag_interp(psdns_ctx* c) {
   if (!c->d_pp) return fail(PSDNS_ESTATE, "no particles (psdns_lag_put)");
   if (!c->spline_ok)
     return fail(PSDNS_ESTATE, "no spline coefficients: psdns_lag_spline first (and no work-array call after it)");
-  psdns::launch_lag_interp(c->c, c->d_pp, c->lag_np, c->spline_gsh);
+  // the interpolation order: particles sorted by their bin of 4 x 4 x 4
+  // cells (a permutation only: each particle's sums are the same)
+  const int np = c->lag_np;
+  const int* order = nullptr;
+  if (const int lb = lag_bits(c)) {
+    // d_lag_sort: the sort's temporary storage, then keys in / out, then
+    // particle numbers in / out
+    unsigned* k0 = (unsigned*)((char*)c->d_lag_sort + c->lag_tmp_bytes);
+    unsigned* k1 = k0 + np;
+    int* i0 = (int*)(k1 + np);
+    int* i1 = i0 + np;
+    psdns::launch_lag_key(c->c, c->d_pp, np, lb, c->spline_gsh, k0, i0);
+    HIPCHK(hipGetLastError());
+    size_t tmp = c->lag_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->d_lag_sort, tmp, k0, k1, i0, i1, np, 0, 3 * lb, c->c.stream));
+    order = i1;
+  }
+  psdns::launch_lag_interp(c->c, c->d_pp, np, c->spline_gsh, order);
   HIPCHK(hipGetLastError());
   return sync(c);
 }

@@ -143,11 +143,15 @@ void launch_vg_deriv(const Ctx& c, void* d);
 void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const double2* forc, int fstride,
                   double* terms, double* efkz);
 void launch_moments(const Ctx& c, double* d_out);
-// Lagrangian particles: the spline spectrum of un (stage-1 shifts, 3
-// cylinder slots at out), the interpolation of the coefficients in ux[0..2]
-// at pp(np, 1:3) into pp(np, 4:6), the RK2 position update
-void launch_spline_prep(const Ctx& c, void* out);
-void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3]);
+// Lagrangian particles: the spline's shift tables (stage-1 shifts times
+// the spline factors, sr layout, state precision, at sr) for the inverse y
+// pass's phase mode, the interpolation of the coefficients in ux[0..2] at
+// pp(np, 1:3) into pp(np, 4:6), the RK2 position update
+void launch_spline_shifts(const Ctx& c, void* sr);
+// bin keys of the interpolation order (lb = log2(n / 4)); interpolation in
+// the order `order` (null: particle order)
+void launch_lag_key(const Ctx& c, const double* pp, int np, int lb, const double gsh1[3], unsigned* key, int* idx);
+void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3], const int* order);
 void launch_lag_advpos(const Ctx& c, double* pp, int np, int pstep, double fact1, double fact2);
 
 }  // namespace psdns

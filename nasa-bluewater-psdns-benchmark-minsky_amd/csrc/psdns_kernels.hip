@@ -2014,32 +2014,82 @@ __global__ void __launch_bounds__(256) k_moments(const R* __restrict__ f, size_t
 // stage-1 phase-shifted grid: the periodic system is diagonal in Fourier
 // space, eigenvalue 4 + 2 cos(2 pi k / n) per direction, so the coefficients
 // are the inverse transform of  u(k) sx sy sz (6 / (4 + 2 cos))^3-product.
-// k_spline_prep forms that spectrum on the mask (0 elsewhere, as the
-// state), the inverse passes of kxtran put the coefficients in ux[0..2].
+// That spectrum on the mask (0 elsewhere, as the state) is formed by the
+// inverse y pass, and the inverse passes of kxtran put the coefficients in
+// ux[0..2].
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double spline_eig(int k, int n) { return 6.0 / (4.0 + 2.0 * cospi(2.0 * k / n)); }
 
+// The spline spectrum is formed on load by the inverse y pass's phase mode
+// (k_y_inv MODE 0, which multiplies by sy(y) sx(x) sz(z) on the mask): its
+// shift tables become the stage-1 shifts times the spline factor of each
+// direction, sr = sx[nxh] | sy[n] | sz[n] in the state precision.
 template <typename R>
 __global__ void __launch_bounds__(256)
-    k_spline_prep(const C<R>* __restrict__ un, C<R>* __restrict__ out, Geo g, const double2* __restrict__ sx,
-                  const double2* __restrict__ sy, const double2* __restrict__ sz) {
-  const size_t tot = (size_t)g.na * g.n;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const Pt p = point(i, g);
-    C<R> o[3] = {{0, 0}, {0, 0}, {0, 0}};
-    if (p.m) {
-      const double e = spline_eig(p.x, g.n) * spline_eig(p.ky, g.n) * spline_eig(p.kz, g.n);
-      const double2 s = zmul(zmul(sx[p.x], sz[p.z]), sy[p.y]);
+    k_spline_shifts(const double2* __restrict__ sx, const double2* __restrict__ sy, const double2* __restrict__ sz,
+                    C<R>* __restrict__ sr, int n, int nxh) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nxh + 2 * n) return;
+  const int dir = r < nxh ? 0 : (r < nxh + n ? 1 : 2);
+  const int i = dir == 0 ? r : (dir == 1 ? r - nxh : r - nxh - n);
+  const double2 s = dir == 0 ? sx[i] : (dir == 1 ? sy[i] : sz[i]);
+  const double e = spline_eig(dir == 0 ? i : kwave(i, n), n);
+  sr[r] = C<R>{(R)(s.x * e), (R)(s.y * e)};
+}
+
+// spcal_local.f:186-203: the cell ib (0 .. n-1) of position p (grid
+// units) relative to the shifted grid (gfn = g + 1), and the offset pos in
+// it.  The reference stops on a position it cannot place (spcal_local.f:
+// 186-190); here such a position (non-finite or beyond 1000 periods) takes
+// node 0 so that every index stays inside the coefficient arrays.
+__device__ __forceinline__ int lag_cell(double p, double g, int n, double& pos) {
+  const double px = p - (g + 1.0) - (double)n;
+  const double fl = fabs(px) < 1e15 ? floor(px) : 0.0;
+  long long jpx = (long long)fl;
+  if (jpx < 0) jpx += 1000LL * n;
+  long long jloc = jpx % n;
+  if (jloc < 0) jloc += n;
+  double bfp = (double)jloc + px - fl;
+  if (!(bfp >= 0.0 && bfp < (double)n)) bfp = 0.0;
+  const int ib = (int)bfp;
+  pos = bfp - ib;
+  return ib;
+}
+
+// c[x0 .. x0 + 3] of one coefficient row as one vector load (4-byte or
+// 8-byte aligned, consecutive x); xs[i] = (x0 + i) mod n element by element
+// where the 4 nodes wrap around x = n
+#ifndef PSDNS_LAG_VEC
+#define PSDNS_LAG_VEC 1
+#endif
+template <typename R>
+__device__ __forceinline__ void lag_row(const R* __restrict__ row, const int xs[4], int n, R out[4]) {
+  typedef R v4 __attribute__((ext_vector_type(4)));
+  typedef v4 v4u __attribute__((aligned(sizeof(R))));
+  if (PSDNS_LAG_VEC && xs[0] <= n - 4) {
+    const v4 q = *reinterpret_cast<const v4u*>(row + xs[0]);
 #pragma unroll
-      for (int v = 0; v < 3; v++) {
-        const C<R> a = un[v * tot + i];
-        const double2 w = zmul(make_double2((double)a.x, (double)a.y), s);
-        o[v] = {(R)(w.x * e), (R)(w.y * e)};
-      }
-    }
+    for (int i = 0; i < 4; i++) out[i] = q[i];
+  } else {
 #pragma unroll
-    for (int v = 0; v < 3; v++) out[v * tot + i] = o[v];
+    for (int i = 0; i < 4; i++) out[i] = row[xs[i]];
   }
+}
+
+// Sort keys of the interpolation order: the bin of 4 x 4 x 4 cells holding
+// each particle's cell, bins numbered as the coefficient arrays c[y][z][x]
+// (x fastest, lb = log2(n / 4) bits each); idx = the particle number.
+__global__ void __launch_bounds__(256)
+    k_lag_key(const double* __restrict__ pp, int np, int n, int lb, double g0, double g1, double g2,
+              unsigned* __restrict__ key, int* __restrict__ idx) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= np) return;
+  double pos;
+  const unsigned bx = (unsigned)lag_cell(pp[m], g0, n, pos) >> 2;
+  const unsigned by = (unsigned)lag_cell(pp[(size_t)np + m], g1, n, pos) >> 2;
+  const unsigned bz = (unsigned)lag_cell(pp[2 * (size_t)np + m], g2, n, pos) >> 2;
+  key[m] = (((by << lb) | bz) << lb) | bx;
+  idx[m] = m;
 }
 
 // spcal_local.f:119-364 (iod = 0, sf = 1), one thread per particle: the
@@ -2047,11 +2097,19 @@ __global__ void __launch_bounds__(256)
 // 1), the B-spline weights of its cell and the 4 x 4 x 4 coefficients of
 // nodes ib - 1 .. ib + 2 (periodic) of the three fields c[v][y][z][x],
 // summed in the reference's order in FP64; pp column-major pp(np, 9).
+// Particles are taken in the order `order` (sorted by bin: particles that
+// share coefficient cache lines run together; null = 0 .. np-1), and the
+// grid's work-groups, a multiple of 8 that go to the 8 XCDs in turn, give
+// each XCD one contiguous range of that order, so each XCD's L2 cache
+// serves one region of the box.
 template <typename R>
 __global__ void __launch_bounds__(256)
-    k_lag_interp(const R* __restrict__ c, double* __restrict__ pp, int np, int n, double g0, double g1, double g2) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= np) return;
+    k_lag_interp(const R* __restrict__ c, double* __restrict__ pp, int np, int n, double g0, double g1, double g2,
+                 const int* __restrict__ order) {
+  const int per = gridDim.x / 8;
+  const int t = ((int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8)) * (int)blockDim.x + (int)threadIdx.x;
+  if (t >= np) return;
+  const int m = order ? order[t] : t;
   const size_t n3 = (size_t)n * n * n;
   const double gs[3] = {g0, g1, g2};
   const double d6 = 1.0 / 6.0, tt = 2.0 / 3.0;
@@ -2059,19 +2117,9 @@ __global__ void __launch_bounds__(256)
   int ib[3];
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    const double px = pp[(size_t)j * np + m] - (gs[j] + 1.0) - (double)n;
-    const double fl = fabs(px) < 1e15 ? floor(px) : 0.0;
-    long long jpx = (long long)fl;
-    if (jpx < 0) jpx += 1000LL * n;
-    long long jloc = jpx % n;
-    if (jloc < 0) jloc += n;
-    double bfp = (double)jloc + px - fl;
-    // the reference stops on a position it cannot place (spcal_local.f:
-    // 186-190); here such a lane (non-finite or beyond 1000 periods) takes
-    // node 0 so that every index stays inside the coefficient arrays
-    if (!(bfp >= 0.0 && bfp < (double)n)) bfp = 0.0;
-    ib[j] = (int)bfp;
-    const double pos = bfp - ib[j], ym = 1.0 - pos, yms = ym * ym;
+    double pos;
+    ib[j] = lag_cell(pp[(size_t)j * np + m], gs[j], n, pos);
+    const double ym = 1.0 - pos, yms = ym * ym;
     bff[0][j] = ym * yms * d6;
     bff[1][j] = tt + 0.5 * pos * (yms - 1.0);
     bff[3][j] = pos * pos * pos * d6;
@@ -2087,6 +2135,14 @@ __global__ void __launch_bounds__(256)
   double func[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
+    // the 4 x nodes of each (v, y) row of plane zs[k]: one 16-byte (FP32) or
+    // two (FP64) loads of consecutive x, element by element where the
+    // stencil wraps around x = n
+    R row[3][4][4];
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) lag_row(c + v * n3 + ((size_t)ys[jj] * n + zs[k]) * n, xs, n, row[v][jj]);
     double sw[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -2095,7 +2151,7 @@ __global__ void __launch_bounds__(256)
       for (int v = 0; v < 3; v++) {
         double bt[4];
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++) bt[jj] = (double)c[v * n3 + ((size_t)ys[jj] * n + zs[k]) * n + xs[i]];
+        for (int jj = 0; jj < 4; jj++) bt[jj] = (double)row[v][jj][i];
         const double term1 = bt[0] * bff[0][1] + bt[2] * bff[2][1];
         const double term2 = bt[1] * bff[1][1] + bt[3] * bff[3][1];
         sw[v] += (term1 + term2) * cst;
@@ -2103,6 +2159,8 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll
     for (int v = 0; v < 3; v++) func[v] += sw[v] * bff[k][2];
+    // one plane's rows in registers at a time
+    issue_fence();
   }
 #pragma unroll
   for (int v = 0; v < 3; v++) pp[(size_t)(3 + v) * np + m] = func[v];
@@ -2672,23 +2730,28 @@ void launch_epfor(const Ctx& c, int mode, const int4* list, int nlist, const dou
                        efkz);
   }
 }
-void launch_spline_prep(const Ctx& c, void* out) {
-  const size_t tot = (size_t)c.geo.na * c.geo.n;
+void launch_spline_shifts(const Ctx& c, void* sr) {
+  const int tot = c.geo.nxh + 2 * c.geo.n;
   if (c.prec == 8)
-    hipLaunchKernelGGL((k_spline_prep<double>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
-                       (const C<double>*)c.un, (C<double>*)out, c.geo, c.sx[0], c.sy[0], c.sz[0]);
+    hipLaunchKernelGGL((k_spline_shifts<double>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.sx[0], c.sy[0],
+                       c.sz[0], (C<double>*)sr, c.geo.n, c.geo.nxh);
   else
-    hipLaunchKernelGGL((k_spline_prep<float>), dim3(grid1d(tot, 256 * 4)), dim3(256), 0, c.stream,
-                       (const C<float>*)c.un, (C<float>*)out, c.geo, c.sx[0], c.sy[0], c.sz[0]);
+    hipLaunchKernelGGL((k_spline_shifts<float>), dim3((tot + 255) / 256), dim3(256), 0, c.stream, c.sx[0], c.sy[0],
+                       c.sz[0], (C<float>*)sr, c.geo.n, c.geo.nxh);
 }
-void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3]) {
-  const dim3 grid((np + 255) / 256), block(256);
+void launch_lag_key(const Ctx& c, const double* pp, int np, int lb, const double gsh1[3], unsigned* key, int* idx) {
+  hipLaunchKernelGGL(k_lag_key, dim3((np + 255) / 256), dim3(256), 0, c.stream, pp, np, c.geo.n, lb, gsh1[0],
+                     gsh1[1], gsh1[2], key, idx);
+}
+void launch_lag_interp(const Ctx& c, double* pp, int np, const double gsh1[3], const int* order) {
+  // a multiple of 8 work-groups (one contiguous range of the order per XCD)
+  const dim3 grid(((np + 255) / 256 + 7) / 8 * 8), block(256);
   if (c.prec == 8)
     hipLaunchKernelGGL((k_lag_interp<double>), grid, block, 0, c.stream, (const double*)c.ux, pp, np, c.geo.n,
-                       gsh1[0], gsh1[1], gsh1[2]);
+                       gsh1[0], gsh1[1], gsh1[2], order);
   else
     hipLaunchKernelGGL((k_lag_interp<float>), grid, block, 0, c.stream, (const float*)c.ux, pp, np, c.geo.n,
-                       gsh1[0], gsh1[1], gsh1[2]);
+                       gsh1[0], gsh1[1], gsh1[2], order);
 }
 void launch_lag_advpos(const Ctx& c, double* pp, int np, int pstep, double fact1, double fact2) {
   hipLaunchKernelGGL(k_lag_advpos, dim3((np + 255) / 256), dim3(256), 0, c.stream, pp, np, pstep, fact1, fact2);
