@@ -2059,14 +2059,11 @@ __device__ __forceinline__ int lag_cell(double p, double g, int n, double& pos) 
 // c[x0 .. x0 + 3] of one coefficient row as one vector load (4-byte or
 // 8-byte aligned, consecutive x); xs[i] = (x0 + i) mod n element by element
 // where the 4 nodes wrap around x = n
-#ifndef PSDNS_LAG_VEC
-#define PSDNS_LAG_VEC 1
-#endif
 template <typename R>
 __device__ __forceinline__ void lag_row(const R* __restrict__ row, const int xs[4], int n, R out[4]) {
   typedef R v4 __attribute__((ext_vector_type(4)));
   typedef v4 v4u __attribute__((aligned(sizeof(R))));
-  if (PSDNS_LAG_VEC && xs[0] <= n - 4) {
+  if (xs[0] <= n - 4) {
     const v4 q = *reinterpret_cast<const v4u*>(row + xs[0]);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = q[i];
