@@ -173,7 +173,9 @@ int psdns_rk4_forcing(psdns_ctx *c, const void *forcing);
  * predicted positions; rk_advpos(2, 0, 1/2) (the first step only
  * interpolates, partic(4)).
  * lag_put: np particles, the first ncols (3, 6 or 9) columns given, the
- *   rest zero; lag_get: all 9 columns.
+ *   rest zero (device memory: pp, 72 bytes per particle, plus the
+ *   interpolation's sort buffers, 16 bytes per particle and the radix
+ *   sort's temporary storage); lag_get: all 9 columns.
  * lag_spline (spxyz_m.f with splx/splz/sply, solsp1.f): the tricubic
  *   spline coefficients of the velocity of un on the phase-shifted grid of
  *   stage 1 of the last set_shifts: the periodic (1,4,1) systems along x,
@@ -182,7 +184,10 @@ int psdns_rk4_forcing(psdns_ctx *c, const void *forcing);
  *   to the grid; kept in ux(1:3) (allocated on first use).  Any call that
  *   writes the work arrays afterwards discards them.
  * lag_interp (spcal_local.f:119-364): pp(:,4:6) = the spline velocity at
- *   pp(:,1:3), B-spline weights of nodes ib-1 .. ib+2 (periodic), FP64 sums.
+ *   pp(:,1:3), B-spline weights of nodes ib-1 .. ib+2 (periodic), FP64 sums
+ *   in the reference's order.  The particles are processed in the order of
+ *   their 4 x 4 x 4-cell bin (a GPU sort each call); the order does not
+ *   change any particle's result.
  * lag_advpos (rk_advpos.f:15-52), fact = cof dt n / (2 pi): pstep 1
  *   pp(:,7:9) = x + fact1 u, x = pp(:,7:9) + fact2 u; pstep 2 x = pp(:,7:9)
  *   + fact2 u. */
