@@ -194,37 +194,60 @@ def run(args):
     les_n.put_state(u0)
     les_m_run = P.Les(les_m, nu, math.pi, closure=m, bf16=True, kf_shell=3, ekinf=None, fused=True)
     les_n_run = P.Les(les_n, nu, math.pi, closure=None, kf_shell=3, ekinf=les_m_run.ekinf, fused=True)
+    runs = {"model": (les_m_run, les_m), "none": (les_n_run, les_n)}
+    # further output gains: the closure with its linear output layer (the
+    # tensor-basis coefficients) scaled by c, so that the stress is c times
+    # the trained prediction; each in its own LES beside the same DNS
+    gain_closures = []
+    nw = sum(int(np.prod(sh)) for sh in shapes[:-2])
+    for c in args.gains[1:]:
+        mg = P.Closure(nl)
+        th = theta.copy()
+        th[nw:] *= c
+        mg.set_params(th)
+        sg = P.Solver(nl, 4, viscos=nu)
+        sg.put_state(u0)
+        runs[f"gain_{c:g}"] = (P.Les(sg, nu, math.pi, closure=mg, bf16=True, kf_shell=3, ekinf=les_m_run.ekinf,
+                                     fused=True), sg)
+        gain_closures.append(mg)
+    if args.gains[0] != 1.0:
+        th = theta.copy()
+        th[nw:] *= args.gains[0]
+        m.set_params(th)
     t0 = t
-    t_les = {"model": t0, "none": t0}
-    wall = {"model": 0.0, "none": 0.0}
-    sims = {"model": 0.0, "none": 0.0}
-    sh_m, sh_n = Shifts(args.seed + 7), Shifts(args.seed + 7)
-    acc = {"dns": [], "model": [], "none": []}
+    t_les = {k: t0 for k in runs}
+    wall = {k: 0.0 for k in runs}
+    sims = {k: 0.0 for k in runs}
+    shf = {k: Shifts(args.seed + 7) for k in runs}
+    acc = {"dns": []}
+    acc.update({k: [] for k in runs})
     next_sample = t0 + (args.eval - 1.0) * te
     end = t0 + args.eval * te
     while t < end:
         dns_step()
-        for key, rr, s_, shf in (("model", les_m_run, les_m, sh_m), ("none", les_n_run, les_n, sh_n)):
+        for key, (rr, s_) in runs.items():
             while t_les[key] < t:
                 a = time.perf_counter()
-                rr.step(shf())
+                rr.step(shf[key]())
                 wall[key] += time.perf_counter() - a
                 sims[key] += rr.dt
                 t_les[key] += rr.dt
         if t >= next_sample:
             acc["dns"].append(spectrum_of(spare, filtered_state(P, m, dns, les_a), kc))
-            acc["model"].append(les_m.shell_stats(kc)[0])
-            acc["none"].append(les_n.shell_stats(kc)[0])
+            for key, (rr, s_) in runs.items():
+                acc[key].append(s_.shell_stats(kc)[0])
             next_sample += 0.1 * te
     ed = np.mean(acc["dns"], axis=0)
     res = {}
-    for key in ("model", "none"):
+    for key, (rr, s_) in runs.items():
         el = np.mean(acc[key], axis=0)
         err = np.abs(el - ed) / np.where(ed > 0, ed, 1.0)  # shell 1 (k = 0) holds no energy
         res[key] = {"max_rel_shell_error": float(err[1:].max()), "shell_of_max": int(np.argmax(err[1:]) + 2),
                     "rel_error_by_shell": [round(float(e), 4) for e in err],
-                    "sim_time_per_wall_s": sims[key] / wall[key], "ms_per_step": 1e3 * wall[key] / len(
-                        (les_m_run if key == "model" else les_n_run).dts)}
+                    "sim_time_per_wall_s": sims[key] / wall[key], "ms_per_step": 1e3 * wall[key] / len(rr.dts)}
+        if key.startswith("gain_"):
+            res[key]["spectrum"] = [float(v) for v in el]
+    out["output_gain"] = args.gains[0]
     dns_rate = dns_sim / dns_wall
     out["dns"] = {"ms_per_step": 1e3 * dns_wall / len(run_d.dts), "sim_time_per_wall_s": dns_rate,
                   "mean_dt": dns_sim / len(run_d.dts)}
@@ -236,8 +259,10 @@ def run(args):
     out["speedup_vs_dns"] = res["model"]["sim_time_per_wall_s"] / dns_rate
     out["accepted"] = res["model"]["max_rel_shell_error"] <= 0.05
     out["result"] = "LES with the learned closure" if out["accepted"] else "falls back to the DNS (spectral error > 5%)"
-    for s in (dns, les_a, les_b, spare, les_m, les_n):
+    for s in [dns, les_a, les_b, spare] + [s_ for rr, s_ in runs.values()]:
         s.close()
+    for mg in gain_closures:
+        mg.close()
     m.close()
     mb.close()
     return out
@@ -255,6 +280,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=122929298)
+    ap.add_argument("--gains", type=lambda v: [float(x) for x in v.split(",")], default=[1.0],
+                    help="output gains c of the closure (stress = c x prediction): the first is the reported "
+                         "model, each further one runs its own LES beside the same DNS")
     args = ap.parse_args()
     print(json.dumps(run(args)), flush=True)
 
