@@ -192,8 +192,8 @@ def run(args):
     les_n = P.Solver(nl, 4, viscos=nu)
     les_m.put_state(u0)
     les_n.put_state(u0)
-    les_m_run = P.Les(les_m, nu, math.pi, closure=m, bf16=True, kf_shell=3, ekinf=None, fused=True)
-    les_n_run = P.Les(les_n, nu, math.pi, closure=None, kf_shell=3, ekinf=les_m_run.ekinf, fused=True)
+    les_m_run = P.Les(les_m, nu, args.les_cfl, closure=m, bf16=True, kf_shell=3, ekinf=None, fused=True)
+    les_n_run = P.Les(les_n, nu, args.les_cfl, closure=None, kf_shell=3, ekinf=les_m_run.ekinf, fused=True)
     runs = {"model": (les_m_run, les_m), "none": (les_n_run, les_n)}
     # further output gains: the closure with its linear output layer (the
     # tensor-basis coefficients) scaled by c, so that the stress is c times
@@ -207,7 +207,7 @@ def run(args):
         mg.set_params(th)
         sg = P.Solver(nl, 4, viscos=nu)
         sg.put_state(u0)
-        runs[f"gain_{c:g}"] = (P.Les(sg, nu, math.pi, closure=mg, bf16=True, kf_shell=3, ekinf=les_m_run.ekinf,
+        runs[f"gain_{c:g}"] = (P.Les(sg, nu, args.les_cfl, closure=mg, bf16=True, kf_shell=3, ekinf=les_m_run.ekinf,
                                      fused=True), sg)
         gain_closures.append(mg)
     if args.gains[0] != 1.0:
@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=122929298)
+    ap.add_argument("--les-cfl", type=float, default=math.pi,
+                    help="CFL number of the LES steps (the DNS keeps pi)")
     ap.add_argument("--gains", type=lambda v: [float(x) for x in v.split(",")], default=[1.0],
                     help="output gains c of the closure (stress = c x prediction): the first is the reported "
                          "model, each further one runs its own LES beside the same DNS")
