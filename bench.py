@@ -61,8 +61,10 @@ def pass_bytes(n, s, num_al, max_al_x, nc=0):
                   proc4b is not stored): per launch on average
                                                                     (4 + 3 + (6 + 3) / 2) * num_al * n * 2s
     and per passive scalar:
-      gradient y  1 cylinder in, 3 transformed derivatives out      4 * num_al * n * 2s
-      z-inverse   3 derivatives, as the velocity's z-inverse         3 * (num_al * n + n * max_al_x * n) * 2s
+      gradient y  1 cylinder in, the scalar and its y derivative
+                  y-transformed out                                 3 * num_al * n * 2s
+      z-inverse   those 2 cylinder slots in, the 3 derivatives out
+                  (i kx and i kz applied in the pass)               (2 * num_al * n + 3 * n * max_al_x * n) * 2s
       x pass      inside the velocity's x fused pass                0
       z-forward   1 line set in, 1 cylinder out                      (n * max_al_x * n + num_al * n) * 2s
       y-forward   inside the update                                 0
@@ -76,8 +78,8 @@ def pass_bytes(n, s, num_al, max_al_x, nc=0):
         "x_fused": (8 + 4 * nc) * n * n * max_al_x * c,
         "z_forward": (5 * n * max_al_x * n + 4 * num_al * n) * c,
         "y_forward_update": (13 + 10) * num_al * n * c // 2,
-        "sc_grad_y": 4 * num_al * n * c,
-        "sc_z_inverse": 3 * (num_al * n + n * max_al_x * n) * c,
+        "sc_grad_y": 3 * num_al * n * c,
+        "sc_z_inverse": (2 * num_al * n + 3 * n * max_al_x * n) * c,
         "sc_x_pass": 0,
         "sc_z_forward": (n * max_al_x * n + num_al * n) * c,
         "sc_y_forward": 0,

@@ -334,7 +334,7 @@ int scalar_gradients(psdns_ctx* c) {
     if ((rc = mark_sc(c, 6 * i))) return rc;
     bad |= psdns::launch_y_grad(k, (char*)c->d_usc + i * vb, c->d_xs);
     if ((rc = mark_sc(c, 6 * i + 1))) return rc;
-    bad |= psdns::launch_pass(kd, Pass::ZInv, 3, 0);
+    bad |= psdns::launch_pass(kd, Pass::ZInv, 3, 2);
     if ((rc = mark_sc(c, 6 * i + 2))) return rc;
     if (bad) return fail(PSDNS_EINVAL, "unsupported grid n = %d", c->n);
   }

@@ -79,7 +79,7 @@ enum class Pass {
   YInvPhase,  // phase1 + y c2c +i, u -> yw (arg: number of variables)
   YInvKx,     // kxtran: zero y Nyquist + y c2c +i, u -> yw
   YInvPlain,  // y c2c +i, u -> yw
-  ZInv,       // gather yw + z c2c +i -> xi (args: number of variables, yw in u)
+  ZInv,       // gather yw + z c2c +i -> xi (args: number of variables; source 0 W1, 1 y-blocked yw in u, 2 a scalar's two gradient slots in W1 -> three xi slots)
   XC2R,       // gather xi + x c2r -> physical (arg: number of variables)
   XR2C,       // physical -> x r2c / n (arg: number of variables)
   XFused,     // c2r u,v,w (xi) -> products (+velmax, arg) -> r2c / n of 5 (X)

@@ -332,9 +332,10 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 // ---------------------------------------------------------------------------
 // Shifted gradient of a passive scalar, y-transformed (phase1.f:44-64,
 // itransform.f:190-203): phi_s = sx sy sz phi on the mask (0 elsewhere);
-// yw slot 0 = i kx Y[phi_s], slot 1 = Y[i ky phi_s], slot 2 = i kz Y[phi_s]
-// with Y the y c2c +i of a column (kx and kz are constant along it, so two
-// transforms give the three derivatives).  Tiles of B columns, natural
+// yw slot 0 = Y[phi_s], slot 1 = Y[i ky phi_s] with Y the y c2c +i of a
+// column.  The x and z derivatives multiply slot 0 by i kx and i kz, which
+// are constant along a column: the inverse z pass applies them (k_z_inv
+// DER), so two slots carry the three derivatives.  Tiles of B columns, natural
 // mapping in and out, yw in the cylinder layout; persistent over tiles as
 // k_y_inv (the next tile's loads in flight during the current transforms;
 // off-mask entries are not read, so they enter as zeros).
@@ -389,15 +390,13 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
     }
     F::template run<false, false, false, false>(b.v, lds, io, io, stw);
     F::template run<false, false, false, false>(b2, lds, io, io, stw);
-    const BufView<R> d0(yw + (size_t)a0 * N, kOob), d1(yw + tot + (size_t)a0 * N, kOob),
-        d2(yw + 2 * tot + (size_t)a0 * N, kOob);
+    const BufView<R> d0(yw + (size_t)a0 * N, kOob), d1(yw + tot + (size_t)a0 * N, kOob);
     const bool act = a0 + li < g.na;
 #pragma unroll
     for (int m = 0; m < E; m++) {
       const unsigned o = act ? ((unsigned)li * N + q + T * m) * (unsigned)sizeof(C<R>) : kOob;
-      d0.store(o, C<R>{-(R)x * b.v[m].y, (R)x * b.v[m].x});
+      d0.store(o, b.v[m]);
       d1.store(o, b2[m]);
-      d2.store(o, C<R>{-(R)kz * b.v[m].y, (R)kz * b.v[m].x});
     }
   };
   Regs<R, E> ba, bb;
@@ -413,11 +412,17 @@ __global__ void __launch_bounds__((YInv<R>::B * Team<N>::T))
 // (which always holds the z Nyquist) read as zero (kxcomm1_clean.f:238-240);
 // z c2c +i; the last stage leaves the lines in the natural mapping for
 // contiguous stores along z.
+// DER (a passive scalar's gradient, from k_y_grad's two slots Y[phi],
+// Y[i ky phi]): three outputs per kx and y block, taken in turn (v fastest,
+// so that the second read of Y[phi] meets the cache): xi slot 0 = i kx
+// Z[Y[phi]] (kx constant per tile, applied after the transform), slot 1 =
+// Z[Y[i ky phi]], slot 2 = Z[i kz Y[phi]] (applied before it).
 // ---------------------------------------------------------------------------
-template <typename R, int N, bool YB>
+template <typename R, int N, bool YB, bool DER = false>
 __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((amdgpu_waves_per_eu(4)))
     k_z_inv(const C<R>* __restrict__ yw, C<R>* __restrict__ xi, Geo g, const C<R>* __restrict__ tw, int nv,
             const int* __restrict__ cutz, const int* __restrict__ col0) {
+  static_assert(!(DER && YB), "the gradient slots are in the cylinder layout");
   constexpr int T = Team<N>::T, E = Team<N>::E, B = RunLines<R>::B;
   constexpr int NYB = N / B;  // y blocks
   using F = TeamFft<R, N, E, B, +1, LineShift<E, B>::IN>;
@@ -429,10 +434,25 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
   const Slot<R> out = F::natural(lds, tid);
   const int lt0 = F::transposed_line(tid), ln0 = F::natural_line(tid);
   __syncthreads();
-  // tile t = (var * max_al_x + kx) * NYB + y block
+  // tile t = (var * max_al_x + kx) * NYB + y block; DER: t = (kx * NYB +
+  // y block) * 3 + output, the input slot 1 for output 1, else 0
+  auto tile = [&](int t, int& var, int& kx, int& y0) {
+    if constexpr (DER) {
+      var = t % 3;
+      const int r = t / 3;
+      kx = r / NYB;
+      y0 = (r % NYB) * B;
+    } else {
+      const int vk = t / NYB;
+      y0 = (t % NYB) * B;
+      var = vk / g.max_al_x;
+      kx = vk % g.max_al_x;
+    }
+  };
   auto load = [&](Regs<R, E>& b, int t, bool valid) {
-    const int vk = t / NYB, y0 = (t % NYB) * B;
-    const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
+    int var, kx, y0;
+    tile(t, var, kx, y0);
+    if constexpr (DER) var = var == 1 ? 1 : 0;
     const int c = cutz[kx];
     // YB: the kx's columns in y block y0 / B, one contiguous run; else
     // 64-byte runs along y of the kx's columns
@@ -448,9 +468,27 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
     }
   };
   auto finish = [&](Regs<R, E>& b, int t) {
+    int var, kx, y0;
+    tile(t, var, kx, y0);
+    if constexpr (DER) {
+      if (var == 2) {
+        // i kz at line point zz (the FFT index of kz), transposed mapping
+        const int qi = opaque(in.q);
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+          const R kz = (R)kwave(qi + T * j, N);
+          b.v[j] = {-kz * b.v[j].y, kz * b.v[j].x};
+        }
+      }
+    }
     F::template run<false, false, true, false>(b.v, lds, in, out, stw);
-    const int vk = t / NYB, y0 = (t % NYB) * B;
-    const int var = vk / g.max_al_x, kx = vk % g.max_al_x;
+    if constexpr (DER) {
+      if (var == 0) {
+        const R rkx = (R)kx;
+#pragma unroll
+        for (int m = 0; m < E; m++) b.v[m] = {-rkx * b.v[m].y, rkx * b.v[m].x};
+      }
+    }
     const int l = opaque(ln0), q = opaque(out.q);
     const BufView<R> dst(xi + xiidx(var, y0, kx, 0, g), kOob);
 #pragma unroll
@@ -458,7 +496,7 @@ __global__ void __launch_bounds__((RunLines<R>::B * Team<N>::T)) __attribute__((
       dst.store(((unsigned)l * g.max_al_x * N + q + T * m) * (unsigned)sizeof(C<R>), b.v[m]);
   };
   Regs<R, E> ba, bb;
-  tile_loop(nv * g.max_al_x * NYB, ba, bb, load, finish);
+  tile_loop((DER ? 3 : nv) * g.max_al_x * NYB, ba, bb, load, finish);
 }
 
 // ---------------------------------------------------------------------------
@@ -1873,7 +1911,9 @@ __global__ void __launch_bounds__(256) k_shifts(double2* __restrict__ s64, C<R>*
 // ---------------------------------------------------------------------------
 // Passive scalars (per scalar, beside the velocity stage):
 //  k_y_grad     (with the y passes) phase1.f:44-64 / itransform.f:190-203:
-//               the shifted gradient, y-transformed
+//               the shifted scalar and its y derivative, y-transformed
+//  k_z_inv<..., DER> the three derivatives, z-transformed (i kx and i kz
+//               applied in the z pass)
 //  k_x_fused<..., SC> (with the x passes) itransform.f:522-531: the
 //               advection term u (G1 + phi_x) + v (G2 + phi_y) + w (G3 + phi_z)
 //               of every scalar, beside the velocity's products
@@ -2329,9 +2369,16 @@ struct Launch {
   }
   // yw from the W1 allocation (from_u = 0, column-major cylinder) or
   // y-blocked from the u slots 0..2 (the fused stage-1 update's output)
+  // from_u: 0 yw in W1, 1 y-blocked yw in u, 2 a scalar's gradient (two
+  // slots in W1 -> three xi slots; nv is not used)
   static void z_inv(const Ctx& c, int nv, int from_u) {
     constexpr int B = RunLines<R>::B, T = Team<N>::T;
-    if (from_u) {
+    if (from_u == 2) {
+      const auto kern = k_z_inv<R, N, false, true>;
+      static const unsigned grid = persistent_grid((const void*)kern, B * T);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.w1, (CR*)c.xi, c.geo,
+                         (const CR*)c.tw, 3, c.geo.cutz, c.geo.col0);
+    } else if (from_u) {
       const auto kern = k_z_inv<R, N, true>;
       static const unsigned grid = persistent_grid((const void*)kern, B * T);
       hipLaunchKernelGGL(kern, dim3(grid), dim3(B * T), 0, c.stream, (const CR*)c.u, (CR*)c.xi, c.geo,
