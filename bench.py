@@ -20,7 +20,7 @@ against the 512^3 DNS it stands for (--no-les skips it), and
 same step without them (--no-particles skips it).
 
 The CPU baseline times the FP64 oracle (a port of the reference algorithm)
-on a bounded sample (RK2 steps at 256^3 for about 10-20 s, all host cores)
+on a bounded sample (RK2 steps at 512^3 for about 15-25 s, all host cores)
 and scales it to n^3 by n^3 log2 n.
 """
 import argparse
@@ -300,12 +300,16 @@ def fft_summary(r, names):
     return timed, fft_bytes, fft_ms
 
 
-def cpu_baseline(n_target, budget_s=15.0):
+def cpu_baseline(n_target, budget_s=15.0, n=512):
+    """The FP64 oracle's RK2 step on the host cores at n^3 (512^3: about
+    4.5 s per step on 16 threads), timed for about budget_s and scaled to
+    n_target^3 by n^3 log2 n (x8.9 from 512^3 to 1024^3; measured from 256^3
+    the same scaling overestimates the 512^3 step by 15%)."""
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle as O  # the checker; timed here only as the reported CPU baseline
 
     O.build()
-    n = 256
+    n = min(n, n_target)
     g = O.Grid(n)
     u0 = g.field()
     r = O.Run(g, u0, 0.01, cfl=math.pi, kshift=1, kf_shell=3)
