@@ -1770,11 +1770,13 @@ __global__ void __launch_bounds__(64 * kShellWaves)
       // half 0: y = 0 .. n/2 - 1; half 1: y = n/2 + 1 .. n - 1
       const int ylo = half == 0 ? 0 : n / 2 + 1, yhi = half == 0 ? n / 2 : n;
       for (int y0 = ylo; y0 < yhi; y0 += 64 * kShellChunks) {
+        // off-mask points (about a third of the cylinder) are not read
         C<R> v[kShellChunks][3];
 #pragma unroll
         for (int c = 0; c < kShellChunks; c++) {
           const int y = y0 + 64 * c + lane;
-          const unsigned off = y < yhi ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob;
+          const bool on = y < yhi && in_mask(x, kwave(y, n), kz, y, z, n);
+          const unsigned off = on ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob;
           v[c][0] = s0.load(off);
           v[c][1] = s1.load(off);
           v[c][2] = s2.load(off);
