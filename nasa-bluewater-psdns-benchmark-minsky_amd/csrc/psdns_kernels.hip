@@ -1736,16 +1736,42 @@ __global__ void __launch_bounds__(256)
 // D(ik) = 2 nu k^2 E over the mask, ik = int(|k| + 1.5), FP64, with the same
 // bits on every run.  Work-group b owns the fixed column range
 // [b na / B, (b + 1) na / B); each wave takes one column at a time and walks
-// its two y halves (ky >= 0 ascending, then ky < 0 with |ky| descending), in
-// which the shell index is monotone, so the lanes of one shell are
-// contiguous: a segmented scan over the 64 lanes sums each shell's run in a
-// fixed tree, and only the run's last lane adds it to the wave's own LDS
-// histogram (one writer per shell per instruction, in program order).  The
+// its two y halves (ky >= 0, then ky < 0) in chunks of 64 lanes ordered by
+// |ky| descending, in which the shell index is monotone, so the lanes of one
+// shell are contiguous: the wave's inclusive prefix sums (DPP moves, a fixed
+// tree) give each shell's run as the difference of the sums at its last
+// lane and before its first, and only the run's last lane adds it to the
+// wave's own LDS histogram (one writer per shell per instruction, in program
+// order).  The descending order makes the prefix sums run from the small
+// high-wavenumber energies to the larger ones, so the differences lose
+// little to cancellation.  The
 // work-group then adds its wave histograms in wave order into its partial
 // row part[b][0..nh) (E) and part[b][nh..2 nh) (D); k_shell_fold_rows and
 // k_shell_fold sum the rows in two fixed levels.  No atomics: the result
 // depends on B only.
 // ---------------------------------------------------------------------------
+// Inclusive sum over the 64 lanes of a wave with DPP moves only: row shifts
+// by 1, 2, 4 and 8 within each row of 16 lanes, then the total of row 0
+// (lane 15) into row 1 and of row 2 (lane 47) into row 3, then that of rows
+// 0-1 (lane 31) into rows 2 and 3.  Lanes without a source take 0.  Every
+// lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_prefix_sum(double x) {
+  x += dpp_move<0x111, 0xF>(x);  // row_shr:1
+  x += dpp_move<0x112, 0xF>(x);  // row_shr:2
+  x += dpp_move<0x114, 0xF>(x);  // row_shr:4
+  x += dpp_move<0x118, 0xF>(x);  // row_shr:8
+  x += dpp_move<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_move<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
 constexpr int kShellWaves = 8;      // waves per work-group of k_shell_partial
 constexpr int kShellChunks = 8;     // 64-entry chunks of a column half per pass
 
@@ -1769,12 +1795,14 @@ __global__ void __launch_bounds__(64 * kShellWaves)
     for (int half = 0; half < 2; half++) {
       // half 0: y = 0 .. n/2 - 1; half 1: y = n/2 + 1 .. n - 1
       const int ylo = half == 0 ? 0 : n / 2 + 1, yhi = half == 0 ? n / 2 : n;
+      // the lanes of a chunk in |ky| descending order (reversed in half 0)
+      const int yl = half == 0 ? 63 - lane : lane;
       for (int y0 = ylo; y0 < yhi; y0 += 64 * kShellChunks) {
         // off-mask points (about a third of the cylinder) are not read
         C<R> v[kShellChunks][3];
 #pragma unroll
         for (int c = 0; c < kShellChunks; c++) {
-          const int y = y0 + 64 * c + lane;
+          const int y = y0 + 64 * c + yl;
           const bool on = y < yhi && in_mask(x, kwave(y, n), kz, y, z, n);
           const unsigned off = on ? (unsigned)y * (unsigned)sizeof(C<R>) : kOob;
           v[c][0] = s0.load(off);
@@ -1784,7 +1812,7 @@ __global__ void __launch_bounds__(64 * kShellWaves)
 #pragma unroll
         for (int c = 0; c < kShellChunks; c++) {
           if (y0 + 64 * c >= yhi) break;  // wave-uniform
-          const int y = y0 + 64 * c + lane, ky = kwave(y, n);
+          const int y = y0 + 64 * c + yl, ky = kwave(y, n);
           const bool on = y < yhi && in_mask(x, ky, kz, y, z, n);
           const int k2 = x * x + ky * ky + kz * kz;
           const double rk2 = (double)k2;
@@ -1800,21 +1828,19 @@ __global__ void __launch_bounds__(64 * kShellWaves)
             for (int i = 0; i < 3; i++) sum += (double)v[c][i].x * v[c][i].x + (double)v[c][i].y * v[c][i].y;
             e = 0.5 * tf * sum;
           }
-          double d = 2.0 * viscos * rk2 * e;
-          // segmented inclusive scan: runs of equal keys are contiguous
-#pragma unroll
-          for (int o = 1; o < 64; o <<= 1) {
-            const int ku = __shfl_up(key, o);
-            const double eu = __shfl_up(e, o), du = __shfl_up(d, o);
-            if (lane >= o && ku == key) {
-              e += eu;
-              d += du;
-            }
-          }
-          const int kn = __shfl_down(key, 1);
-          if (key > 0 && (lane == 63 || kn != key)) {
-            he[key - 1] += e;
-            hd[key - 1] += d;
+          const double d = 2.0 * viscos * rk2 * e;
+          // runs of equal keys are contiguous: each run's sum is the
+          // difference of the wave's inclusive prefix sums at its last lane
+          // and before its first (run starts from a ballot of key changes)
+          const double pe = wave_prefix_sum(e), pd = wave_prefix_sum(d);
+          const int kp = __shfl_up(key, 1);
+          const unsigned long long starts = __ballot(lane == 0 || kp != key);
+          const int first = 63 - __clzll(starts & ((2ull << lane) - 1ull));
+          const double pe0 = __shfl(pe, first > 0 ? first - 1 : 0), pd0 = __shfl(pd, first > 0 ? first - 1 : 0);
+          const bool last = lane == 63 || (((starts >> lane) >> 1) & 1ull);
+          if (key > 0 && last) {
+            he[key - 1] += first > 0 ? pe - pe0 : pe;
+            hd[key - 1] += first > 0 ? pd - pd0 : pd;
           }
         }
       }
