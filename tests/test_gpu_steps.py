@@ -305,7 +305,7 @@ def test_fused_step_reproducible(P, O, n, prec):
     s.close()
 
 
-@pytest.mark.parametrize("n,prec", [(32, 8), (256, 8), (256, 4)])
+@pytest.mark.parametrize("n,prec", [(32, 8), (64, 8), (128, 8), (256, 8), (64, 4), (256, 4)])
 def test_forced_step_reproducible(P, O, n, prec):
     """Two forced steps (FEK shells k <= 2) from the same state give the
     same bits: the shell energies the rescale uses are fixed-order FP64
